@@ -1,0 +1,341 @@
+/*
+ * ufc_hip.h -- C-ABI of the MI355X (gfx950) static-stiffness library.
+ *
+ * Drop-in for UFC's hot path: C3D8 element stiffness (L4_PH), fixed-pattern
+ * CSR assembly with penalty BCs and face loads (L5_RT), CSR SpMV and
+ * Jacobi-PCG in FP64, FP32 and FP32-in-FP64-refinement form plus the adjoint
+ * solve and element sensitivities (L2_NM / new), and the J2 material update
+ * (L4_PH Material).  Each entry point takes UFC's call-bundle shape
+ * (desc, state, algo, ctx, args) as C structs of plain pointers and sizes;
+ * there are no torch or HIP types in any signature.
+ *
+ * Citations are to the reference (zning8251-jpg/ufc-fem-kernel), paths
+ * relative to its root, "file:line" of the interface each entry replaces.
+ *
+ * Conventions
+ *   - Array arguments are plain pointers to host memory or to device memory
+ *     of the context's GPU (both are accepted; copies use unified addressing).
+ *     Mesh arrays of ufc_mesh_desc must be host memory.
+ *   - Index arrays carry an explicit index_base: 1 for Fortran callers, 0 for C.
+ *   - Equations are numbered eq = 3(node-1)+d, d = 1..3, 1-based
+ *     (RT_Asm_DofMap.f90:199-205); every node has 3 equations, clamped ones
+ *     included (Dirichlet conditions are diagonal penalties).
+ *   - Voigt order [xx, yy, zz, xy, yz, xz] with engineering shear strains
+ *     (PH_Elem_C3D8.f90:550-573).
+ *   - Every call returns ufc_status.status_code (also the return value); it
+ *     never aborts.  Calls on one context are not re-entrant; calls are
+ *     blocking (results are complete when the call returns).
+ */
+#ifndef UFC_HIP_H
+#define UFC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: IF_STATUS_* of L1_IF/Error/IF_Err_Brg.f90:95-109 ---- */
+#define UFC_STATUS_OK 0
+#define UFC_STATUS_INVALID 1
+#define UFC_STATUS_ERROR 2
+#define UFC_STATUS_CONVERGED 3
+#define UFC_STATUS_NOT_CONVERGED 4
+#define UFC_STATUS_WARN 6
+#define UFC_STATUS_MEM_ERROR 7
+#define UFC_STATUS_FATAL 10
+#define UFC_STATUS_UNSUPPORTED 11
+
+/* ---- precision selectors ---- */
+#define UFC_PREC_FP64 0  /* FP64 element integration, values, vectors        */
+#define UFC_PREC_FP32 1  /* FP32 element integration, values, vectors        */
+#define UFC_PREC_MIXED 2 /* FP32 values + inner PCG, FP64 outer refinement   */
+
+/* ---- material selectors for assembly ---- */
+#define UFC_MAT_ELASTIC 0 /* isotropic, E per element, one nu               */
+#define UFC_MAT_J2 1      /* per-Gauss-point tangent from the J2 state      */
+
+/* ---- face load kinds ---- */
+#define UFC_LOAD_TRACTION 0 /* traction vector x area/4 per face node       */
+#define UFC_LOAD_PRESSURE 1 /* consistent pressure along the outward normal */
+
+/* Mirrors ErrorStatusType (L1_IF/Error/IF_Err_Def.f90:121-136): code,
+ * severity, category, message(512), source(64), line number. */
+typedef struct ufc_status {
+  int32_t status_code;
+  int32_t severity;
+  int32_t category;
+  int32_t line_number;
+  char message[512];
+  char source[64];
+} ufc_status;
+
+/* Opaque context: owns every device buffer (the NM_Solver_Ctx work arrays,
+ * NM_Solv_Def.f90:122-132, and the CSR matrix the caller owns on the Fortran
+ * side, RT_Solv_Sparse.f90:67-70).  One GPU per context. */
+typedef struct ufc_ctx ufc_ctx;
+
+/* Mesh description (L3 mesh pool data the assembly reads; element coordinates
+ * as in elem_coords_cache(3,8,iel), PH_Elem_Domain.f90:167-168). */
+typedef struct ufc_mesh_desc {
+  int32_t n_nodes;
+  int32_t n_elems;
+  int32_t index_base;    /* base of conn                                    */
+  int32_t reserved;
+  const double* coords;  /* (3, n_nodes) column-major                       */
+  const int32_t* conn;   /* (8, n_elems) Abaqus C3D8 node order             */
+} ufc_mesh_desc;
+
+/* ====================================================================== */
+/* Context                                                                */
+/* ====================================================================== */
+
+/* Creates a context on HIP device `device` (no counterpart: the reference
+ * keeps its state in the g_ufc_global singleton, L0_Global). */
+int ufc_ctx_create(int32_t device, ufc_ctx** ctx, ufc_status* st);
+int ufc_ctx_destroy(ufc_ctx* ctx, ufc_status* st);
+
+/* Sizes after ufc_rt_asm_build_pattern. */
+typedef struct ufc_rt_asm_info {
+  int64_t n_dof;
+  int64_t nnz;
+  int32_t n_nodes;
+  int32_t n_elems;
+  int32_t max_row_nnz;
+  int32_t reserved;
+} ufc_rt_asm_info;
+int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
+
+/* ====================================================================== */
+/* L4_PH: element kernels                                                 */
+/* ====================================================================== */
+
+typedef struct ufc_ph_ke_algo {
+  int32_t precision; /* UFC_PREC_FP64 or UFC_PREC_FP32 (Ke written in it)  */
+  int32_t reserved;
+} ufc_ph_ke_algo;
+
+/* PH_Elem_Eval_Ke_Arg (PH_Elem_Def.f90:218-222) for a batch of C3D8. */
+typedef struct ufc_ph_ke_args {
+  int32_t n_elems;
+  int32_t reserved;
+  const double* coords; /* (3, 8, n_elems)                                  */
+  const double* young;  /* (n_elems): mat_props_in(1)                        */
+  double nu;            /* mat_props_in(2)                                   */
+  void* ke;             /* (24, 24, n_elems) column-major, FP64 or FP32 out  */
+} ufc_ph_ke_args;
+
+/* Batched Ke = sum_gp detJ w B^T D B, 2x2x2 Gauss: replaces
+ * PH_Elem_Eval_Ke(desc,algo,arg,status) (PH_Elem_Eval.f90:35-81) ->
+ * Compute_Ke_C3D8 (PH_ElemKeDispatch.f90:207-237) ->
+ * PH_Elem_C3D8_StiffMatrix (PH_Elem_C3D8.f90:759-780). */
+int ufc_ph_c3d8_eval_ke(ufc_ctx* ctx, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* args,
+                        ufc_status* st);
+
+/* Adjoint element sensitivities for E_e = E_e(rho_e):
+ *   grad[e] = -dEdrho[e] * lambda_e^T K_e(E=1) u_e,
+ * recomputing K_e(E=1) instead of storing it.  No counterpart in the
+ * reference (it has no sensitivity kernel); consumes the output of
+ * ufc_nm_adjoint_solve (NM_Adjoint_Solve, NM_Solv_GMRESTranspose.f90:229-291). */
+typedef struct ufc_ph_sens_args {
+  const double* u;      /* (n_dof) forward solution                          */
+  const double* lambda; /* (n_dof) adjoint solution                          */
+  const double* dedrho; /* (n_elems) dE_e/drho_e                             */
+  double nu;
+  double* grad;         /* (n_elems) out                                     */
+} ufc_ph_sens_args;
+int ufc_ph_c3d8_sensitivity(ufc_ctx* ctx, const ufc_ph_sens_args* args, ufc_status* st);
+
+/* J2 props (PH_J2_Props, PH_Mat_Plast_J2_Iso_Core.f90:52-91), linear
+ * isotropic hardening only: [E, nu, sigma_y0, H]. */
+typedef struct ufc_ph_j2_props {
+  double young;
+  double nu;
+  double sigma_y0;
+  double hardening;
+} ufc_ph_j2_props;
+
+/* Allocates the per-Gauss-point FP64 state of every element (13 doubles per
+ * point: stress(6), eq. plastic strain, plastic strain(6); the statev layout
+ * of PH_Mat_Plast_J2_UMAT_Core.f90:502-532 with stress in front) in a
+ * committed and a trial copy, both zero. */
+int ufc_ph_j2_init(ufc_ctx* ctx, const ufc_ph_j2_props* props, ufc_status* st);
+
+/* Strain increments B(u - u_n) at every Gauss point, then
+ * PH_J2_ComputeStress (PH_Mat_Plast_J2_Iso_Core.f90:133-207) from the
+ * committed state into the trial state, with the consistent tangent kept on
+ * the device for assembly (UFC_MAT_J2).  u, u_n: (n_dof).  n_yield, if not
+ * NULL, receives the number of Gauss points on the plastic branch. */
+int ufc_ph_j2_update(ufc_ctx* ctx, const double* u, const double* u_n, int64_t* n_yield,
+                     ufc_status* st);
+/* trial -> committed (end of a converged increment, RT_Step_Exec.f90:459-470) */
+int ufc_ph_j2_commit(ufc_ctx* ctx, ufc_status* st);
+/* copy the committed state out / in (checkpoint; RT_Out_Restart.f90) */
+int ufc_ph_j2_get_state(ufc_ctx* ctx, double* state, ufc_status* st);
+int ufc_ph_j2_set_state(ufc_ctx* ctx, const double* state, ufc_status* st);
+
+/* ====================================================================== */
+/* L5_RT: pattern, assembly, boundary conditions, loads, residual          */
+/* ====================================================================== */
+
+/* Uploads the mesh and builds the merged, column-sorted CSR pattern
+ * (RT_CSR_FromTripletMerged, RT_Solv_Sparse.f90:336-447) directly from node
+ * adjacency, plus the precomputed element-to-slot map that lets assembly
+ * write every CSR value once without atomics.  Replaces the full-build path of
+ * RT_Asm_GlobalStiffness (RT_Asm_Solv.f90:1902-1965).  Nodes may belong to at
+ * most 8 elements and have at most 32 neighbours (UFC_STATUS_UNSUPPORTED
+ * otherwise). */
+int ufc_rt_asm_build_pattern(ufc_ctx* ctx, const ufc_mesh_desc* mesh, ufc_rt_asm_info* info,
+                             ufc_status* st);
+
+/* Exports row_ptr (n_dof+1) and col_idx (nnz) in the caller's index base,
+ * the RT_CSRMatrix rowPtr/colInd (NM_Assem_Sparse.f90:45-56). */
+int ufc_rt_asm_get_pattern(ufc_ctx* ctx, int32_t index_base, int32_t* row_ptr,
+                           int32_t* col_idx, ufc_status* st);
+
+typedef struct ufc_rt_asm_algo {
+  int32_t precision; /* FP64 values; FP32 values; MIXED = FP32 values        */
+  int32_t material;  /* UFC_MAT_ELASTIC or UFC_MAT_J2                        */
+} ufc_rt_asm_algo;
+
+typedef struct ufc_rt_asm_args {
+  const double* young; /* (n_elems) for UFC_MAT_ELASTIC, else ignored        */
+  double nu;
+} ufc_rt_asm_args;
+
+/* K = sum_e L_e^T K_e L_e into the fixed pattern, then the registered
+ * penalties on the diagonal.  Replaces the reuse path of
+ * RT_Asm_GlobalStiffness (RT_Asm_Solv.f90:1840-1900) with
+ * RT_Asm_ScatterKe_CSR_Atomic (:3467-3493). */
+int ufc_rt_asm_assemble_k(ufc_ctx* ctx, const ufc_rt_asm_algo* algo,
+                          const ufc_rt_asm_args* args, ufc_status* st);
+
+/* Copies the values of precision `precision` (FP64 -> double, FP32 -> float). */
+int ufc_rt_asm_get_values(ufc_ctx* ctx, int32_t precision, void* values, ufc_status* st);
+
+/* Dirichlet conditions as diagonal penalties, RT_Asm_ApplyBC _Idx path
+ * (RT_Asm_Solv.f90:318-344): K(eq,eq) += penalty, F(eq) += penalty*value.
+ * The set is kept and re-applied by every later ufc_rt_asm_assemble_k;
+ * calling again replaces the set (penalties on the current K are swapped). */
+typedef struct ufc_rt_bc_args {
+  int32_t n_bc;
+  int32_t index_base;
+  const int32_t* eq;    /* (n_bc) equation ids                               */
+  const double* value;  /* (n_bc) prescribed values, NULL = 0                */
+  double penalty;       /* 1e12 in the reference                             */
+} ufc_rt_bc_args;
+int ufc_rt_asm_apply_bc(ufc_ctx* ctx, const ufc_rt_bc_args* args, ufc_status* st);
+
+/* Face loads into the load vector F (RT_Asm_GlobalLoad surface branch,
+ * RT_Asm_Solv.f90:1614-1647; lumping of LoadBC_DistributeLoad_ToSurface,
+ * MD_LBC_Apply.f90:258-293; consistent pressure of
+ * PH_Elem_C3D8_FormFacePressure, PH_Elem_C3D8.f90:2360-2541, positive
+ * pressure along the outward normal).  Faces are numbered 1..6 as in
+ * PH_ELEM_C3D8_FACE_NODES (PH_Elem_C3D8.f90:120-121) when index_base is 1. */
+typedef struct ufc_rt_load_args {
+  int32_t n_faces;
+  int32_t kind;         /* UFC_LOAD_TRACTION or UFC_LOAD_PRESSURE            */
+  int32_t index_base;   /* base of elem and face                             */
+  int32_t accumulate;   /* 0: F is cleared first                             */
+  const int32_t* elem;  /* (n_faces)                                         */
+  const int32_t* face;  /* (n_faces)                                         */
+  double magnitude;     /* pressure (UFC_LOAD_PRESSURE)                      */
+  double traction[3];   /* traction vector (UFC_LOAD_TRACTION)               */
+} ufc_rt_load_args;
+int ufc_rt_asm_load(ufc_ctx* ctx, const ufc_rt_load_args* args, ufc_status* st);
+
+/* Sets / reads the load vector F (n_dof); penalty terms of apply_bc are
+ * included when read. */
+int ufc_rt_set_rhs(ufc_ctx* ctx, const double* f, ufc_status* st);
+int ufc_rt_get_rhs(ufc_ctx* ctx, double* f, ufc_status* st);
+
+/* Residual R = F - F_int(u) - P u on penalty rows (F includes P ubar):
+ * RT_Asm_ComputeResidual (RT_Asm_Solv.f90:1311-1426).  F_int comes from the
+ * J2 trial stresses when the J2 state exists, else from the elastic field
+ * given at the last assembly.  norms[0] = ||R||_2, norms[1] = ||F_int||_2. */
+int ufc_rt_residual(ufc_ctx* ctx, const double* u, double* r, double* norms, ufc_status* st);
+
+/* ====================================================================== */
+/* L2_NM: SpMV and solvers                                                */
+/* ====================================================================== */
+
+/* y = K x with the assembled values of `precision` (FP64: scalar CSR as
+ * NM_CSR_MatVec_Optimized, NM_Mtx_Sparse.f90:341-365; FP32: node-block index
+ * view of the same CSR).  x, y: (n_dof) FP64. */
+int ufc_nm_spmv(ufc_ctx* ctx, int32_t precision, const double* x, double* y, ufc_status* st);
+
+/* NM_Solver_Algo (NM_Solv_Def.f90:86-100) with the fields the solver reads
+ * (NM_Solv_Core.f90:392-399 uses maxiter/rtol). */
+typedef struct ufc_nm_solver_algo {
+  int32_t precision;   /* UFC_PREC_FP64, UFC_PREC_FP32 or UFC_PREC_MIXED    */
+  int32_t max_iter;    /* PCG iterations (per inner solve for MIXED)        */
+  double rtol;         /* ||r||/||b|| target (outer, FP64, for MIXED)        */
+  double inner_rtol;   /* MIXED: inner FP32 PCG tolerance                    */
+  int32_t max_outer;   /* MIXED: refinement steps                            */
+  int32_t check_every; /* host poll interval in iterations (0 = 32); the    */
+                       /* stopping iteration is decided on the device        */
+} ufc_nm_solver_algo;
+
+/* NM_Solver_State (NM_Solv_Def.f90:102-120). */
+typedef struct ufc_nm_solver_state {
+  double* x;          /* (n_dof) in: x0 / out: solution                      */
+  const double* b;    /* (n_dof) right-hand side; NULL = the load vector F   */
+  int32_t niter;      /* out: PCG iterations (sum over inner solves)         */
+  int32_t n_outer;    /* out: refinement steps (MIXED)                       */
+  double rnorm;       /* out: ||r||/||b|| at exit (MIXED: FP64 true residual) */
+  int32_t converged;  /* out                                                 */
+  int32_t x0_zero;    /* in: 1 = ignore x on entry and start from 0          */
+} ufc_nm_solver_state;
+
+/* Jacobi-preconditioned CG on the assembled K: NM_Solver_PCG
+ * (NM_Solv_Core.f90:363-423) with NM_Solver_Jacobi_Precond (:131-147).
+ * MIXED wraps FP32 PCG (FP64 dot accumulation) in FP64 iterative refinement:
+ * r = F - K u recomputed element by element in FP64, u += d in FP64.
+ * Returns UFC_STATUS_OK when converged, UFC_STATUS_INVALID when max_iter is
+ * reached (the reference's code, NM_Solv_Core.f90:419-422), UFC_STATUS_ERROR
+ * on a breakdown (p.Ap <= 0 or non-finite). */
+int ufc_nm_pcg(ufc_ctx* ctx, const ufc_nm_solver_algo* algo, ufc_nm_solver_state* state,
+               ufc_status* st);
+
+/* K^T lambda = dJdu with the same Jacobi-PCG (K symmetric, so K^T = K):
+ * replaces NM_Adjoint_Solve (NM_Solv_GMRESTranspose.f90:229-291) ->
+ * NM_CG_Solve_Transpose (:118-224, unpreconditioned, lambda_0 = 0).
+ * state->x receives lambda; state->b is ignored. */
+int ufc_nm_adjoint_solve(ufc_ctx* ctx, const ufc_nm_solver_algo* algo, const double* dJdu,
+                         ufc_nm_solver_state* state, ufc_status* st);
+
+/* Device time in ms of the last assembly and the last solve on this context
+ * (HIP events on the context's stream), and the PCG iteration counts. */
+typedef struct ufc_timing {
+  double assemble_ms;
+  double solve_ms;
+  double spmv_ms_avg;  /* mean SpMV kernel time of the last solve (events)   */
+  int32_t spmv_calls;
+  int32_t reserved;
+} ufc_timing;
+int ufc_ctx_timing(ufc_ctx* ctx, ufc_timing* t, ufc_status* st);
+/* per_kernel = 1 brackets every SpMV launch with a HIP event pair so that
+ * spmv_ms_avg is the mean kernel duration (no counterpart in the reference,
+ * whose timers are CPU_TIME around whole phases, RT_Asm_Global.f90:354,399). */
+int ufc_ctx_enable_timing(ufc_ctx* ctx, int32_t per_kernel, ufc_status* st);
+
+/* ====================================================================== */
+/* Mesh and field generators for the structured benchmark inputs          */
+/* (no counterpart in the reference, which reads meshes from input decks) */
+/* ====================================================================== */
+
+/* Box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz C3D8, node id i + (nx+1)(j + (ny+1)k),
+ * elements in the same order; conn in index_base.  jitter > 0 moves interior
+ * nodes by U[-jitter, jitter] x h per coordinate (seeded). */
+int ufc_mesh_box(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz,
+                 double jitter, uint64_t seed, int32_t index_base, double* coords,
+                 int32_t* conn, ufc_status* st);
+/* Per-element fields from a counter-based generator keyed by (seed, element):
+ * kind 0: lognormal exp(a + b z); kind 1: uniform a + (b - a) u. */
+int ufc_field(int32_t kind, int64_t n, uint64_t seed, double a, double b, double* out,
+              ufc_status* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UFC_HIP_H */

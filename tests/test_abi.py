@@ -1,0 +1,75 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/ufc_hip.h declares, its host-side generators agree with the oracle's
+bit for bit, and context creation without a device reports an error code
+instead of aborting."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "ufc_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\bint\s+(ufc_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ufc_hip
+    L = C.CDLL(ufc_hip.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes mirror binds exactly these names
+    from ufc_hip import _abi
+    assert sorted(_abi._SIGS) == names
+
+
+def test_status_codes_mirror_reference():
+    # IF_Err_Brg.f90:95-109
+    from ufc_hip import _abi as A
+    assert (A.STATUS_OK, A.STATUS_INVALID, A.STATUS_ERROR, A.STATUS_CONVERGED,
+            A.STATUS_NOT_CONVERGED, A.STATUS_WARN, A.STATUS_MEM_ERROR, A.STATUS_FATAL,
+            A.STATUS_UNSUPPORTED) == (0, 1, 2, 3, 4, 6, 7, 10, 11)
+    assert C.sizeof(A.Status) == 4 * 4 + 512 + 64
+
+
+def test_generators_match_oracle(orc):
+    import ufc_hip
+    for (nx, ny, nz, jit) in [(3, 2, 4, 0.0), (5, 4, 3, 0.1)]:
+        c1, e1 = ufc_hip.mesh_box(nx, ny, nz, 2.0, 1.0, 3.0, jitter=jit, seed=17)
+        c2, e2 = orc.mesh_box(nx, ny, nz, 2.0, 1.0, 3.0, jitter=jit, seed=17)
+        assert np.array_equal(c1, c2) and np.array_equal(e1, e2)
+        c3, e3 = ufc_hip.mesh_box(nx, ny, nz, index_base=1)
+        assert np.array_equal(e3, orc.mesh_box(nx, ny, nz)[1] + 1)
+    a = ufc_hip.field("lognormal", 1000, 20251203, np.log(1e9), 0.5)
+    b = orc.field_lognormal(1000, 20251203, np.log(1e9), 0.5)
+    assert np.array_equal(a, b)
+    assert np.array_equal(ufc_hip.field("uniform", 1000, 5, 0.3, 1.0),
+                          orc.field_uniform(1000, 5, 0.3, 1.0))
+    # lognormal sample statistics: log E ~ N(ln 1e9, 0.5)
+    la = np.log(ufc_hip.field("lognormal", 200000, 1, np.log(1e9), 0.5))
+    assert abs(la.mean() - np.log(1e9)) < 0.01 and abs(la.std() - 0.5) < 0.01
+
+
+def test_problem_definitions():
+    from ufc_hip import problems as P
+    pr = P.cantilever()
+    assert pr.n_dof == 1575 and pr.bc_eq.size == 75 and pr.load_elem.size == 16
+    pc = P.cube_pressure(4)
+    assert pc.bc_eq.size == 3 * 25 and pc.load_elem.size == 16 and set(pc.load_face) == {1}
+
+
+def test_errors_are_status_codes_without_gpu():
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    import ufc_hip
+    with pytest.raises(ufc_hip.UfcError) as ei:
+        ufc_hip.Context(0)
+    assert ei.value.code in (1, 2)

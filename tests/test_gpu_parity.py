@@ -1,0 +1,220 @@
+"""HIP library vs the FP64 oracle, through the C-ABI (libufc_hip.so).
+
+Tolerances (stated per test): CSR row_ptr/col_idx bit-exact; FP64 matrix
+values |dK| <= 1e-12 max|K|; FP32 values within FP32 rounding of the FP64
+ones; displacements 1e-8 relative where both sides are driven to a tight
+residual.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+UNIT = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                 [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _problem_small(kind):
+    from ufc_hip import problems as P
+    if kind == "cantilever":
+        return P.cantilever()
+    if kind == "cube8":
+        return P.cube_pressure(8, name="cube8")
+    if kind == "jitter6":
+        return P.cube_pressure(6, name="jitter6", jitter=0.1)
+    raise KeyError(kind)
+
+
+def _oracle_system(orc, pr, penalty=1e12):
+    rp, ci = orc.csr_pattern(pr.coords.shape[0], pr.conn)
+    val = orc.assemble(pr.conn, pr.coords, pr.young, pr.nu, rp, ci)
+    F = np.zeros(pr.n_dof)
+    if pr.load_kind == "traction":
+        orc.face_loads_traction(pr.load_elem, pr.load_face, pr.conn, pr.coords, pr.load_value, F)
+    else:
+        orc.face_loads_pressure(pr.load_elem, pr.load_face, pr.conn, pr.coords,
+                                pr.load_value[0], F)
+    f0 = F.copy()
+    orc.apply_penalty(pr.bc_eq, pr.bc_val, penalty, rp, ci, val, F)
+    return rp, ci, val, F, f0
+
+
+def test_ke_batch_fp64_fp32(ctx, orc):
+    rng = np.random.default_rng(0)
+    n = 200
+    xe = UNIT[None] * rng.uniform(0.5, 2.0, (n, 1, 3)) + 0.15 * rng.uniform(-1, 1, (n, 8, 3))
+    E = rng.uniform(1e9, 2e11, n)
+    say("eval_ke n", n)
+    K = ctx.eval_ke(xe, E, 0.3)
+    Ko = np.stack([orc.ke(xe[i], E[i], 0.3) for i in range(n)])
+    err = np.abs(K - Ko).max(axis=(1, 2)) / np.abs(Ko).max(axis=(1, 2))
+    assert err.max() < 1e-12
+    K32 = ctx.eval_ke(xe, E, 0.3, precision=1)
+    err32 = np.abs(K32 - Ko).max(axis=(1, 2)) / np.abs(Ko).max(axis=(1, 2))
+    assert err32.max() < 2e-5
+
+
+@pytest.mark.parametrize("mesh", ["1x1x1", "2x1x1", "3x2x2", "20x4x4"])
+def test_pattern_bitexact_vs_reference(ctx, golden_dir, mesh):
+    import ufc_hip
+    g = np.load(os.path.join(golden_dir, f"ref_csr_{mesh}.npz"))
+    nx, ny, nz = (int(v) for v in g["mesh"])
+    coords, conn = ufc_hip.mesh_box(nx, ny, nz, index_base=1)
+    say("pattern", mesh)
+    info = ctx.build_pattern(coords, conn, index_base=1)
+    assert info.nnz == g["ja"].size
+    rp, ci = ctx.get_pattern(index_base=1)
+    assert np.array_equal(rp, g["ia"]) and np.array_equal(ci, g["ja"])
+
+
+@pytest.mark.parametrize("kind", ["cantilever", "cube8", "jitter6"])
+def test_assembly_fp64_fp32_vs_oracle(ctx, orc, kind):
+    from ufc_hip import problems as P
+    pr = _problem_small(kind)
+    rp_o, ci_o, val_o, F_o, _ = _oracle_system(orc, pr)
+    say("assembly", kind, "dof", pr.n_dof)
+    P.setup_context(ctx, pr, 0)
+    rp, ci = ctx.get_pattern()
+    assert np.array_equal(rp, rp_o) and np.array_equal(ci, ci_o)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    v64 = ctx.get_values(0)
+    scale = np.abs(val_o).max()
+    assert np.abs(v64 - val_o).max() <= 1e-12 * scale
+    assert np.array_equal(ctx.get_rhs(), F_o) or np.abs(ctx.get_rhs() - F_o).max() < 1e-12 * np.abs(F_o).max()
+    # bitwise run-to-run determinism of the atomics-free scatter
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    assert np.array_equal(ctx.get_values(0), v64)
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    v32 = ctx.get_values(1).astype(np.float64)
+    assert np.abs(v32 - val_o).max() <= 1e-6 * scale
+
+
+def test_spmv_fp64_fp32(ctx, orc):
+    from ufc_hip import problems as P
+    pr = _problem_small("cube8")
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    x = np.random.default_rng(1).normal(size=pr.n_dof)
+    y = ctx.spmv(x, precision=0)
+    yo = orc.spmv(rp, ci, val, x)
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    y32 = ctx.spmv(x, precision=1)
+    assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("kind", ["cantilever", "jitter6"])
+def test_pcg_fp64_vs_oracle(ctx, orc, kind):
+    from ufc_hip import problems as P
+    pr = _problem_small(kind)
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    uo, st, it_o, rn_o = orc.pcg(rp, ci, val, F, rtol=1e-10)
+    assert st == 0
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    say("pcg fp64", kind)
+    u, info = ctx.pcg(precision=0, rtol=1e-10)
+    assert info.status == 0 and info.converged and info.rnorm < 1e-10
+    # the same algorithm: the stopping iteration matches up to rounding drift
+    assert abs(info.niter - it_o) <= max(2, it_o // 100)
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+    # both driven to a tight residual: displacements agree to 1e-8 (here ~1e-12)
+    uo2, _, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
+    u2, info2 = ctx.pcg(precision=0, rtol=1e-14, max_iter=200000)
+    assert np.linalg.norm(u2 - uo2) / np.linalg.norm(uo2) < 1e-10
+
+
+def test_pcg_mixed_refinement_vs_oracle(ctx, orc):
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(12, name="cube12")
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
+    P.setup_context(ctx, pr, 2)
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    say("pcg mixed cube12")
+    u, info = ctx.pcg(precision=2, rtol=1e-10, inner_rtol=1e-5)
+    assert info.status == 0 and info.rnorm < 1e-10 and info.n_outer >= 2
+    # true FP64 residual of the oracle's operator
+    K_u = orc.spmv(rp, ci, val, u)
+    assert np.linalg.norm(F - K_u) / np.linalg.norm(F) < 2e-10
+    err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
+    say("mixed err vs oracle", err, "outer", info.n_outer, "inner", info.niter)
+    assert err < 1e-8
+
+
+def test_pcg_status_codes(ctx):
+    from ufc_hip import problems as P
+    pr = _problem_small("cantilever")
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    u, info = ctx.pcg(precision=0, rtol=1e-10, max_iter=5)
+    assert info.status == 1 and not info.converged and info.niter == 5
+    # negative modulus: p.Ap <= 0 breakdown is reported, not a NaN result
+    ctx.assemble_k(-pr.young, pr.nu, precision=0)
+    ctx.apply_bc(pr.bc_eq, pr.bc_val, -1e12)
+    u, info = ctx.pcg(precision=0, rtol=1e-10, max_iter=50)
+    assert info.status == 2 and not info.converged
+
+
+def test_adjoint_and_sensitivity_vs_oracle_and_fd(ctx, orc):
+    from ufc_hip import problems as P
+    pr = P.simp_cube(4)
+    rho = pr.extra["rho"]
+    rp, ci, val, F, f0 = _oracle_system(orc, pr)
+    uo, _, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=100000)
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    u, info = ctx.pcg(precision=0, rtol=1e-13)
+    lam, ainfo = ctx.adjoint_solve(f0, precision=0, rtol=1e-13)
+    # reference adjoint algorithm (NM_CG_Solve_Transpose) on the same K
+    lo, st, _, _ = orc.cg_transpose(rp, ci, val, f0, tol=1e-13, max_iter=100000)
+    assert st == 0 and np.linalg.norm(lam - lo) / np.linalg.norm(lo) < 1e-9
+    dedrho = 3 * rho ** 2 * 1e9
+    g = ctx.sensitivity(u, lam, dedrho, pr.nu)
+    go = orc.sensitivity(pr.conn, pr.coords, pr.nu, dedrho, uo, lo)
+    assert np.abs(g - go).max() < 1e-9 * np.abs(go).max()
+
+
+def test_j2_update_and_newton_vs_oracle(ctx, orc):
+    import ufc_hip
+    from ufc_hip import problems as P
+    coords, conn = ufc_hip.mesh_box(3, 3, 3, jitter=0.1, seed=9)
+    bottom = np.nonzero(coords[:, 2] == 0)[0]
+    top = np.nonzero(coords[:, 2] == 1)[0]
+    eq = np.concatenate([(3 * bottom[:, None] + np.arange(3)).ravel(), 3 * top + 2]).astype(np.int32)
+    props = (200e9, 0.3, 250e6, 2e9)
+    ctx.build_pattern(coords, conn)
+    ctx.j2_init(*props)
+    n = 3 * coords.shape[0]
+    u_n = np.zeros(n)
+    state = np.zeros(conn.shape[0] * 8 * 13)
+    uo = np.zeros(n)
+    for inc in (1, 2, 3):
+        val = np.concatenate([np.zeros(3 * bottom.size), np.full(top.size, 1e-3 * inc)])
+        st, uo, state, nn, rn = orc.j2_newton_step(coords.shape[0], conn, coords, props, eq,
+                                                   val, 1e12, state, uo)
+        assert st == 0
+        ctx.apply_bc(eq, val, 1e12)
+        u = u_n.copy()
+        for it in range(25):
+            ctx.j2_update(u, u_n)
+            r, norms = ctx.residual(u)
+            if it > 0 and norms[0] <= 1e-8 * norms[1]:
+                break
+            ctx.assemble_k(None, 0.3, precision=0, material=1)
+            du, info = ctx.pcg(precision=0, rtol=1e-12, b=r)
+            u = u + du
+        ctx.j2_commit()
+        u_n = u.copy()
+        say("j2 inc", inc, "newton", it, "oracle", nn)
+        assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-7
+    sg = ctx.j2_get_state().ravel()
+    so = state
+    assert np.abs(sg - so).max() < 1e-6 * np.abs(so).max()

@@ -1,0 +1,938 @@
+// ufc_api.cpp -- the C-ABI of libufc_hip.so: context lifetime, pattern and
+// element-to-slot map construction, load and boundary-condition bundles, and
+// the solver orchestration (FP64 / FP32 PCG, FP32-in-FP64 iterative
+// refinement, adjoint).  Compiled as HIP host code.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <thread>
+
+#include "ufc_internal.hpp"
+
+using ufc::Error;
+
+namespace {
+
+void set_status(ufc_status* st, int code, const std::string& msg, const char* src) {
+  if (!st) return;
+  st->status_code = code;
+  st->severity = code == UFC_STATUS_OK ? 0 : (code == UFC_STATUS_WARN ? 1 : 3);
+  st->category = code;
+  st->line_number = 0;
+  std::snprintf(st->message, sizeof st->message, "%s", msg.c_str());
+  std::snprintf(st->source, sizeof st->source, "%s", src);
+}
+
+template <typename F>
+int guarded(ufc_status* st, const char* src, F&& body) {
+  try {
+    std::string msg;
+    int code = body(msg);
+    set_status(st, code, msg, src);
+    return code;
+  } catch (const Error& e) {
+    set_status(st, e.code, e.msg, src);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_status(st, UFC_STATUS_MEM_ERROR, "host allocation failed", src);
+    return UFC_STATUS_MEM_ERROR;
+  } catch (...) {
+    set_status(st, UFC_STATUS_FATAL, "unexpected exception", src);
+    return UFC_STATUS_FATAL;
+  }
+}
+
+template <typename F>
+void parallel_for(int64_t n, F&& f) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 65536) nt = 1;
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int64_t b = int64_t(t) * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([&, b, e] {
+      for (int64_t i = b; i < e; ++i) f(i);
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
+// true when p is device memory of the current device
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
+// device view of a caller array of n elements: the pointer itself when it is
+// device memory, else a copy in tmp
+template <typename T>
+const T* dev_in(ufc_ctx* c, const T* p, size_t n, ufc::DBuf<T>& tmp) {
+  if (is_device_ptr(p)) return p;
+  tmp.resize(n);
+  UFC_HIP_CHECK(hipMemcpyAsync(tmp.get(), p, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return tmp.get();
+}
+
+template <typename T>
+void copy_out(ufc_ctx* c, T* dst, const T* src, size_t n) {
+  UFC_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDefault, c->stream));
+  UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void check_ctx(ufc_ctx* c) { UFC_REQUIRE(c != nullptr, UFC_STATUS_INVALID, "null context"); }
+void check_mesh(ufc_ctx* c) {
+  check_ctx(c);
+  UFC_REQUIRE(c->n_nodes > 0 && c->nb_ptr.get(), UFC_STATUS_INVALID,
+              "no mesh: call ufc_rt_asm_build_pattern first");
+}
+
+// face node table, PH_ELEM_C3D8_FACE_NODES (PH_Elem_C3D8.f90:120-121), 0-based
+constexpr int kFaceNodes[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 1, 5, 4},
+                                  {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 5, 6, 2}};
+constexpr double kGaussPt = 0.577350269189626;
+
+// 2x2 Gauss over face f of element coordinates x (8,3): per point the four
+// face shape values (trilinear 1/8 form on the face), the surface Jacobian
+// dA and the outward unit normal -- PH_Elem_C3D8_FormFacePressure,
+// PH_Elem_C3D8.f90:2360-2541.
+template <typename F>
+void face_quadrature(const double (&x)[8][3], int f, F&& visit) {
+  static const int fixed[6] = {2, 2, 1, 1, 0, 0};
+  static const double fval[6] = {-1, 1, -1, 1, -1, 1};
+  static const int cu[6] = {0, 0, 0, 0, 1, 1};
+  static const int cv[6] = {1, 1, 2, 2, 2, 2};
+  static const double sgn[6] = {-1, 1, -1, 1, -1, 1};
+  static const double fu[4] = {-kGaussPt, kGaussPt, -kGaussPt, kGaussPt};
+  static const double fv[4] = {-kGaussPt, -kGaussPt, kGaussPt, kGaussPt};
+  static const double XS[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+  static const double YS[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+  static const double ZS[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+  for (int ip = 0; ip < 4; ++ip) {
+    double nat[3];
+    nat[fixed[f]] = fval[f];
+    nat[cu[f]] = fu[ip];
+    nat[cv[f]] = fv[ip];
+    double N[8], dn[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const double fx = 1 + XS[a] * nat[0], fy = 1 + YS[a] * nat[1], fz = 1 + ZS[a] * nat[2];
+      N[a] = 0.125 * fx * fy * fz;
+      dn[a][0] = 0.125 * XS[a] * fy * fz;
+      dn[a][1] = 0.125 * fx * YS[a] * fz;
+      dn[a][2] = 0.125 * fx * fy * ZS[a];
+    }
+    double du[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+      const int a = kFaceNodes[f][q];
+      for (int d = 0; d < 3; ++d) {
+        du[d] += dn[a][cu[f]] * x[a][d];
+        dv[d] += dn[a][cv[f]] * x[a][d];
+      }
+    }
+    double n[3] = {du[1] * dv[2] - du[2] * dv[1], du[2] * dv[0] - du[0] * dv[2],
+                   du[0] * dv[1] - du[1] * dv[0]};
+    const double dA = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (dA < 1.0e-15) continue;
+    for (double& v : n) v = v / dA;
+    for (double& v : n) v *= sgn[f];
+    double Nf[4];
+    for (int q = 0; q < 4; ++q) Nf[q] = N[kFaceNodes[f][q]];
+    visit(Nf, dA, n);
+  }
+}
+
+// rebuild the device load vector: f = F + sum P*ubar
+void upload_rhs(ufc_ctx* c) {
+  std::vector<double> h(c->h_f);
+  for (size_t i = 0; i < c->h_bc_eq.size(); ++i) h[size_t(c->h_bc_eq[i])] += c->h_bc_rhs[i];
+  c->f.resize(size_t(c->n_dof));
+  UFC_HIP_CHECK(hipMemcpyAsync(c->f.get(), h.data(), h.size() * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+  UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// FP64 operator for refinement residuals: y = K u (no penalties), matrix-free
+void apply_fp64_operator(ufc_ctx* c, const double* u, double* y) {
+  if (c->last_material == UFC_MAT_J2)
+    ufc::launch_apply_elem64(c, 1, u, y);
+  else
+    ufc::launch_apply_elem64(c, 0, u, y);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ======================================================================= ctx
+int ufc_ctx_create(int32_t device, ufc_ctx** ctx, ufc_status* st) {
+  return guarded(st, "ufc_ctx_create", [&](std::string& msg) {
+    UFC_REQUIRE(ctx != nullptr, UFC_STATUS_INVALID, "ctx out-pointer is null");
+    int ndev = 0;
+    UFC_HIP_CHECK(hipGetDeviceCount(&ndev));
+    UFC_REQUIRE(device >= 0 && device < ndev, UFC_STATUS_INVALID, "no such HIP device");
+    UFC_HIP_CHECK(hipSetDevice(device));
+    auto* c = new ufc_ctx();
+    c->device = device;
+    try {
+      UFC_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      UFC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_scalars), sizeof(ufc::PcgScalars),
+                                  hipHostMallocDefault));
+      UFC_HIP_CHECK(hipEventCreate(&c->ev0));
+      UFC_HIP_CHECK(hipEventCreate(&c->ev1));
+      UFC_HIP_CHECK(hipEventCreate(&c->ev2));
+      UFC_HIP_CHECK(hipEventCreate(&c->ev3));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *ctx = c;
+    msg = "context created";
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ctx_destroy(ufc_ctx* c, ufc_status* st) {
+  return guarded(st, "ufc_ctx_destroy", [&](std::string& msg) {
+    if (!c) return UFC_STATUS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->spmv_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev0, c->ev1, c->ev2, c->ev3})
+      if (e) (void)hipEventDestroy(e);
+    if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    hipStream_t s = c->stream;
+    delete c;  // frees device buffers
+    if (s) (void)hipStreamDestroy(s);
+    msg = "context destroyed";
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
+  return guarded(st, "ufc_ctx_info", [&](std::string&) {
+    check_ctx(c);
+    UFC_REQUIRE(info != nullptr, UFC_STATUS_INVALID, "info is null");
+    info->n_dof = c->n_dof;
+    info->nnz = c->nnz;
+    info->n_nodes = c->n_nodes;
+    info->n_elems = c->n_elems;
+    info->max_row_nnz = c->max_row_nnz;
+    info->reserved = 0;
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ctx_enable_timing(ufc_ctx* c, int32_t per_kernel, ufc_status* st) {
+  return guarded(st, "ufc_ctx_enable_timing", [&](std::string&) {
+    check_ctx(c);
+    c->per_kernel = per_kernel != 0;
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ctx_timing(ufc_ctx* c, ufc_timing* t, ufc_status* st) {
+  return guarded(st, "ufc_ctx_timing", [&](std::string&) {
+    check_ctx(c);
+    UFC_REQUIRE(t != nullptr, UFC_STATUS_INVALID, "timing out-pointer is null");
+    t->assemble_ms = c->assemble_ms;
+    t->solve_ms = c->solve_ms;
+    t->spmv_ms_avg = c->spmv_ms_avg;
+    t->spmv_calls = c->spmv_calls;
+    t->reserved = 0;
+    return UFC_STATUS_OK;
+  });
+}
+
+// ======================================================================= L4_PH
+int ufc_ph_c3d8_eval_ke(ufc_ctx* c, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* a,
+                        ufc_status* st) {
+  return guarded(st, "ufc_ph_c3d8_eval_ke", [&](std::string&) {
+    check_ctx(c);
+    UFC_REQUIRE(algo && a && a->coords && a->young && a->ke, UFC_STATUS_INVALID,
+                "null argument");
+    UFC_REQUIRE(a->n_elems >= 0, UFC_STATUS_INVALID, "n_elems < 0");
+    UFC_REQUIRE(algo->precision == UFC_PREC_FP64 || algo->precision == UFC_PREC_FP32,
+                UFC_STATUS_INVALID, "precision must be FP64 or FP32");
+    UFC_REQUIRE(a->nu > -1.0 && a->nu < 0.5, UFC_STATUS_INVALID, "nu out of (-1, 0.5)");
+    if (a->n_elems == 0) return UFC_STATUS_OK;
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    const size_t ne = size_t(a->n_elems);
+    ufc::DBuf<double> tc, ty;
+    const double* dc = dev_in(c, a->coords, ne * 24, tc);
+    const double* dy = dev_in(c, a->young, ne, ty);
+    const bool fp32 = algo->precision == UFC_PREC_FP32;
+    const size_t esz = fp32 ? sizeof(float) : sizeof(double);
+    void* out = a->ke;
+    ufc::DBuf<char> tk;
+    if (!is_device_ptr(a->ke)) {
+      tk.resize(ne * 576 * esz);
+      out = tk.get();
+    }
+    ufc::launch_eval_ke(a->n_elems, dc, dy, a->nu, out, fp32, c->stream);
+    if (out != a->ke)
+      UFC_HIP_CHECK(hipMemcpyAsync(a->ke, out, ne * 576 * esz, hipMemcpyDeviceToHost, c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ph_c3d8_sensitivity(ufc_ctx* c, const ufc_ph_sens_args* a, ufc_status* st) {
+  return guarded(st, "ufc_ph_c3d8_sensitivity", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(a && a->u && a->lambda && a->dedrho && a->grad, UFC_STATUS_INVALID,
+                "null argument");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    ufc::DBuf<double> tu, tl, td, tg;
+    const double* du = dev_in(c, a->u, size_t(c->n_dof), tu);
+    const double* dl = dev_in(c, a->lambda, size_t(c->n_dof), tl);
+    const double* dd = dev_in(c, a->dedrho, size_t(c->n_elems), td);
+    double* g = a->grad;
+    if (!is_device_ptr(g)) {
+      tg.resize(size_t(c->n_elems));
+      g = tg.get();
+    }
+    ufc::launch_sensitivity(c, du, dl, dd, a->nu, g);
+    if (g != a->grad) copy_out(c, a->grad, g, size_t(c->n_elems));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ph_j2_init(ufc_ctx* c, const ufc_ph_j2_props* p, ufc_status* st) {
+  return guarded(st, "ufc_ph_j2_init", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(p != nullptr, UFC_STATUS_INVALID, "props is null");
+    // PH_J2_Init validation, PH_Mat_Plast_J2_Iso_Core.f90:220-245
+    UFC_REQUIRE(p->young > 0.0 && p->nu >= 0.0 && p->nu < 0.5, UFC_STATUS_ERROR,
+                "Invalid elastic parameters (E<=0 or nu out of range)");
+    UFC_REQUIRE(p->sigma_y0 > 0.0, UFC_STATUS_ERROR, "sigma_y0 must be positive");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    const size_t ngp = size_t(c->n_elems) * 8;
+    c->j2 = *p;
+    c->j2_commit.resize(ngp * 13);
+    c->j2_trial.resize(ngp * 13);
+    c->j2_tangent.resize(ngp * 36);
+    c->j2_flags.resize(ngp);
+    UFC_HIP_CHECK(hipMemsetAsync(c->j2_commit.get(), 0, c->j2_commit.bytes(), c->stream));
+    UFC_HIP_CHECK(hipMemsetAsync(c->j2_trial.get(), 0, c->j2_trial.bytes(), c->stream));
+    UFC_HIP_CHECK(hipMemsetAsync(c->j2_tangent.get(), 0, c->j2_tangent.bytes(), c->stream));
+    UFC_HIP_CHECK(hipMemsetAsync(c->j2_flags.get(), 0, c->j2_flags.bytes(), c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->have_j2 = true;
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ph_j2_update(ufc_ctx* c, const double* u, const double* u_n, int64_t* n_yield,
+                     ufc_status* st) {
+  return guarded(st, "ufc_ph_j2_update", [&](std::string& msg) {
+    check_mesh(c);
+    UFC_REQUIRE(c->have_j2, UFC_STATUS_INVALID, "call ufc_ph_j2_init first");
+    UFC_REQUIRE(u && u_n, UFC_STATUS_INVALID, "null argument");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    ufc::DBuf<double> tu, tn;
+    const double* du = dev_in(c, u, size_t(c->n_dof), tu);
+    const double* dn = dev_in(c, u_n, size_t(c->n_dof), tn);
+    ufc::launch_j2_update(c, du, dn);
+    std::vector<int32_t> fl(static_cast<size_t>(c->n_elems) * 8);
+    UFC_HIP_CHECK(hipMemcpyAsync(fl.data(), c->j2_flags.get(), fl.size() * sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    int64_t ny = 0, nfail = 0;
+    for (int32_t f : fl) {
+      ny += (f == 1);
+      nfail += (f == 2);
+    }
+    if (n_yield) *n_yield = ny;
+    if (nfail) {
+      // local Newton failure -> the caller should cut the increment back
+      // (pnewdt = 0.25, PH_Mat_Plast_J2_Iso_Core.f90:419-427)
+      msg = "J2 local Newton failed at " + std::to_string(nfail) + " Gauss points";
+      return UFC_STATUS_ERROR;
+    }
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ph_j2_commit(ufc_ctx* c, ufc_status* st) {
+  return guarded(st, "ufc_ph_j2_commit", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(c->have_j2, UFC_STATUS_INVALID, "call ufc_ph_j2_init first");
+    UFC_HIP_CHECK(hipMemcpyAsync(c->j2_commit.get(), c->j2_trial.get(), c->j2_trial.bytes(),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ph_j2_get_state(ufc_ctx* c, double* state, ufc_status* st) {
+  return guarded(st, "ufc_ph_j2_get_state", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(c->have_j2 && state, UFC_STATUS_INVALID, "no J2 state or null argument");
+    copy_out(c, state, c->j2_commit.get(), c->j2_commit.n);
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_ph_j2_set_state(ufc_ctx* c, const double* state, ufc_status* st) {
+  return guarded(st, "ufc_ph_j2_set_state", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(c->have_j2 && state, UFC_STATUS_INVALID, "no J2 state or null argument");
+    UFC_HIP_CHECK(hipMemcpyAsync(c->j2_commit.get(), state, c->j2_commit.bytes(),
+                                 hipMemcpyDefault, c->stream));
+    UFC_HIP_CHECK(hipMemcpyAsync(c->j2_trial.get(), c->j2_commit.get(), c->j2_commit.bytes(),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return UFC_STATUS_OK;
+  });
+}
+
+// ======================================================================= L5_RT
+int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info* info,
+                             ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_build_pattern", [&](std::string& msg) {
+    check_ctx(c);
+    UFC_REQUIRE(m && m->coords && m->conn, UFC_STATUS_INVALID, "null mesh argument");
+    UFC_REQUIRE(m->n_nodes > 0 && m->n_elems > 0, UFC_STATUS_INVALID, "empty mesh");
+    UFC_REQUIRE(int64_t(m->n_nodes) * 3 < INT32_MAX, UFC_STATUS_UNSUPPORTED,
+                "3*n_nodes exceeds int32 equation ids");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    const int32_t nn = m->n_nodes, ne = m->n_elems;
+    const int base = m->index_base;
+    // mesh copies (0-based connectivity), validated
+    c->h_coords.assign(m->coords, m->coords + size_t(nn) * 3);
+    c->h_conn.resize(size_t(ne) * 8);
+    for (size_t i = 0; i < c->h_conn.size(); ++i) {
+      const int32_t v = m->conn[i] - base;
+      UFC_REQUIRE(v >= 0 && v < nn, UFC_STATUS_INVALID,
+                  "connectivity entry out of range at element " + std::to_string(i / 8 + base));
+      c->h_conn[i] = v;
+    }
+    for (int64_t e = 0; e < ne; ++e)
+      for (int a = 0; a < 8; ++a)
+        for (int b = a + 1; b < 8; ++b)
+          UFC_REQUIRE(c->h_conn[8 * e + a] != c->h_conn[8 * e + b], UFC_STATUS_INVALID,
+                      "element " + std::to_string(e + base) + " repeats a node");
+    // node -> (element, local index), elements ascending
+    std::vector<int32_t> cnt(static_cast<size_t>(nn), 0);
+    for (int32_t v : c->h_conn) cnt[size_t(v)]++;
+    for (int32_t n = 0; n < nn; ++n)
+      UFC_REQUIRE(cnt[size_t(n)] <= ufc::kMaxNodeElems, UFC_STATUS_UNSUPPORTED,
+                  "node " + std::to_string(n + base) + " belongs to more than 8 elements");
+    std::vector<int32_t> s_elem(static_cast<size_t>(nn) * 8, -1);
+    std::vector<uint8_t> s_loc(static_cast<size_t>(nn) * 8, 0);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int32_t e = 0; e < ne; ++e)
+      for (int a = 0; a < 8; ++a) {
+        const int32_t n = c->h_conn[8 * size_t(e) + a];
+        const int s = cnt[size_t(n)]++;
+        s_elem[8 * size_t(n) + s] = e;
+        s_loc[8 * size_t(n) + s] = uint8_t(a);
+      }
+    // neighbour lists (sorted, unique)
+    std::vector<int32_t> deg(static_cast<size_t>(nn));
+    std::vector<int32_t> nbtmp(static_cast<size_t>(nn) * ufc::kMaxNodeNbrs);
+    std::vector<int32_t> bad(1, -1);
+    parallel_for(nn, [&](int64_t n) {
+      int32_t* L = nbtmp.data() + n * ufc::kMaxNodeNbrs;
+      int32_t buf[64];
+      int k = 0;
+      for (int s = 0; s < 8; ++s) {
+        const int32_t e = s_elem[8 * size_t(n) + s];
+        if (e < 0) break;
+        for (int a = 0; a < 8; ++a) buf[k++] = c->h_conn[8 * size_t(e) + a];
+      }
+      std::sort(buf, buf + k);
+      const int u = int(std::unique(buf, buf + k) - buf);
+      deg[size_t(n)] = u;
+      if (u > ufc::kMaxNodeNbrs) {
+        bad[0] = int32_t(n);
+        return;
+      }
+      std::copy(buf, buf + u, L);
+    });
+    UFC_REQUIRE(bad[0] < 0, UFC_STATUS_UNSUPPORTED,
+                "node " + std::to_string(bad[0] + base) + " has more than 32 neighbours");
+    c->h_nb_ptr.assign(size_t(nn) + 1, 0);
+    int32_t maxdeg = 0;
+    for (int32_t n = 0; n < nn; ++n) {
+      UFC_REQUIRE(deg[size_t(n)] > 0, UFC_STATUS_INVALID,
+                  "node " + std::to_string(n + base) + " belongs to no element");
+      c->h_nb_ptr[size_t(n) + 1] = c->h_nb_ptr[size_t(n)] + deg[size_t(n)];
+      maxdeg = std::max(maxdeg, deg[size_t(n)]);
+    }
+    const int64_t npairs = c->h_nb_ptr[size_t(nn)];
+    const int64_t nnz = 9 * npairs;
+    UFC_REQUIRE(nnz < INT32_MAX, UFC_STATUS_UNSUPPORTED,
+                "nnz exceeds the int32 CSR of the reference (NM_Assem_Sparse.f90:45-56)");
+    std::vector<int32_t> nb_idx(static_cast<size_t>(npairs));
+    std::vector<int32_t> row_ptr(static_cast<size_t>(nn) * 3 + 1);
+    std::vector<int32_t> diag_pos(static_cast<size_t>(nn) * 3);
+    std::vector<uint64_t> s_code(static_cast<size_t>(nn) * 8, 0);
+    parallel_for(nn, [&](int64_t n) {
+      const int32_t q0 = c->h_nb_ptr[size_t(n)];
+      const int dg = deg[size_t(n)];
+      const int32_t* L = nbtmp.data() + n * ufc::kMaxNodeNbrs;
+      std::copy(L, L + dg, nb_idx.begin() + q0);
+      const int kself = int(std::lower_bound(L, L + dg, int32_t(n)) - L);
+      for (int d = 0; d < 3; ++d) {
+        row_ptr[3 * size_t(n) + d] = 9 * q0 + d * 3 * dg;
+        diag_pos[3 * size_t(n) + d] = 9 * q0 + d * 3 * dg + 3 * kself + d;
+      }
+      for (int s = 0; s < 8; ++s) {
+        const int32_t e = s_elem[8 * size_t(n) + s];
+        if (e < 0) break;
+        uint64_t code = uint64_t(s_loc[8 * size_t(n) + s]) << 40;
+        for (int b = 0; b < 8; ++b) {
+          const int32_t m2 = c->h_conn[8 * size_t(e) + b];
+          const uint64_t pos = uint64_t(std::lower_bound(L, L + dg, m2) - L);
+          code |= pos << (5 * b);
+        }
+        s_code[8 * size_t(n) + s] = code;
+      }
+    });
+    row_ptr[size_t(nn) * 3] = int32_t(nnz);
+
+    c->n_nodes = nn;
+    c->n_elems = ne;
+    c->n_dof = int64_t(nn) * 3;
+    c->nnz = nnz;
+    c->max_row_nnz = 3 * maxdeg;
+    c->have64 = c->have32 = false;
+    c->h_f.assign(size_t(c->n_dof), 0.0);
+    c->h_bc_eq.clear();
+    c->h_bc_pen.clear();
+    c->h_bc_rhs.clear();
+    c->coords.resize(size_t(nn) * 3);
+    c->conn.resize(size_t(ne) * 8);
+    c->nb_ptr.resize(size_t(nn) + 1);
+    c->nb_idx.resize(size_t(npairs));
+    c->row_ptr.resize(size_t(nn) * 3 + 1);
+    c->diag_pos.resize(size_t(nn) * 3);
+    c->slot_elem.resize(size_t(nn) * 8);
+    c->slot_code.resize(size_t(nn) * 8);
+    auto up = [&](void* d, const void* h, size_t bytes) {
+      UFC_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    up(c->coords.get(), c->h_coords.data(), c->coords.bytes());
+    up(c->conn.get(), c->h_conn.data(), c->conn.bytes());
+    up(c->nb_ptr.get(), c->h_nb_ptr.data(), c->nb_ptr.bytes());
+    up(c->nb_idx.get(), nb_idx.data(), c->nb_idx.bytes());
+    up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
+    up(c->diag_pos.get(), diag_pos.data(), c->diag_pos.bytes());
+    up(c->slot_elem.get(), s_elem.data(), c->slot_elem.bytes());
+    up(c->slot_code.get(), s_code.data(), c->slot_code.bytes());
+    c->col_idx.release();
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    upload_rhs(c);
+    if (info) {
+      info->n_dof = c->n_dof;
+      info->nnz = c->nnz;
+      info->n_nodes = nn;
+      info->n_elems = ne;
+      info->max_row_nnz = c->max_row_nnz;
+      info->reserved = 0;
+    }
+    msg = "pattern built: nnz " + std::to_string(nnz);
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_get_pattern(ufc_ctx* c, int32_t index_base, int32_t* row_ptr, int32_t* col_idx,
+                           ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_get_pattern", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(row_ptr || col_idx, UFC_STATUS_INVALID, "null argument");
+    UFC_REQUIRE(index_base == 0 || index_base == 1, UFC_STATUS_INVALID, "index_base must be 0 or 1");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    if (!c->col_idx.get()) ufc::build_col_idx(c);
+    if (row_ptr) {
+      std::vector<int32_t> h(static_cast<size_t>(c->n_dof) + 1);
+      UFC_HIP_CHECK(hipMemcpyAsync(h.data(), c->row_ptr.get(), h.size() * 4,
+                                   hipMemcpyDeviceToHost, c->stream));
+      UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (auto& v : h) v += index_base;
+      UFC_HIP_CHECK(hipMemcpy(row_ptr, h.data(), h.size() * 4, hipMemcpyDefault));
+    }
+    if (col_idx) {
+      std::vector<int32_t> h(static_cast<size_t>(c->nnz));
+      UFC_HIP_CHECK(hipMemcpyAsync(h.data(), c->col_idx.get(), h.size() * 4,
+                                   hipMemcpyDeviceToHost, c->stream));
+      UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+      if (index_base)
+        parallel_for(int64_t(h.size()), [&](int64_t i) { h[size_t(i)] += 1; });
+      UFC_HIP_CHECK(hipMemcpy(col_idx, h.data(), h.size() * 4, hipMemcpyDefault));
+    }
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_assemble_k(ufc_ctx* c, const ufc_rt_asm_algo* algo, const ufc_rt_asm_args* a,
+                          ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_assemble_k", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(algo != nullptr, UFC_STATUS_INVALID, "algo is null");
+    const int prec = algo->precision;
+    UFC_REQUIRE(prec == UFC_PREC_FP64 || prec == UFC_PREC_FP32 || prec == UFC_PREC_MIXED,
+                UFC_STATUS_INVALID, "unknown precision");
+    UFC_REQUIRE(algo->material == UFC_MAT_ELASTIC || algo->material == UFC_MAT_J2,
+                UFC_STATUS_INVALID, "unknown material");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    if (algo->material == UFC_MAT_ELASTIC) {
+      UFC_REQUIRE(a && a->young, UFC_STATUS_INVALID, "elastic assembly needs young[]");
+      UFC_REQUIRE(a->nu > -1.0 && a->nu < 0.5, UFC_STATUS_INVALID, "nu out of (-1, 0.5)");
+      c->young.resize(size_t(c->n_elems));
+      UFC_HIP_CHECK(hipMemcpyAsync(c->young.get(), a->young, c->young.bytes(), hipMemcpyDefault,
+                                   c->stream));
+      c->nu = a->nu;
+      c->have_young = true;
+    } else {
+      UFC_REQUIRE(c->have_j2, UFC_STATUS_INVALID, "J2 assembly needs ufc_ph_j2_init/update");
+    }
+    const bool fp32 = prec != UFC_PREC_FP64;
+    UFC_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+    if (fp32) {
+      c->val32.resize(size_t(c->nnz));
+      c->diag32.resize(size_t(c->n_dof));
+    } else {
+      c->val64.resize(size_t(c->nnz));
+      c->diag64.resize(size_t(c->n_dof));
+    }
+    ufc::launch_assemble(c, fp32 ? UFC_PREC_FP32 : UFC_PREC_FP64, algo->material);
+    ufc::launch_penalty(c, fp32, +1.0);
+    ufc::launch_diag(c, fp32);
+    UFC_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    UFC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->assemble_ms = ms;
+    if (fp32) {
+      c->have32 = true;
+      c->have64 = false;  // a later FP64 solve must not use stale values
+    } else {
+      c->have64 = true;
+      c->have32 = false;
+    }
+    c->last_material = algo->material;
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_get_values(ufc_ctx* c, int32_t precision, void* values, ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_get_values", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(values != nullptr, UFC_STATUS_INVALID, "values is null");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    if (precision == UFC_PREC_FP64) {
+      UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
+      copy_out(c, static_cast<double*>(values), c->val64.get(), size_t(c->nnz));
+    } else {
+      UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
+      copy_out(c, static_cast<float*>(values), c->val32.get(), size_t(c->nnz));
+    }
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_apply_bc(ufc_ctx* c, const ufc_rt_bc_args* a, ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_apply_bc", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(a != nullptr, UFC_STATUS_INVALID, "args is null");
+    UFC_REQUIRE(a->n_bc >= 0 && (a->n_bc == 0 || a->eq), UFC_STATUS_INVALID, "bad BC list");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    // remove the previous set from assembled values
+    if (c->have64) ufc::launch_penalty(c, false, -1.0);
+    if (c->have32) ufc::launch_penalty(c, true, -1.0);
+    std::map<int32_t, std::pair<double, double>> acc;  // eq -> (penalty sum, rhs sum)
+    std::vector<int32_t> heq(static_cast<size_t>(a->n_bc));
+    std::vector<double> hval(static_cast<size_t>(a->n_bc), 0.0);
+    if (a->n_bc) {
+      UFC_HIP_CHECK(hipMemcpy(heq.data(), a->eq, heq.size() * 4, hipMemcpyDefault));
+      if (a->value)
+        UFC_HIP_CHECK(hipMemcpy(hval.data(), a->value, hval.size() * 8, hipMemcpyDefault));
+    }
+    for (int32_t i = 0; i < a->n_bc; ++i) {
+      const int32_t q = heq[size_t(i)] - a->index_base;
+      UFC_REQUIRE(q >= 0 && q < c->n_dof, UFC_STATUS_INVALID,
+                  "BC equation out of range: " + std::to_string(heq[size_t(i)]));
+      auto& pr = acc[q];
+      pr.first += a->penalty;
+      pr.second += a->penalty * hval[size_t(i)];
+    }
+    c->h_bc_eq.clear();
+    c->h_bc_pen.clear();
+    c->h_bc_rhs.clear();
+    for (auto& kv : acc) {
+      c->h_bc_eq.push_back(kv.first);
+      c->h_bc_pen.push_back(kv.second.first);
+      c->h_bc_rhs.push_back(kv.second.second);
+    }
+    c->bc_eq.resize(std::max<size_t>(1, c->h_bc_eq.size()));
+    c->bc_pen.resize(std::max<size_t>(1, c->h_bc_eq.size()));
+    if (!c->h_bc_eq.empty()) {
+      UFC_HIP_CHECK(hipMemcpyAsync(c->bc_eq.get(), c->h_bc_eq.data(), c->h_bc_eq.size() * 4,
+                                   hipMemcpyHostToDevice, c->stream));
+      UFC_HIP_CHECK(hipMemcpyAsync(c->bc_pen.get(), c->h_bc_pen.data(), c->h_bc_pen.size() * 8,
+                                   hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->have64) {
+      ufc::launch_penalty(c, false, +1.0);
+      ufc::launch_diag(c, false);
+    }
+    if (c->have32) {
+      ufc::launch_penalty(c, true, +1.0);
+      ufc::launch_diag(c, true);
+    }
+    upload_rhs(c);
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_load(ufc_ctx* c, const ufc_rt_load_args* a, ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_load", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(a != nullptr, UFC_STATUS_INVALID, "args is null");
+    UFC_REQUIRE(a->kind == UFC_LOAD_TRACTION || a->kind == UFC_LOAD_PRESSURE, UFC_STATUS_INVALID,
+                "unknown load kind");
+    UFC_REQUIRE(a->n_faces >= 0 && (a->n_faces == 0 || (a->elem && a->face)), UFC_STATUS_INVALID,
+                "bad face list");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    if (!a->accumulate) std::fill(c->h_f.begin(), c->h_f.end(), 0.0);
+    std::vector<int32_t> he(static_cast<size_t>(a->n_faces)), hf(size_t(a->n_faces));
+    if (a->n_faces) {
+      UFC_HIP_CHECK(hipMemcpy(he.data(), a->elem, he.size() * 4, hipMemcpyDefault));
+      UFC_HIP_CHECK(hipMemcpy(hf.data(), a->face, hf.size() * 4, hipMemcpyDefault));
+    }
+    for (int32_t i = 0; i < a->n_faces; ++i) {
+      const int32_t e = he[size_t(i)] - a->index_base;
+      const int f = hf[size_t(i)] - a->index_base;
+      UFC_REQUIRE(e >= 0 && e < c->n_elems, UFC_STATUS_INVALID, "load element out of range");
+      UFC_REQUIRE(f >= 0 && f < 6, UFC_STATUS_INVALID, "face id out of range");
+      double x[8][3];
+      for (int q = 0; q < 8; ++q)
+        for (int d = 0; d < 3; ++d)
+          x[q][d] = c->h_coords[3 * size_t(c->h_conn[8 * size_t(e) + q]) + d];
+      if (a->kind == UFC_LOAD_TRACTION) {
+        // LoadBC_DistributeLoad_ToSurface: traction * area / 4 per face node
+        double area = 0.0;
+        face_quadrature(x, f, [&](const double*, double dA, const double*) { area += dA; });
+        for (int q = 0; q < 4; ++q) {
+          const size_t n = size_t(c->h_conn[8 * size_t(e) + kFaceNodes[f][q]]);
+          for (int d = 0; d < 3; ++d) c->h_f[3 * n + d] += a->traction[d] * (area / 4.0);
+        }
+      } else {
+        // PH_Elem_C3D8_FormFacePressure: sum over points of N p n dA
+        double fe[4][3] = {};
+        face_quadrature(x, f, [&](const double* Nf, double dA, const double* n) {
+          for (int q = 0; q < 4; ++q)
+            for (int d = 0; d < 3; ++d) fe[q][d] += Nf[q] * a->magnitude * n[d] * dA;
+        });
+        for (int q = 0; q < 4; ++q) {
+          const size_t n = size_t(c->h_conn[8 * size_t(e) + kFaceNodes[f][q]]);
+          for (int d = 0; d < 3; ++d) c->h_f[3 * n + d] += fe[q][d];
+        }
+      }
+    }
+    upload_rhs(c);
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_set_rhs(ufc_ctx* c, const double* f, ufc_status* st) {
+  return guarded(st, "ufc_rt_set_rhs", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(f != nullptr, UFC_STATUS_INVALID, "f is null");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    UFC_HIP_CHECK(hipMemcpy(c->h_f.data(), f, c->h_f.size() * 8, hipMemcpyDefault));
+    upload_rhs(c);
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_get_rhs(ufc_ctx* c, double* f, ufc_status* st) {
+  return guarded(st, "ufc_rt_get_rhs", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(f != nullptr, UFC_STATUS_INVALID, "f is null");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    copy_out(c, f, c->f.get(), size_t(c->n_dof));
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_residual(ufc_ctx* c, const double* u, double* r, double* norms, ufc_status* st) {
+  return guarded(st, "ufc_rt_residual", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(u != nullptr, UFC_STATUS_INVALID, "u is null");
+    UFC_REQUIRE(c->have_j2 || c->have_young, UFC_STATUS_INVALID,
+                "no material: assemble or update J2 first");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    ufc::ensure_solver_buffers(c, true, false);
+    ufc::DBuf<double> tu;
+    const double* du = dev_in(c, u, size_t(c->n_dof), tu);
+    if (c->have_j2)
+      ufc::launch_apply_elem64(c, 2, nullptr, c->w_t.get());
+    else
+      ufc::launch_apply_elem64(c, 0, du, c->w_t.get());
+    ufc::residual64(c, c->f.get(), c->w_t.get(), du, c->w_r.get());
+    if (norms) {
+      norms[0] = ufc::norm2_64(c, c->w_r.get());
+      norms[1] = ufc::norm2_64(c, c->w_t.get());
+    }
+    if (r) copy_out(c, r, c->w_r.get(), size_t(c->n_dof));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return UFC_STATUS_OK;
+  });
+}
+
+// ======================================================================= L2_NM
+int ufc_nm_spmv(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_status* st) {
+  return guarded(st, "ufc_nm_spmv", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(x && y, UFC_STATUS_INVALID, "null argument");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    ufc::ensure_solver_buffers(c, true, precision != UFC_PREC_FP64);
+    ufc::DBuf<double> tx;
+    const double* dx = dev_in(c, x, size_t(c->n_dof), tx);
+    if (precision == UFC_PREC_FP64) {
+      UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
+      if (!c->col_idx.get()) ufc::build_col_idx(c);
+      ufc::spmv64(c, dx, c->w_t.get());
+      copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
+    } else {
+      UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
+      ufc::cast_f64_to_f32(c, dx, c->s_x.get());
+      ufc::spmv32(c, c->s_x.get(), c->s_ap.get());
+      ufc::cast_f32_to_f64(c, c->s_ap.get(), c->w_t.get());
+      copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
+    }
+    c->spmv_ev_used = 0;
+    return UFC_STATUS_OK;
+  });
+}
+
+static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* b_user,
+                      ufc_nm_solver_state* s, std::string& msg) {
+  UFC_REQUIRE(algo && s && s->x, UFC_STATUS_INVALID, "null argument");
+  UFC_REQUIRE(algo->max_iter >= 0, UFC_STATUS_INVALID, "max_iter < 0");
+  UFC_HIP_CHECK(hipSetDevice(c->device));
+  const int prec = algo->precision;
+  const size_t n = size_t(c->n_dof);
+  ufc::ensure_solver_buffers(c, true, prec != UFC_PREC_FP64);
+  c->spmv_ev_used = 0;
+  // right-hand side into w_b
+  const double* b = c->f.get();
+  if (b_user) {
+    UFC_HIP_CHECK(hipMemcpyAsync(c->w_b.get(), b_user, n * 8, hipMemcpyDefault, c->stream));
+    b = c->w_b.get();
+  }
+  UFC_HIP_CHECK(hipEventRecord(c->ev2, c->stream));
+  int code = UFC_STATUS_OK;
+  int niter = 0, nouter = 0;
+  double rnorm = 0.0;
+  if (prec == UFC_PREC_FP64) {
+    UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
+    if (!c->col_idx.get()) ufc::build_col_idx(c);
+    if (!s->x0_zero)
+      UFC_HIP_CHECK(hipMemcpyAsync(c->w_x.get(), s->x, n * 8, hipMemcpyDefault, c->stream));
+    code = ufc::pcg64(c, b, c->w_x.get(), s->x0_zero != 0, algo->max_iter, algo->rtol,
+                      algo->check_every, &niter, &rnorm);
+  } else if (prec == UFC_PREC_FP32) {
+    UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
+    UFC_REQUIRE(s->x0_zero, UFC_STATUS_UNSUPPORTED, "FP32 solve starts from x0 = 0");
+    code = ufc::pcg32(c, b, c->s_x.get(), algo->max_iter, algo->rtol, algo->check_every, &niter,
+                      &rnorm);
+    ufc::cast_f32_to_f64(c, c->s_x.get(), c->w_x.get());
+  } else {
+    // FP32 inner PCG in FP64 iterative refinement:
+    //   r = b - K u (FP64, element by element), d = PCG32(K32, r), u += d
+    UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
+    UFC_REQUIRE(c->last_material == UFC_MAT_J2 ? c->have_j2 : c->have_young, UFC_STATUS_INVALID,
+                "no material for the FP64 residual");
+    double* u = c->w_x.get();
+    if (s->x0_zero)
+      UFC_HIP_CHECK(hipMemsetAsync(u, 0, n * 8, c->stream));
+    else
+      UFC_HIP_CHECK(hipMemcpyAsync(u, s->x, n * 8, hipMemcpyDefault, c->stream));
+    const double bnorm0 = ufc::norm2_64(c, b);
+    const double bnorm = bnorm0 < 1.0e-30 ? 1.0 : bnorm0;
+    const int max_outer = algo->max_outer > 0 ? algo->max_outer : 20;
+    const double inner = algo->inner_rtol > 0.0 ? algo->inner_rtol : 1.0e-5;
+    double* r = c->w_r.get();
+    code = UFC_STATUS_INVALID;
+    for (int k = 0; k <= max_outer; ++k) {
+      apply_fp64_operator(c, u, c->w_t.get());
+      ufc::residual64(c, b, c->w_t.get(), u, r);
+      rnorm = ufc::norm2_64(c, r) / bnorm;
+      if (!std::isfinite(rnorm)) {
+        code = UFC_STATUS_ERROR;
+        break;
+      }
+      if (rnorm < algo->rtol) {
+        code = UFC_STATUS_OK;
+        break;
+      }
+      if (k == max_outer) break;
+      int it = 0;
+      double rin = 0.0;
+      const int ic = ufc::pcg32(c, r, c->s_x.get(), algo->max_iter, inner, algo->check_every,
+                                &it, &rin);
+      niter += it;
+      ++nouter;
+      if (ic == UFC_STATUS_ERROR) {
+        code = UFC_STATUS_ERROR;
+        break;
+      }
+      ufc::axpy_f32_to_f64(c, c->s_x.get(), u);
+    }
+  }
+  UFC_HIP_CHECK(hipEventRecord(c->ev3, c->stream));
+  UFC_HIP_CHECK(hipMemcpyAsync(s->x, c->w_x.get(), n * 8, hipMemcpyDefault, c->stream));
+  UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  UFC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev2, c->ev3));
+  c->solve_ms = ms;
+  c->spmv_ms_avg = ufc::spmv_event_avg_ms(c, &c->spmv_calls);
+  s->niter = niter;
+  s->n_outer = nouter;
+  s->rnorm = rnorm;
+  s->converged = code == UFC_STATUS_OK;
+  if (code == UFC_STATUS_OK)
+    msg = "converged";
+  else if (code == UFC_STATUS_ERROR)
+    msg = "breakdown (p.Ap <= 0 or non-finite)";
+  else
+    msg = "[NM_Solver_PCG]: did not converge";
+  return code;
+}
+
+int ufc_nm_pcg(ufc_ctx* c, const ufc_nm_solver_algo* algo, ufc_nm_solver_state* s,
+               ufc_status* st) {
+  return guarded(st, "ufc_nm_pcg", [&](std::string& msg) {
+    check_mesh(c);
+    return solve_impl(c, algo, s ? s->b : nullptr, s, msg);
+  });
+}
+
+int ufc_nm_adjoint_solve(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* dJdu,
+                         ufc_nm_solver_state* s, ufc_status* st) {
+  return guarded(st, "ufc_nm_adjoint_solve", [&](std::string& msg) {
+    check_mesh(c);
+    UFC_REQUIRE(dJdu != nullptr, UFC_STATUS_INVALID, "dJdu is null");
+    UFC_REQUIRE(s != nullptr, UFC_STATUS_INVALID, "state is null");
+    // lambda_0 = 0 as in NM_CG_Solve_Transpose (NM_Solv_GMRESTranspose.f90:163-166)
+    const int32_t keep = s->x0_zero;
+    s->x0_zero = 1;
+    const int code = solve_impl(c, algo, dJdu, s, msg);
+    s->x0_zero = keep;
+    return code;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+// ufc_internal.hpp -- context layout and kernel-launch interfaces shared by
+// the HIP translation units of libufc_hip.so (not installed, not part of the
+// C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ufc_hip.h"
+
+namespace ufc {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define UFC_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::ufc::Error{e_ == hipErrorOutOfMemory ? UFC_STATUS_MEM_ERROR : UFC_STATUS_ERROR, \
+                         std::string(#expr) + ": " + hipGetErrorString(e_)};            \
+  } while (0)
+
+#define UFC_REQUIRE(cond, code, text)                    \
+  do {                                                   \
+    if (!(cond)) throw ::ufc::Error{(code), (text)};     \
+  } while (0)
+
+// ---------------------------------------------------------------- device buffer
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void resize(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    UFC_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    n = count;
+  }
+  T* get() const { return p; }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Layout of the scalars of one PCG run, kept on the device so that the
+// iteration never waits for the host.
+struct PcgScalars {
+  double rz;      // r.z of the current iterate
+  double pap;     // p.Ap
+  double alpha;
+  double beta;
+  double rr;      // r.r
+  double bnorm;   // ||b||
+  double rnorm;   // ||r|| / ||b||
+  double rtol;
+  int32_t iter;   // completed iterations
+  int32_t done;   // 0 running, 1 converged, 2 breakdown, 3 max_iter
+  int32_t max_iter;
+  int32_t pad;
+  uint32_t ticket[4];
+};
+
+constexpr int kSolveBlocks = 2048;  // 8 workgroups per CU on 256 CUs
+constexpr int kSolveThreads = 256;  // 4 wave64 per workgroup
+constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
+constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
+
+}  // namespace ufc
+
+// ---------------------------------------------------------------- context
+struct ufc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // mesh
+  int32_t n_nodes = 0, n_elems = 0;
+  int64_t n_dof = 0;
+  std::vector<double> h_coords;  // (n_nodes, 3)
+  std::vector<int32_t> h_conn;   // (n_elems, 8) 0-based
+  ufc::DBuf<double> coords;
+  ufc::DBuf<int32_t> conn;
+
+  // pattern: node adjacency (the node-block index view of the CSR) + CSR
+  int64_t nnz = 0;
+  int32_t max_row_nnz = 0;
+  std::vector<int32_t> h_nb_ptr;  // (n_nodes + 1)
+  ufc::DBuf<int32_t> nb_ptr;      // (n_nodes + 1)
+  ufc::DBuf<int32_t> nb_idx;      // (nnz / 9)
+  ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)
+  ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
+  ufc::DBuf<int32_t> diag_pos;    // (n_dof) position of K(i,i) in values
+  // element-to-slot map, node major: for node n and slot s < 8, the element
+  // (ascending, -1 if none) and a code: bits 5b..5b+4 = position in n's
+  // neighbour list of the element's local node b, bits 40..42 = n's local index
+  ufc::DBuf<int32_t> slot_elem;   // (n_nodes * 8)
+  ufc::DBuf<uint64_t> slot_code;  // (n_nodes * 8)
+
+  // values
+  ufc::DBuf<double> val64;
+  ufc::DBuf<float> val32;
+  ufc::DBuf<double> diag64;
+  ufc::DBuf<float> diag32;
+  bool have64 = false, have32 = false;
+  int last_material = UFC_MAT_ELASTIC;
+
+  // material of the last elastic assembly
+  ufc::DBuf<double> young;
+  double nu = 0.3;
+  bool have_young = false;
+
+  // boundary conditions (unique equations, summed penalties)
+  std::vector<int32_t> h_bc_eq;
+  std::vector<double> h_bc_pen;
+  std::vector<double> h_bc_rhs;  // penalty * value
+  ufc::DBuf<int32_t> bc_eq;
+  ufc::DBuf<double> bc_pen;
+
+  // load vector (FP64): F without penalty terms, and the penalty part
+  std::vector<double> h_f;
+  ufc::DBuf<double> f;  // F + P ubar
+
+  // J2 state
+  bool have_j2 = false;
+  ufc_ph_j2_props j2{};
+  ufc::DBuf<double> j2_commit;   // (13, 8, n_elems)
+  ufc::DBuf<double> j2_trial;    // (13, 8, n_elems)
+  ufc::DBuf<double> j2_tangent;  // (36, 8, n_elems)
+  ufc::DBuf<int32_t> j2_flags;   // (8, n_elems) 1 = plastic branch, 2 = failure
+
+  // solver work (FP64)
+  ufc::DBuf<double> w_r, w_p, w_ap, w_x, w_b;
+  // solver work (FP32)
+  ufc::DBuf<float> s_r, s_p, s_ap, s_x;
+  // element force scratch (24 per element) and a spare FP64 vector
+  ufc::DBuf<double> elem_w;
+  ufc::DBuf<double> w_t;
+  // reductions
+  ufc::DBuf<double> partials;            // kSolveBlocks * 4
+  ufc::DBuf<ufc::PcgScalars> scalars;    // 1
+  ufc::PcgScalars* h_scalars = nullptr;  // pinned mirror
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  bool per_kernel = false;
+  std::vector<hipEvent_t> spmv_ev;  // pairs
+  size_t spmv_ev_used = 0;
+  double assemble_ms = 0.0, solve_ms = 0.0, spmv_ms_avg = 0.0;
+  int32_t spmv_calls = 0;
+};
+
+namespace ufc {
+
+// ---------------------------------------------------------------- launchers
+// element kernels (ufc_elem.hip)
+void launch_eval_ke(int32_t n_elems, const double* coords, const double* young, double nu,
+                    void* ke, bool fp32, hipStream_t s);
+void launch_assemble(ufc_ctx* c, int precision, int material);
+void launch_diag(ufc_ctx* c, bool fp32);
+void launch_penalty(ufc_ctx* c, bool fp32, double sign);
+// y = K u in FP64 without penalties: element forces then node gather.
+// mode 0: elastic with c->young; mode 1: J2 tangent D_gp; mode 2: J2 trial stress (u unused)
+void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y);
+void launch_sensitivity(ufc_ctx* c, const double* u, const double* lam, const double* dedrho,
+                        double nu, double* grad);
+void launch_j2_update(ufc_ctx* c, const double* u, const double* u_n);
+
+// solver kernels (ufc_solve.hip)
+void spmv64(ufc_ctx* c, const double* x, double* y);
+void spmv32(ufc_ctx* c, const float* x, float* y);
+// r = b - y - pen*u (penalty rows); returns nothing, norms via pcg scalars
+void residual64(ufc_ctx* c, const double* b, const double* y, const double* u, double* r);
+double norm2_64(ufc_ctx* c, const double* v);
+int pcg64(ufc_ctx* c, const double* b, double* x, bool x0_zero, int max_iter, double rtol,
+          int check_every, int* niter, double* rnorm);
+int pcg32(ufc_ctx* c, const double* b64, float* x, int max_iter, double rtol, int check_every,
+          int* niter, double* rnorm);
+void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u);
+void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o);
+void cast_f32_to_f64(ufc_ctx* c, const float* a, double* o);
+void build_col_idx(ufc_ctx* c);
+double spmv_event_avg_ms(ufc_ctx* c, int* calls);
+void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32);
+
+}  // namespace ufc
