@@ -45,6 +45,14 @@ def _oracle_system(orc, pr, penalty=1e12):
     return rp, ci, val, F, f0
 
 
+def _difference_form(orc, pr, rp, ci, penalty=1e12):
+    val = orc.assemble(pr.conn, pr.coords, pr.young, pr.nu, rp, ci)
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    val[(ci // 3) == (rows // 3)] = 0.0
+    orc.apply_penalty(pr.bc_eq, pr.bc_val, penalty, rp, ci, val, None)
+    return val
+
+
 def test_ke_batch_fp64_fp32(ctx, orc):
     rng = np.random.default_rng(0)
     n = 200
@@ -92,7 +100,17 @@ def test_assembly_fp64_fp32_vs_oracle(ctx, orc, kind):
     assert np.array_equal(ctx.get_values(0), v64)
     ctx.assemble_k(pr.young, pr.nu, precision=1)
     v32 = ctx.get_values(1).astype(np.float64)
-    assert np.abs(v32 - val_o).max() <= 1e-6 * scale
+    # FP32 values are kept in difference form: self 3x3 blocks hold only the
+    # penalty diagonal (every other block as assembled, rounded to FP32)
+    expect = _difference_form(orc, pr, rp_o, ci_o)
+    assert np.abs(v32 - expect).max() <= 1e-6 * scale
+    # and the block rows of the FP64 K sum to zero (what makes that exact)
+    node_of_row = np.repeat(np.arange(pr.coords.shape[0]), 3)
+    rows = np.repeat(np.arange(rp_o.size - 1), np.diff(rp_o))
+    v_nopen = orc.assemble(pr.conn, pr.coords, pr.young, pr.nu, rp_o, ci_o)
+    rowsum = np.zeros((rp_o.size - 1, 3))
+    np.add.at(rowsum, (rows, ci_o % 3), v_nopen)
+    assert np.abs(rowsum).max() <= 1e-12 * scale
 
 
 def test_spmv_fp64_fp32(ctx, orc):

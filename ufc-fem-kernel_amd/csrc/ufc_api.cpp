@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <thread>
 
@@ -156,6 +157,14 @@ void upload_rhs(ufc_ctx* c) {
   UFC_HIP_CHECK(hipMemcpyAsync(c->f.get(), h.data(), h.size() * sizeof(double),
                                hipMemcpyHostToDevice, c->stream));
   UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+bool verbose() {
+  static const bool v = [] {
+    const char* e = std::getenv("UFC_VERBOSE");
+    return e && e[0] && e[0] != '0';
+  }();
+  return v;
 }
 
 // FP64 operator for refinement residuals: y = K u (no penalties), matrix-free
@@ -601,13 +610,18 @@ int ufc_rt_asm_assemble_k(ufc_ctx* c, const ufc_rt_asm_algo* algo, const ufc_rt_
     if (fp32) {
       c->val32.resize(size_t(c->nnz));
       c->diag32.resize(size_t(c->n_dof));
+      c->kdiag32.resize(size_t(c->n_dof));
     } else {
       c->val64.resize(size_t(c->nnz));
       c->diag64.resize(size_t(c->n_dof));
     }
     ufc::launch_assemble(c, fp32 ? UFC_PREC_FP32 : UFC_PREC_FP64, algo->material);
-    ufc::launch_penalty(c, fp32, +1.0);
-    ufc::launch_diag(c, fp32);
+    if (fp32) {
+      ufc::launch_fp32_difference_form(c);
+    } else {
+      ufc::launch_penalty(c, false, +1.0);
+      ufc::launch_diag(c, false);
+    }
     UFC_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
     UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
     float ms = 0.f;
@@ -688,7 +702,7 @@ int ufc_rt_asm_apply_bc(ufc_ctx* c, const ufc_rt_bc_args* a, ufc_status* st) {
     }
     if (c->have32) {
       ufc::launch_penalty(c, true, +1.0);
-      ufc::launch_diag(c, true);
+      ufc::launch_diag32_penalty(c);
     }
     upload_rhs(c);
     return UFC_STATUS_OK;
@@ -812,7 +826,7 @@ int ufc_nm_spmv(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_s
       ufc::cast_f32_to_f64(c, c->s_ap.get(), c->w_t.get());
       copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
     }
-    c->spmv_ev_used = 0;
+    ufc::reset_spmv_events(c);
     return UFC_STATUS_OK;
   });
 }
@@ -825,7 +839,7 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
   const int prec = algo->precision;
   const size_t n = size_t(c->n_dof);
   ufc::ensure_solver_buffers(c, true, prec != UFC_PREC_FP64);
-  c->spmv_ev_used = 0;
+  ufc::reset_spmv_events(c);
   // right-hand side into w_b
   const double* b = c->f.get();
   if (b_user) {
@@ -846,12 +860,13 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
   } else if (prec == UFC_PREC_FP32) {
     UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
     UFC_REQUIRE(s->x0_zero, UFC_STATUS_UNSUPPORTED, "FP32 solve starts from x0 = 0");
-    code = ufc::pcg32(c, b, c->s_x.get(), algo->max_iter, algo->rtol, algo->check_every, &niter,
-                      &rnorm);
-    ufc::cast_f32_to_f64(c, c->s_x.get(), c->w_x.get());
+    code = ufc::pcg32(c, b, c->w_x.get(), true, false, algo->max_iter, algo->rtol,
+                      algo->check_every, &niter, &rnorm);
   } else {
     // FP32 inner PCG in FP64 iterative refinement:
     //   r = b - K u (FP64, element by element), d = PCG32(K32, r), u += d
+    // The inner iteration accumulates d into u in FP64: an FP32 d would carry
+    // rounding noise that K amplifies to ~eps32 * cond(K) in the next residual.
     UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
     UFC_REQUIRE(c->last_material == UFC_MAT_J2 ? c->have_j2 : c->have_young, UFC_STATUS_INVALID,
                 "no material for the FP64 residual");
@@ -879,17 +894,31 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
         break;
       }
       if (k == max_outer) break;
+      // Inner tolerance of this step: inner_rtol, loosened to 0.1 rtol/||r||
+      // for a step that can finish the solve (no inner iterations spent below
+      // what the outer target needs).  Tightening below inner_rtol does not
+      // pay: an FP32 inner solve contracts the FP64 residual by ~1e-5 at best
+      // on these problems, whatever its own tolerance.
+      double tol_k = inner;
+      const double loose = 0.1 * algo->rtol / rnorm;
+      if (loose > inner) tol_k = std::min(loose, 0.1);
       int it = 0;
       double rin = 0.0;
-      const int ic = ufc::pcg32(c, r, c->s_x.get(), algo->max_iter, inner, algo->check_every,
+      // d = PCG32(K32, r) from d = 0, accumulated straight into u (u += d in
+      // FP64).  Each step restarts the inner Krylov space: continuing the old
+      // search direction after a replacement of this size (the FP64 residual
+      // sits ~10x above the inner one) stalls CG.
+      const int ic = ufc::pcg32(c, r, u, false, false, algo->max_iter, tol_k, algo->check_every,
                                 &it, &rin);
+      if (verbose())
+        std::fprintf(stderr, "[ufc_hip] refinement %d: ||r||/||b|| %.3e, inner tol %.2e, %d it\n",
+                     k, rnorm, tol_k, it);
       niter += it;
       ++nouter;
       if (ic == UFC_STATUS_ERROR) {
         code = UFC_STATUS_ERROR;
         break;
       }
-      ufc::axpy_f32_to_f64(c, c->s_x.get(), u);
     }
   }
   UFC_HIP_CHECK(hipEventRecord(c->ev3, c->stream));

@@ -273,6 +273,28 @@ __global__ void k_diag(int64_t n, const int32_t* __restrict__ pos, const T* __re
   if (i < n) diag[i] = val[pos[i]];
 }
 
+// FP32 difference form: zero the 3x3 self block of every node (the SpMV then
+// applies sum_{m != n} K_nm (x_m - x_n) plus the penalty diagonal stored in
+// the self block)
+__global__ void k_zero_self(int32_t n_nodes, const int32_t* __restrict__ nb_ptr,
+                            const int32_t* __restrict__ diag_pos, float* __restrict__ val) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  const int d3 = 3 * (nb_ptr[n + 1] - nb_ptr[n]);
+  const int64_t base = diag_pos[3 * n];  // self block entry (0, 0)
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) val[base + d * d3 + c] = 0.f;
+}
+
+// diag[eq[i]] += pen[i]
+__global__ void k_penalty_vec(int32_t n_bc, const int32_t* __restrict__ eq,
+                              const double* __restrict__ pen, float* __restrict__ diag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_bc) diag[eq[i]] = float(double(diag[eq[i]]) + pen[i]);
+}
+
 template <typename T>
 __global__ void k_penalty(int32_t n_bc, const int32_t* __restrict__ eq,
                           const double* __restrict__ pen, const int32_t* __restrict__ pos,
@@ -615,6 +637,27 @@ void launch_diag(ufc_ctx* c, bool fp32) {
   else
     hipLaunchKernelGGL(k_diag<double>, dim3(grid), dim3(256), 0, c->stream, c->n_dof,
                        c->diag_pos.get(), c->val64.get(), c->diag64.get());
+  UFC_HIP_CHECK(hipGetLastError());
+}
+
+void launch_fp32_difference_form(ufc_ctx* c) {
+  // kdiag32 = diagonal of the assembled FP32 K (no penalties)
+  hipLaunchKernelGGL(k_diag<float>, dim3(blocks_for(c->n_dof, 256)), dim3(256), 0, c->stream,
+                     c->n_dof, c->diag_pos.get(), c->val32.get(), c->kdiag32.get());
+  hipLaunchKernelGGL(k_zero_self, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
+                     c->n_nodes, c->nb_ptr.get(), c->diag_pos.get(), c->val32.get());
+  UFC_HIP_CHECK(hipGetLastError());
+  launch_penalty(c, true, +1.0);
+  launch_diag32_penalty(c);
+}
+
+void launch_diag32_penalty(ufc_ctx* c) {
+  UFC_HIP_CHECK(hipMemcpyAsync(c->diag32.get(), c->kdiag32.get(), c->kdiag32.bytes(),
+                               hipMemcpyDeviceToDevice, c->stream));
+  const int32_t nb = int32_t(c->h_bc_eq.size());
+  if (nb)
+    hipLaunchKernelGGL(k_penalty_vec, dim3(blocks_for(nb, 256)), dim3(256), 0, c->stream, nb,
+                       c->bc_eq.get(), c->bc_pen.get(), c->diag32.get());
   UFC_HIP_CHECK(hipGetLastError());
 }
 

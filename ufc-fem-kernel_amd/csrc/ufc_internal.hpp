@@ -73,11 +73,14 @@ struct PcgScalars {
   int32_t done;   // 0 running, 1 converged, 2 breakdown, 3 max_iter
   int32_t max_iter;
   int32_t pad;
-  uint32_t ticket[4];
 };
 
-constexpr int kSolveBlocks = 2048;  // 8 workgroups per CU on 256 CUs
-constexpr int kSolveThreads = 256;  // 4 wave64 per workgroup
+constexpr int kSolveBlocks = 2048;  // vector kernels: 8 workgroups per CU on 256 CUs
+constexpr int kSolveThreads = 256;  // vector kernels: 4 wave64 per workgroup
+constexpr int kSpmvThreads = 512;   // SpMV: 8 wave64 = 16 node groups per workgroup
+constexpr int kSpmvBlocks64 = 32768;  // FP64 CSR SpMV grid (measured best on gfx950)
+constexpr int kSpmvBlocks32 = 8192;   // FP32 SpMV grid
+constexpr int kMaxPartials = 32768;   // per-workgroup partial sums
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
 
@@ -115,7 +118,8 @@ struct ufc_ctx {
   ufc::DBuf<double> val64;
   ufc::DBuf<float> val32;
   ufc::DBuf<double> diag64;
-  ufc::DBuf<float> diag32;
+  ufc::DBuf<float> diag32;   // K(i,i) + penalty (Jacobi)
+  ufc::DBuf<float> kdiag32;  // K(i,i) of the FP32 assembly, no penalty
   bool have64 = false, have32 = false;
   int last_material = UFC_MAT_ELASTIC;
 
@@ -160,6 +164,7 @@ struct ufc_ctx {
   bool per_kernel = false;
   std::vector<hipEvent_t> spmv_ev;  // pairs
   size_t spmv_ev_used = 0;
+  std::vector<char> spmv_active;    // per pair: 1 = the launch did work
   double assemble_ms = 0.0, solve_ms = 0.0, spmv_ms_avg = 0.0;
   int32_t spmv_calls = 0;
 };
@@ -173,6 +178,10 @@ void launch_eval_ke(int32_t n_elems, const double* coords, const double* young, 
 void launch_assemble(ufc_ctx* c, int precision, int material);
 void launch_diag(ufc_ctx* c, bool fp32);
 void launch_penalty(ufc_ctx* c, bool fp32, double sign);
+// FP32 values -> difference form: K(i,i) saved to kdiag32, self blocks replaced
+// by the penalty diagonal, diag32 = kdiag32 + penalty
+void launch_fp32_difference_form(ufc_ctx* c);
+void launch_diag32_penalty(ufc_ctx* c);
 // y = K u in FP64 without penalties: element forces then node gather.
 // mode 0: elastic with c->young; mode 1: J2 tangent D_gp; mode 2: J2 trial stress (u unused)
 void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y);
@@ -188,13 +197,16 @@ void residual64(ufc_ctx* c, const double* b, const double* y, const double* u, d
 double norm2_64(ufc_ctx* c, const double* v);
 int pcg64(ufc_ctx* c, const double* b, double* x, bool x0_zero, int max_iter, double rtol,
           int check_every, int* niter, double* rnorm);
-int pcg32(ufc_ctx* c, const double* b64, float* x, int max_iter, double rtol, int check_every,
-          int* niter, double* rnorm);
+// FP32 PCG into the FP64 iterate x; resume = continue the previous FP32 CG
+// after a residual replacement (b is the new true residual)
+int pcg32(ufc_ctx* c, const double* b64, double* x, bool zero_x, bool resume, int max_iter,
+          double rtol, int check_every, int* niter, double* rnorm);
 void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u);
 void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o);
 void cast_f32_to_f64(ufc_ctx* c, const float* a, double* o);
 void build_col_idx(ufc_ctx* c);
 double spmv_event_avg_ms(ufc_ctx* c, int* calls);
+void reset_spmv_events(ufc_ctx* c);
 void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32);
 
 }  // namespace ufc

@@ -7,49 +7,64 @@
 // with ||r|| / ||b|| < rtol (||b|| < 1e-30 -> 1), z = r / diag,
 // beta = (r.z)_new / (r.z)_old, p = z + beta p.
 //
-// Each iteration is three kernels, all scalars live on the device:
-//   spmv_dot : Ap = K p and p.Ap; the last workgroup forms alpha
-//   update   : x += alpha p, r -= alpha Ap, r.r and r.(r/diag); the last
-//              workgroup forms ||r||/||b||, decides convergence, forms beta
-//   p_update : p = r/diag + beta p
-// Reductions are deterministic: every workgroup writes its partial sum, the
-// last one to arrive (ticket counter) adds the partials in index order.
-// Once `done` is set every later kernel returns at entry, so the host only
-// polls every few iterations and the stopping iteration is still exact.
+// One iteration is five launches; every scalar stays on the device:
+//   spmv_dot  : Ap = K p and per-workgroup partials of p.Ap
+//   fin_alpha : one workgroup adds the partials in index order, alpha
+//   update    : x += alpha p, r -= alpha Ap, partials of r.r and r.(r/diag)
+//   fin_beta  : ||r||/||b||, convergence / breakdown / max_iter, beta
+//   p_update  : p = r/diag + beta p
+// The ordered two-level sums make every dot product bit-reproducible.  Once
+// `done` is set every later kernel returns at entry, so the host polls only
+// every few iterations while the stopping iteration stays exact.
+// (A last-workgroup-arrives reduction inside the producing kernel needs an
+// agent-scope release per workgroup, which on gfx950 writes back the XCD's
+// L2 and costs ~55 us per kernel; the separate one-workgroup finalize costs
+// ~4 us.)
 //
-// SpMV: one wave64 per node, i.e. per group of 3 CSR rows whose values are
-// one contiguous segment (node n's rows start at 9*nb_ptr[n]); lanes stride
-// the segment (coalesced), keep one partial per row and reduce with shuffles.
-//   FP64: scalar CSR (values + col_idx), the reference's storage.
-//   FP32: node-block index view of the same CSR -- the column of value
-//         3*deg*d + 3*k + c is 3*nb_idx[nb_ptr[n]+k] + c, so one 4-byte index
-//         per 9 values replaces col_idx.
+// SpMV layout: node n's three CSR rows are one contiguous segment of 9*deg
+// values starting at 9*nb_ptr[n]; inside it, row d (0..2) holds the columns
+// 3*nb_idx[nb_ptr[n]+k] + c for neighbours k < deg (ascending) and c = 0..2.
+// A group of 32 lanes takes one node, lane k one neighbour k: it reads its
+// three 3-value runs (one per row, contiguous across the lanes, so every load
+// instruction is coalesced), gathers the neighbour's 3 entries of x and
+// keeps three row partials; the group reduces them with shuffles.
+//   FP64: plain CSR -- every value's column comes from col_idx, as
+//         NM_CSR_MatVec_Optimized (NM_Mtx_Sparse.f90:341-365) reads it.
+//   FP32: node-block index view of the same CSR -- one 4-byte neighbour id per
+//         9 values replaces col_idx, and x is gathered once per neighbour --
+//         applied in difference form.  Every element matrix annihilates rigid
+//         translations, so each block row of K sums to zero and
+//             (K x)_n = sum_{m != n} K_nm (x_m - x_n)   (+ P_n x_n on penalty rows).
+//         The self blocks of the FP32 values therefore hold only the penalty
+//         diagonal.  FP32 rounding then acts on x_m - x_n instead of x_m, so a
+//         smooth x no longer loses ~eps32 * cond(K) to cancellation: the
+//         refinement residual after one inner solve falls to the inner
+//         tolerance instead of stalling near 1e-3.
 #include "ufc_internal.hpp"
 
 namespace ufc {
 namespace {
 
 constexpr int WAVE = 64;
+constexpr int GROUP = 32;  // lanes per node
 constexpr int WAVES_PER_BLOCK = kSolveThreads / WAVE;
+constexpr int GROUPS_PER_BLOCK = kSpmvThreads / GROUP;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
-}
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+template <typename T>
+__device__ __forceinline__ T sum_lanes(T v, int width) {
+  for (int o = width / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
   return v;
 }
 
-// Sum of v over the workgroup (kSolveThreads threads), valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[WAVES_PER_BLOCK]) {
+// Sum of v[k] over a workgroup of BS threads, valid in thread 0 (fixed order).
+template <int NV, int BS>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  constexpr int NW = BS / WAVE;
+  __shared__ double sh[NV][NW];
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    v[k] = wave_sum(v[k]);
+    v[k] = sum_lanes(v[k], WAVE);
     if (lane == 0) sh[k][wid] = v[k];
   }
   __syncthreads();
@@ -58,202 +73,143 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[WAVES_PE
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
 #pragma unroll
-      for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += sh[k][w];
+      for (int w = 0; w < NW; ++w) s += sh[k][w];
       v[k] = s;
     }
   }
 }
 
-// Writes this workgroup's partials, then returns true in every thread of the
-// last workgroup to finish, with all partials of the grid visible to it.
-template <int NV>
-__device__ __forceinline__ bool publish_partials(const double (&v)[NV], double* partials,
-                                                 uint32_t* ticket) {
-  __shared__ bool last;
+// partial k of workgroup b lives at partials[k * kMaxPartials + b]
+template <int NV, int BS>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* partials) {
+  block_sum<NV, BS>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partials[k * kSolveBlocks + blockIdx.x] = v[k];
-    const uint32_t t =
-        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
+    for (int k = 0; k < NV; ++k) partials[k * kMaxPartials + blockIdx.x] = v[k];
   }
-  __syncthreads();
-  if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return last;
 }
 
-// In the last workgroup: ordered sum of the grid's partials (same order on
-// every run), valid in thread 0.  Resets the ticket.
+constexpr int kFinThreads = 1024;
+
+// one workgroup of kFinThreads: ordered sum of nblocks partials per value,
+// valid in thread 0
 template <int NV>
-__device__ __forceinline__ void reduce_partials(double (&out)[NV], const double* partials,
-                                                uint32_t* ticket,
-                                                double (*sh)[WAVES_PER_BLOCK]) {
-  double v[NV];
+__device__ __forceinline__ void sum_partials(double (&out)[NV], const double* partials,
+                                             int nblocks) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < int(gridDim.x); b += kSolveThreads)
-      s += partials[k * kSolveBlocks + b];
-    v[k] = s;
+    for (int b = threadIdx.x; b < nblocks; b += kFinThreads) s += partials[k * kMaxPartials + b];
+    out[k] = s;
   }
-  __syncthreads();
-  block_sum<NV>(v, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = v[k];
-    *ticket = 0u;
-  }
+  block_sum<NV, kFinThreads>(out);
 }
 
-// ---------------------------------------------------------------- SpMV bodies
-// y(3n..3n+2) for node n, FP64 scalar CSR; returns the row sums in lane 0..2 order
-__device__ __forceinline__ void node_rows64(int64_t n, const int32_t* __restrict__ row_ptr,
-                                            const int32_t* __restrict__ col_idx,
-                                            const double* __restrict__ val,
-                                            const double* __restrict__ x, double (&acc)[3]) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int32_t s0 = row_ptr[3 * n], s1 = row_ptr[3 * n + 1], s2 = row_ptr[3 * n + 2],
-                s3 = row_ptr[3 * n + 3];
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int32_t o = s0 + lane; o < s3; o += WAVE) {
-    const double v = val[o] * x[col_idx[o]];
-    if (o < s1) a0 += v;
-    else if (o < s2) a1 += v;
-    else a2 += v;
-  }
-  acc[0] = wave_sum(a0);
-  acc[1] = wave_sum(a1);
-  acc[2] = wave_sum(a2);
-}
-
-// FP32 node-block view; products and row sums in FP32
-__device__ __forceinline__ void node_rows32(int64_t n, const int32_t* __restrict__ nb_ptr,
-                                            const int32_t* __restrict__ nb_idx,
-                                            const float* __restrict__ val,
-                                            const float* __restrict__ x, float (&acc)[3]) {
-  const int lane = threadIdx.x & (WAVE - 1);
+// ---------------------------------------------------------------- SpMV body
+// Row sums of node n in a[0..2] (valid in every lane of the 32-lane group).
+template <typename T, bool BLOCK_INDEX>
+__device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __restrict__ nb_ptr,
+                                          const int32_t* __restrict__ nb_idx,
+                                          const int32_t* __restrict__ col_idx,
+                                          const T* __restrict__ val, const T* __restrict__ x,
+                                          T (&a)[3]) {
   const int32_t q0 = nb_ptr[n];
   const int deg = nb_ptr[n + 1] - q0;
-  const int d3 = 3 * deg;
-  const int len = 9 * deg;
-  const float* __restrict__ v = val + 9 * int64_t(q0);
-  const int32_t* __restrict__ nb = nb_idx + q0;
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-  for (int o = lane; o < len; o += WAVE) {
-    const int d = (o >= d3) + (o >= 2 * d3);
-    const int rem = o - d * d3;
-    const int k = rem / 3;
-    const int c = rem - 3 * k;
-    const float t = v[o] * x[3 * nb[k] + c];
-    if (d == 0) a0 += t;
-    else if (d == 1) a1 += t;
-    else a2 += t;
-  }
-  acc[0] = wave_sum(a0);
-  acc[1] = wave_sum(a1);
-  acc[2] = wave_sum(a2);
-}
-
-// ---------------------------------------------------------------- kernels
-// plain y = K x
-__global__ __launch_bounds__(kSolveThreads) void k_spmv64(int32_t n_nodes,
-                                                         const int32_t* __restrict__ row_ptr,
-                                                         const int32_t* __restrict__ col_idx,
-                                                         const double* __restrict__ val,
-                                                         const double* __restrict__ x,
-                                                         double* __restrict__ y) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = int64_t(blockIdx.x) * WAVES_PER_BLOCK + threadIdx.x / WAVE;
-  const int64_t nw = int64_t(gridDim.x) * WAVES_PER_BLOCK;
-  for (int64_t n = wave; n < n_nodes; n += nw) {
-    double acc[3];
-    node_rows64(n, row_ptr, col_idx, val, x, acc);
-    if (lane < 3) y[3 * n + lane] = acc[lane == 0 ? 0 : (lane == 1 ? 1 : 2)];
-  }
-}
-
-__global__ __launch_bounds__(kSolveThreads) void k_spmv32(int32_t n_nodes,
-                                                         const int32_t* __restrict__ nb_ptr,
-                                                         const int32_t* __restrict__ nb_idx,
-                                                         const float* __restrict__ val,
-                                                         const float* __restrict__ x,
-                                                         float* __restrict__ y) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = int64_t(blockIdx.x) * WAVES_PER_BLOCK + threadIdx.x / WAVE;
-  const int64_t nw = int64_t(gridDim.x) * WAVES_PER_BLOCK;
-  for (int64_t n = wave; n < n_nodes; n += nw) {
-    float acc[3];
-    node_rows32(n, nb_ptr, nb_idx, val, x, acc);
-    if (lane < 3) y[3 * n + lane] = acc[lane == 0 ? 0 : (lane == 1 ? 1 : 2)];
-  }
-}
-
-// Ap = K p, p.Ap; last workgroup: alpha = rz / pAp (breakdown if pAp <= 0 or not finite)
-template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void k_spmv_dot(
-    int32_t n_nodes, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col_idx,
-    const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
-    const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ ap,
-    double* __restrict__ partials, PcgScalars* __restrict__ sc) {
-  __shared__ double sh[1][WAVES_PER_BLOCK];
-  if (sc->done) return;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = int64_t(blockIdx.x) * WAVES_PER_BLOCK + threadIdx.x / WAVE;
-  const int64_t nw = int64_t(gridDim.x) * WAVES_PER_BLOCK;
-  double dot = 0.0;
-  for (int64_t n = wave; n < n_nodes; n += nw) {
-    T acc[3];
-    if constexpr (sizeof(T) == 8) {
-      double a[3];
-      node_rows64(n, row_ptr, col_idx, reinterpret_cast<const double*>(val),
-                  reinterpret_cast<const double*>(p), a);
-      acc[0] = a[0]; acc[1] = a[1]; acc[2] = a[2];
+  a[0] = a[1] = a[2] = T(0);
+  if (k < deg) {
+    const int64_t base = 9 * int64_t(q0) + 3 * k;
+    const int d3 = 3 * deg;
+    if constexpr (BLOCK_INDEX) {
+      // difference form: the self block holds the penalty diagonal and
+      // multiplies x_n; every other block multiplies x_m - x_n
+      const int64_t m = nb_idx[q0 + k];
+      T x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
+      if (m != n) {
+        x0 -= x[3 * n];
+        x1 -= x[3 * n + 1];
+        x2 -= x[3 * n + 2];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T* v = val + base + d * d3;
+        a[d] = v[0] * x0 + v[1] * x1 + v[2] * x2;
+      }
     } else {
-      float a[3];
-      node_rows32(n, nb_ptr, nb_idx, reinterpret_cast<const float*>(val),
-                  reinterpret_cast<const float*>(p), a);
-      acc[0] = a[0]; acc[1] = a[1]; acc[2] = a[2];
-    }
-    if (lane < 3) {
-      const T yv = acc[lane == 0 ? 0 : (lane == 1 ? 1 : 2)];
-      ap[3 * n + lane] = yv;
-      dot += double(yv) * double(p[3 * n + lane]);
-    }
-  }
-  double v[1] = {dot};
-  block_sum<1>(v, sh);
-  if (publish_partials<1>(v, partials, &sc->ticket[0])) {
-    double out[1];
-    reduce_partials<1>(out, partials, &sc->ticket[0], sh);
-    if (threadIdx.x == 0) {
-      const double pap = out[0];
-      sc->pap = pap;
-      if (!(pap > 0.0) || !isfinite(pap)) {
-        sc->done = 2;
-      } else {
-        sc->alpha = sc->rz / pap;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T* v = val + base + d * d3;
+        const int32_t* c = col_idx + base + d * d3;
+        a[d] = v[0] * x[c[0]] + v[1] * x[c[1]] + v[2] * x[c[2]];
       }
     }
   }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) a[d] = sum_lanes(a[d], GROUP);
 }
 
-// x += alpha p, r -= alpha Ap; r.r and r.z with z = r/diag
+// y = K x; with sc != null also per-workgroup partials of x.y (p.Ap in PCG)
+// and an early return once the solve is done
+template <typename T, bool BLOCK_INDEX>
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(
+    int32_t n_nodes, const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
+    const int32_t* __restrict__ col_idx, const T* __restrict__ val, const T* __restrict__ x,
+    T* __restrict__ y, double* __restrict__ partials, const PcgScalars* __restrict__ sc) {
+  if (sc && sc->done) return;
+  const int k = threadIdx.x & (GROUP - 1);
+  const int64_t g0 = int64_t(blockIdx.x) * GROUPS_PER_BLOCK + threadIdx.x / GROUP;
+  const int64_t ng = int64_t(gridDim.x) * GROUPS_PER_BLOCK;
+  double dot = 0.0;
+  for (int64_t n = g0; n < n_nodes; n += ng) {
+    T a[3];
+    node_rows<T, BLOCK_INDEX>(n, k, nb_ptr, nb_idx, col_idx, val, x, a);
+    if (k < 3) {
+      const T yv = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+      y[3 * n + k] = yv;
+      if (sc) dot += double(yv) * double(x[3 * n + k]);
+    }
+  }
+  if (sc) {
+    double v[1] = {dot};
+    write_partials<1, kSpmvThreads>(v, partials);
+  }
+}
+
+__global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restrict__ partials,
+                                                          int nblocks,
+                                                          PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  double s[1];
+  sum_partials<1>(s, partials, nblocks);
+  if (threadIdx.x == 0) {
+    const double pap = s[0];
+    sc->pap = pap;
+    if (!(pap > 0.0) || !isfinite(pap))
+      sc->done = 2;
+    else
+      sc->alpha = sc->rz / pap;
+  }
+}
+
+// x += alpha p, r -= alpha Ap; partials of r.r and r.z with z = r/diag.
+// The iterate x is FP64 in both solvers: an FP32 iterate carries rounding
+// noise of eps32 |x|, which K amplifies to eps32 * lambda_max |x| in the true
+// residual -- for the smooth fields of an elastic solve that stalls the FP64
+// refinement near eps32 * cond(K) instead of the inner tolerance.
 template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restrict__ x,
+__global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, double* __restrict__ x,
                                                          T* __restrict__ r,
                                                          const T* __restrict__ p,
                                                          const T* __restrict__ ap,
                                                          const T* __restrict__ diag,
                                                          double* __restrict__ partials,
-                                                         PcgScalars* __restrict__ sc) {
-  __shared__ double sh[2][WAVES_PER_BLOCK];
+                                                         const PcgScalars* __restrict__ sc) {
   if (sc->done) return;
   const double alpha = sc->alpha;
   const T a = T(alpha);
   double rr = 0.0, rz = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * kSolveThreads) {
-    x[i] = x[i] + a * p[i];
+    x[i] = x[i] + alpha * double(p[i]);
     const T ri = r[i] - a * ap[i];
     r[i] = ri;
     const T d = diag[i];
@@ -262,25 +218,28 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
     rz += double(ri) * double(zi);
   }
   double v[2] = {rr, rz};
-  block_sum<2>(v, sh);
-  if (publish_partials<2>(v, partials, &sc->ticket[1])) {
-    double out[2];
-    reduce_partials<2>(out, partials, &sc->ticket[1], sh);
-    if (threadIdx.x == 0) {
-      sc->rr = out[0];
-      sc->iter += 1;
-      const double rn = sqrt(out[0]) / sc->bnorm;
-      sc->rnorm = rn;
-      if (!isfinite(rn)) {
-        sc->done = 2;
-      } else if (rn < sc->rtol) {
-        sc->done = 1;
-      } else if (sc->iter >= sc->max_iter) {
-        sc->done = 3;
-      } else {
-        sc->beta = out[1] / sc->rz;
-        sc->rz = out[1];
-      }
+  write_partials<2, kSolveThreads>(v, partials);
+}
+
+__global__ __launch_bounds__(kFinThreads) void k_fin_beta(const double* __restrict__ partials,
+                                                           PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  double s[2];
+  sum_partials<2>(s, partials, kSolveBlocks);
+  if (threadIdx.x == 0) {
+    sc->rr = s[0];
+    sc->iter += 1;
+    const double rn = sqrt(s[0]) / sc->bnorm;
+    sc->rnorm = rn;
+    if (!isfinite(rn)) {
+      sc->done = 2;
+    } else if (rn < sc->rtol) {
+      sc->done = 1;
+    } else if (sc->iter >= sc->max_iter) {
+      sc->done = 3;
+    } else {
+      sc->beta = s[1] / sc->rz;
+      sc->rz = s[1];
     }
   }
 }
@@ -302,17 +261,15 @@ __global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restr
   }
 }
 
-// Setup: r = b - Ax (Ax may be null: x0 = 0), z = r/diag, p = z, x cast in;
-// r.z and b.b; last workgroup initialises the scalars.
+// r = b - Ax (ax null: x0 = 0), p = z = r/diag (p kept when keep_p);
+// partials of r.z and b.b
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void k_setup(int64_t n, const double* __restrict__ b,
                                                         const T* __restrict__ ax,
                                                         T* __restrict__ r, T* __restrict__ p,
                                                         const T* __restrict__ diag,
                                                         double* __restrict__ partials,
-                                                        PcgScalars* __restrict__ sc, double rtol,
-                                                        int32_t max_iter) {
-  __shared__ double sh[2][WAVES_PER_BLOCK];
+                                                        int keep_p) {
   double rz = 0.0, bb = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * kSolveThreads) {
@@ -321,46 +278,54 @@ __global__ __launch_bounds__(kSolveThreads) void k_setup(int64_t n, const double
     r[i] = ri;
     const T d = diag[i];
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
-    p[i] = zi;
+    if (!keep_p) p[i] = zi;
     rz += double(ri) * double(zi);
     bb += bi * bi;
   }
   double v[2] = {rz, bb};
-  block_sum<2>(v, sh);
-  if (publish_partials<2>(v, partials, &sc->ticket[2])) {
-    double out[2];
-    reduce_partials<2>(out, partials, &sc->ticket[2], sh);
-    if (threadIdx.x == 0) {
-      sc->rz = out[0];
-      double bn = sqrt(out[1]);
-      if (bn < 1.0e-30) bn = 1.0;
-      sc->bnorm = bn;
-      sc->rtol = rtol;
-      sc->max_iter = max_iter;
-      sc->iter = 0;
-      sc->done = max_iter > 0 ? 0 : 3;
-      sc->rnorm = 0.0;
-      sc->alpha = sc->beta = sc->pap = sc->rr = 0.0;
-    }
+  write_partials<2, kSolveThreads>(v, partials);
+}
+
+// replace != 0: residual replacement in a running CG -- the new r.z enters
+// beta = (r.z)_new / (r.z)_old for p = z + beta p_old (k_pupdate follows)
+__global__ __launch_bounds__(kFinThreads) void k_fin_setup(const double* __restrict__ partials,
+                                                            PcgScalars* __restrict__ sc,
+                                                            double rtol, int32_t max_iter,
+                                                            int replace) {
+  double s[2];
+  sum_partials<2>(s, partials, kSolveBlocks);
+  if (threadIdx.x == 0) {
+    const double rz_old = sc->rz;
+    sc->rz = s[0];
+    double bn = sqrt(s[1]);
+    if (bn < 1.0e-30) bn = 1.0;
+    sc->bnorm = bn;
+    sc->rtol = rtol;
+    sc->max_iter = max_iter;
+    sc->iter = 0;
+    sc->done = max_iter > 0 ? 0 : 3;
+    sc->rnorm = 0.0;
+    sc->alpha = sc->pap = sc->rr = 0.0;
+    sc->beta = (replace && rz_old > 0.0 && isfinite(rz_old)) ? s[0] / rz_old : 0.0;
   }
 }
 
-// v.v with the deterministic two-level reduction; result in sc->rr
-__global__ __launch_bounds__(kSolveThreads) void k_norm2(int64_t n, const double* __restrict__ v,
-                                                        double* __restrict__ partials,
-                                                        PcgScalars* __restrict__ sc) {
-  __shared__ double sh[1][WAVES_PER_BLOCK];
+// v.v, ordered two-level sum; result in sc->rr
+__global__ __launch_bounds__(kSolveThreads) void k_sumsq(int64_t n, const double* __restrict__ v,
+                                                        double* __restrict__ partials) {
   double s = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * kSolveThreads)
     s += v[i] * v[i];
   double a[1] = {s};
-  block_sum<1>(a, sh);
-  if (publish_partials<1>(a, partials, &sc->ticket[3])) {
-    double out[1];
-    reduce_partials<1>(out, partials, &sc->ticket[3], sh);
-    if (threadIdx.x == 0) sc->rr = out[0];
-  }
+  write_partials<1, kSolveThreads>(a, partials);
+}
+
+__global__ __launch_bounds__(kFinThreads) void k_fin_sumsq(const double* __restrict__ partials,
+                                                            PcgScalars* __restrict__ sc) {
+  double s[1];
+  sum_partials<1>(s, partials, kSolveBlocks);
+  if (threadIdx.x == 0) sc->rr = s[0];
 }
 
 __global__ void k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
@@ -414,6 +379,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_expand_cols(int32_t n_nodes,
 
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
+// SpMV launches are bracketed by event pairs when per-kernel timing is on;
+// `active` marks the pairs whose launch did work (not a post-convergence no-op).
 void spmv_event_begin(ufc_ctx* c) {
   if (!c->per_kernel) return;
   if (c->spmv_ev_used + 2 > c->spmv_ev.size()) {
@@ -429,6 +396,7 @@ void spmv_event_end(ufc_ctx* c) {
   if (!c->per_kernel) return;
   UFC_HIP_CHECK(hipEventRecord(c->spmv_ev[c->spmv_ev_used + 1], c->stream));
   c->spmv_ev_used += 2;
+  c->spmv_active.push_back(1);
 }
 
 void poll(ufc_ctx* c) {
@@ -437,29 +405,54 @@ void poll(ufc_ctx* c) {
   UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+// SpMV grid: up to kSpmvBlocks{64,32} workgroups, never more than the nodes need
 template <typename T>
-int pcg_run(ufc_ctx* c, const double* b, T* x, const T* ax_or_null, T* r, T* p, T* ap,
+int spmv_blocks(const ufc_ctx* c) {
+  const int cap = sizeof(T) == 8 ? kSpmvBlocks64 : kSpmvBlocks32;
+  const int64_t need = (int64_t(c->n_nodes) + GROUPS_PER_BLOCK - 1) / GROUPS_PER_BLOCK;
+  return int(need < cap ? need : cap);
+}
+
+template <typename T, bool BLOCK_INDEX>
+void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool with_dot) {
+  hipLaunchKernelGGL((k_spmv<T, BLOCK_INDEX>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads), 0,
+                     c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(), c->col_idx.get(),
+                     val, x, y, c->partials.get(), with_dot ? c->scalars.get() : nullptr);
+}
+
+// resume: continue a CG after a residual replacement -- r := b (the new true
+// residual), p := z + beta p keeps the search direction (and with it the
+// Krylov information) instead of restarting from p = z
+template <typename T, bool BLOCK_INDEX>
+int pcg_run(ufc_ctx* c, const double* b, double* x, const T* ax_or_null, T* r, T* p, T* ap,
             const T* diag, const T* val, int max_iter, double rtol, int check_every,
-            int* niter, double* rnorm) {
+            int* niter, double* rnorm, bool resume = false) {
   const int64_t n = c->n_dof;
-  const unsigned vgrid = kSolveBlocks;
-  hipLaunchKernelGGL(k_setup<T>, dim3(vgrid), dim3(kSolveThreads), 0, c->stream, n, b,
-                     ax_or_null, r, p, diag, c->partials.get(), c->scalars.get(), rtol,
-                     int32_t(max_iter));
+  double* part = c->partials.get();
+  PcgScalars* sc = c->scalars.get();
+  hipLaunchKernelGGL(k_setup<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, b,
+                     ax_or_null, r, p, diag, part, int(resume));
+  hipLaunchKernelGGL(k_fin_setup, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc, rtol,
+                     int32_t(max_iter), int(resume));
+  if (resume)
+    hipLaunchKernelGGL(k_pupdate<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, p,
+                       r, diag, sc);
+  const int sblocks = spmv_blocks<T>(c);
   UFC_HIP_CHECK(hipGetLastError());
   const int every = check_every > 0 ? check_every : 32;
+  const size_t first_event = c->spmv_active.size();
   int launched = 0;
   for (;;) {
     for (int k = 0; k < every && launched < max_iter; ++k, ++launched) {
       spmv_event_begin(c);
-      hipLaunchKernelGGL(k_spmv_dot<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
-                         c->n_nodes, c->row_ptr.get(), c->col_idx.get(), c->nb_ptr.get(),
-                         c->nb_idx.get(), val, p, ap, c->partials.get(), c->scalars.get());
+      launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
       spmv_event_end(c);
-      hipLaunchKernelGGL(k_update<T>, dim3(vgrid), dim3(kSolveThreads), 0, c->stream, n, x, r, p,
-                         ap, diag, c->partials.get(), c->scalars.get());
-      hipLaunchKernelGGL(k_pupdate<T>, dim3(vgrid), dim3(kSolveThreads), 0, c->stream, n, p, r,
-                         diag, c->scalars.get());
+      hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part, sblocks, sc);
+      hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
+                         x, r, p, ap, diag, part, sc);
+      hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc);
+      hipLaunchKernelGGL(k_pupdate<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
+                         p, r, diag, sc);
     }
     UFC_HIP_CHECK(hipGetLastError());
     poll(c);
@@ -467,6 +460,9 @@ int pcg_run(ufc_ctx* c, const double* b, T* x, const T* ax_or_null, T* r, T* p, 
   }
   *niter = c->h_scalars->iter;
   *rnorm = c->h_scalars->rnorm;
+  // SpMV launches after the converging iteration returned at entry
+  for (size_t i = first_event + size_t(*niter); i < c->spmv_active.size(); ++i)
+    c->spmv_active[i] = 0;
   const int done = c->h_scalars->done;
   if (done == 1) return UFC_STATUS_OK;
   if (done == 2) return UFC_STATUS_ERROR;
@@ -478,25 +474,21 @@ int pcg_run(ufc_ctx* c, const double* b, T* x, const T* ax_or_null, T* r, T* p, 
 // ---------------------------------------------------------------- host entry points
 void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32) {
   const size_t n = size_t(c->n_dof);
+  c->w_r.resize(n);
+  c->w_x.resize(n);
+  c->w_b.resize(n);
+  c->w_t.resize(n);
   if (need64) {
-    c->w_r.resize(n);
     c->w_p.resize(n);
     c->w_ap.resize(n);
-    c->w_x.resize(n);
-    c->w_b.resize(n);
-    c->w_t.resize(n);
   }
   if (need32) {
     c->s_r.resize(n);
     c->s_p.resize(n);
     c->s_ap.resize(n);
     c->s_x.resize(n);
-    c->w_b.resize(n);
-    c->w_t.resize(n);
-    c->w_r.resize(n);
-    c->w_x.resize(n);
   }
-  c->partials.resize(size_t(kSolveBlocks) * 4);
+  c->partials.resize(size_t(kMaxPartials) * 2);
   if (!c->scalars.get()) {
     c->scalars.resize(1);
     UFC_HIP_CHECK(hipMemsetAsync(c->scalars.get(), 0, sizeof(PcgScalars), c->stream));
@@ -512,16 +504,14 @@ void build_col_idx(ufc_ctx* c) {
 
 void spmv64(ufc_ctx* c, const double* x, double* y) {
   spmv_event_begin(c);
-  hipLaunchKernelGGL(k_spmv64, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
-                     c->n_nodes, c->row_ptr.get(), c->col_idx.get(), c->val64.get(), x, y);
+  launch_spmv<double, false>(c, c->val64.get(), x, y, false);
   spmv_event_end(c);
   UFC_HIP_CHECK(hipGetLastError());
 }
 
 void spmv32(ufc_ctx* c, const float* x, float* y) {
   spmv_event_begin(c);
-  hipLaunchKernelGGL(k_spmv32, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
-                     c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), x, y);
+  launch_spmv<float, true>(c, c->val32.get(), x, y, false);
   spmv_event_end(c);
   UFC_HIP_CHECK(hipGetLastError());
 }
@@ -537,8 +527,10 @@ void residual64(ufc_ctx* c, const double* b, const double* y, const double* u, d
 }
 
 double norm2_64(ufc_ctx* c, const double* v) {
-  hipLaunchKernelGGL(k_norm2, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, c->n_dof, v,
-                     c->partials.get(), c->scalars.get());
+  hipLaunchKernelGGL(k_sumsq, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, c->n_dof, v,
+                     c->partials.get());
+  hipLaunchKernelGGL(k_fin_sumsq, dim3(1), dim3(kFinThreads), 0, c->stream, c->partials.get(),
+                     c->scalars.get());
   UFC_HIP_CHECK(hipGetLastError());
   poll(c);
   return std::sqrt(c->h_scalars->rr);
@@ -553,17 +545,19 @@ int pcg64(ufc_ctx* c, const double* b, double* x, bool x0_zero, int max_iter, do
     spmv64(c, x, c->w_ap.get());
     ax = c->w_ap.get();
   }
-  return pcg_run<double>(c, b, x, ax, c->w_r.get(), c->w_p.get(), c->w_ap.get(),
-                         c->diag64.get(), c->val64.get(), max_iter, rtol, check_every, niter,
-                         rnorm);
+  return pcg_run<double, false>(c, b, x, ax, c->w_r.get(), c->w_p.get(), c->w_ap.get(),
+                                c->diag64.get(), c->val64.get(), max_iter, rtol, check_every,
+                                niter, rnorm);
 }
 
-int pcg32(ufc_ctx* c, const double* b64, float* x, int max_iter, double rtol, int check_every,
-          int* niter, double* rnorm) {
-  UFC_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(float) * size_t(c->n_dof), c->stream));
-  return pcg_run<float>(c, b64, x, nullptr, c->s_r.get(), c->s_p.get(), c->s_ap.get(),
-                        c->diag32.get(), c->val32.get(), max_iter, rtol, check_every, niter,
-                        rnorm);
+int pcg32(ufc_ctx* c, const double* b64, double* x, bool zero_x, bool resume, int max_iter,
+          double rtol, int check_every, int* niter, double* rnorm) {
+  // the iteration starts from 0; without zero_x the correction accumulates
+  // into the caller's x (the refinement's u += d)
+  if (zero_x) UFC_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * size_t(c->n_dof), c->stream));
+  return pcg_run<float, true>(c, b64, x, nullptr, c->s_r.get(), c->s_p.get(), c->s_ap.get(),
+                              c->diag32.get(), c->val32.get(), max_iter, rtol, check_every,
+                              niter, rnorm, resume);
 }
 
 void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o) {
@@ -582,6 +576,7 @@ double spmv_event_avg_ms(ufc_ctx* c, int* calls) {
   double tot = 0.0;
   int k = 0;
   for (size_t i = 0; i + 1 < c->spmv_ev_used; i += 2) {
+    if (!c->spmv_active[i / 2]) continue;
     float ms = 0.f;
     UFC_HIP_CHECK(hipEventElapsedTime(&ms, c->spmv_ev[i], c->spmv_ev[i + 1]));
     tot += ms;
@@ -589,6 +584,11 @@ double spmv_event_avg_ms(ufc_ctx* c, int* calls) {
   }
   *calls = k;
   return k ? tot / k : 0.0;
+}
+
+void reset_spmv_events(ufc_ctx* c) {
+  c->spmv_ev_used = 0;
+  c->spmv_active.clear();
 }
 
 void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u) {

@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libufc_hip.so")
+LIB_PATH = os.environ.get("UFC_HIP_LIB") or os.path.join(PKG_DIR, "libufc_hip.so")
 
 # status codes: IF_STATUS_* (L1_IF/Error/IF_Err_Brg.f90:95-109)
 STATUS_OK = 0
