@@ -1,0 +1,248 @@
+! ufc_hip_iface.f90 -- ISO_C_BINDING interface to libufc_hip.so (include/ufc_hip.h).
+!
+! The Fortran side of the drop-in boundary: BIND(C) mirrors of the C call
+! bundles and one interface per C entry point, so UFC's step driver calls the
+! MI355X path with its own (desc, state, algo, ctx, args) shape.  Paths in the
+! citations are relative to the reference root:
+!   ufc_status            <-> ErrorStatusType      (L1_IF/Error/IF_Err_Def.f90:121-136)
+!   UFC_STATUS_*          <-> IF_STATUS_*          (L1_IF/Error/IF_Err_Brg.f90:95-109)
+!   ufc_nm_solver_algo    <-> NM_Solver_Algo       (L2_NM/Solver/NM_Solv_Def.f90:86-100)
+!   ufc_nm_solver_state   <-> NM_Solver_State      (L2_NM/Solver/NM_Solv_Def.f90:102-120)
+! Arrays pass as C_LOC of contiguous Fortran arrays; index arrays are 1-based
+! (index_base = 1).
+MODULE ufc_hip_iface
+  USE, INTRINSIC :: ISO_C_BINDING
+  IMPLICIT NONE
+  PRIVATE
+
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_STATUS_OK = 0, UFC_STATUS_INVALID = 1, &
+       UFC_STATUS_ERROR = 2, UFC_STATUS_CONVERGED = 3, UFC_STATUS_NOT_CONVERGED = 4, &
+       UFC_STATUS_WARN = 6, UFC_STATUS_MEM_ERROR = 7, UFC_STATUS_FATAL = 10, &
+       UFC_STATUS_UNSUPPORTED = 11
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_PREC_FP64 = 0, UFC_PREC_FP32 = 1, &
+       UFC_PREC_MIXED = 2
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_MAT_ELASTIC = 0, UFC_MAT_J2 = 1
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_LOAD_TRACTION = 0, UFC_LOAD_PRESSURE = 1
+
+  TYPE, BIND(C), PUBLIC :: ufc_status
+    INTEGER(C_INT32_T) :: status_code = 0
+    INTEGER(C_INT32_T) :: severity = 0
+    INTEGER(C_INT32_T) :: category = 0
+    INTEGER(C_INT32_T) :: line_number = 0
+    CHARACTER(KIND=C_CHAR) :: message(512) = C_NULL_CHAR
+    CHARACTER(KIND=C_CHAR) :: source(64) = C_NULL_CHAR
+  END TYPE ufc_status
+
+  TYPE, BIND(C), PUBLIC :: ufc_mesh_desc
+    INTEGER(C_INT32_T) :: n_nodes = 0
+    INTEGER(C_INT32_T) :: n_elems = 0
+    INTEGER(C_INT32_T) :: index_base = 1
+    INTEGER(C_INT32_T) :: reserved = 0
+    TYPE(C_PTR) :: coords = C_NULL_PTR
+    TYPE(C_PTR) :: conn = C_NULL_PTR
+  END TYPE ufc_mesh_desc
+
+  TYPE, BIND(C), PUBLIC :: ufc_rt_asm_info
+    INTEGER(C_INT64_T) :: n_dof = 0
+    INTEGER(C_INT64_T) :: nnz = 0
+    INTEGER(C_INT32_T) :: n_nodes = 0
+    INTEGER(C_INT32_T) :: n_elems = 0
+    INTEGER(C_INT32_T) :: max_row_nnz = 0
+    INTEGER(C_INT32_T) :: reserved = 0
+  END TYPE ufc_rt_asm_info
+
+  TYPE, BIND(C), PUBLIC :: ufc_rt_asm_algo
+    INTEGER(C_INT32_T) :: precision = UFC_PREC_FP64
+    INTEGER(C_INT32_T) :: material = UFC_MAT_ELASTIC
+  END TYPE ufc_rt_asm_algo
+
+  TYPE, BIND(C), PUBLIC :: ufc_rt_asm_args
+    TYPE(C_PTR) :: young = C_NULL_PTR
+    REAL(C_DOUBLE) :: nu = 0.3_C_DOUBLE
+  END TYPE ufc_rt_asm_args
+
+  TYPE, BIND(C), PUBLIC :: ufc_rt_bc_args
+    INTEGER(C_INT32_T) :: n_bc = 0
+    INTEGER(C_INT32_T) :: index_base = 1
+    TYPE(C_PTR) :: eq = C_NULL_PTR
+    TYPE(C_PTR) :: value = C_NULL_PTR
+    REAL(C_DOUBLE) :: penalty = 1.0E12_C_DOUBLE
+  END TYPE ufc_rt_bc_args
+
+  TYPE, BIND(C), PUBLIC :: ufc_rt_load_args
+    INTEGER(C_INT32_T) :: n_faces = 0
+    INTEGER(C_INT32_T) :: kind = UFC_LOAD_TRACTION
+    INTEGER(C_INT32_T) :: index_base = 1
+    INTEGER(C_INT32_T) :: accumulate = 0
+    TYPE(C_PTR) :: elem = C_NULL_PTR
+    TYPE(C_PTR) :: face = C_NULL_PTR
+    REAL(C_DOUBLE) :: magnitude = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: traction(3) = 0.0_C_DOUBLE
+  END TYPE ufc_rt_load_args
+
+  TYPE, BIND(C), PUBLIC :: ufc_nm_solver_algo
+    INTEGER(C_INT32_T) :: precision = UFC_PREC_FP64
+    INTEGER(C_INT32_T) :: max_iter = 100000
+    REAL(C_DOUBLE) :: rtol = 1.0E-10_C_DOUBLE
+    REAL(C_DOUBLE) :: inner_rtol = 1.0E-5_C_DOUBLE
+    INTEGER(C_INT32_T) :: max_outer = 20
+    INTEGER(C_INT32_T) :: check_every = 0
+  END TYPE ufc_nm_solver_algo
+
+  TYPE, BIND(C), PUBLIC :: ufc_nm_solver_state
+    TYPE(C_PTR) :: x = C_NULL_PTR
+    TYPE(C_PTR) :: b = C_NULL_PTR
+    INTEGER(C_INT32_T) :: niter = 0
+    INTEGER(C_INT32_T) :: n_outer = 0
+    REAL(C_DOUBLE) :: rnorm = 0.0_C_DOUBLE
+    INTEGER(C_INT32_T) :: converged = 0
+    INTEGER(C_INT32_T) :: x0_zero = 1
+  END TYPE ufc_nm_solver_state
+
+  TYPE, BIND(C), PUBLIC :: ufc_ph_j2_props
+    REAL(C_DOUBLE) :: young = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: nu = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: sigma_y0 = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: hardening = 0.0_C_DOUBLE
+  END TYPE ufc_ph_j2_props
+
+  TYPE, BIND(C), PUBLIC :: ufc_timing
+    REAL(C_DOUBLE) :: assemble_ms = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: solve_ms = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: spmv_ms_avg = 0.0_C_DOUBLE
+    INTEGER(C_INT32_T) :: spmv_calls = 0
+    INTEGER(C_INT32_T) :: reserved = 0
+  END TYPE ufc_timing
+
+  PUBLIC :: ufc_ctx_create, ufc_ctx_destroy, ufc_ctx_timing, ufc_rt_asm_build_pattern, &
+            ufc_rt_asm_get_pattern, ufc_rt_asm_assemble_k, ufc_rt_asm_apply_bc, &
+            ufc_rt_asm_load, ufc_rt_get_rhs, ufc_rt_residual, ufc_nm_spmv, ufc_nm_pcg, &
+            ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_status_message
+
+  INTERFACE
+    INTEGER(C_INT) FUNCTION ufc_ctx_create(device, ctx, st) BIND(C, NAME='ufc_ctx_create')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      INTEGER(C_INT32_T), VALUE :: device
+      TYPE(C_PTR), INTENT(OUT) :: ctx
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ctx_destroy(ctx, st) BIND(C, NAME='ufc_ctx_destroy')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ctx_timing(ctx, t, st) BIND(C, NAME='ufc_ctx_timing')
+      IMPORT :: C_INT, C_PTR, ufc_timing, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_timing), INTENT(INOUT) :: t
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_build_pattern(ctx, mesh, info, st) &
+        BIND(C, NAME='ufc_rt_asm_build_pattern')
+      IMPORT :: C_INT, C_PTR, ufc_mesh_desc, ufc_rt_asm_info, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_mesh_desc), INTENT(IN) :: mesh
+      TYPE(ufc_rt_asm_info), INTENT(INOUT) :: info
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_get_pattern(ctx, index_base, row_ptr, col_idx, st) &
+        BIND(C, NAME='ufc_rt_asm_get_pattern')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: index_base
+      TYPE(C_PTR), VALUE :: row_ptr, col_idx
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_assemble_k(ctx, algo, args, st) &
+        BIND(C, NAME='ufc_rt_asm_assemble_k')
+      IMPORT :: C_INT, C_PTR, ufc_rt_asm_algo, ufc_rt_asm_args, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_rt_asm_algo), INTENT(IN) :: algo
+      TYPE(ufc_rt_asm_args), INTENT(IN) :: args
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_apply_bc(ctx, args, st) BIND(C, NAME='ufc_rt_asm_apply_bc')
+      IMPORT :: C_INT, C_PTR, ufc_rt_bc_args, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_rt_bc_args), INTENT(IN) :: args
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_load(ctx, args, st) BIND(C, NAME='ufc_rt_asm_load')
+      IMPORT :: C_INT, C_PTR, ufc_rt_load_args, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_rt_load_args), INTENT(IN) :: args
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_get_rhs(ctx, f, st) BIND(C, NAME='ufc_rt_get_rhs')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx, f
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_residual(ctx, u, r, norms, st) BIND(C, NAME='ufc_rt_residual')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx, u, r, norms
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_nm_spmv(ctx, precision, x, y, st) BIND(C, NAME='ufc_nm_spmv')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: precision
+      TYPE(C_PTR), VALUE :: x, y
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_nm_pcg(ctx, algo, state, st) BIND(C, NAME='ufc_nm_pcg')
+      IMPORT :: C_INT, C_PTR, ufc_nm_solver_algo, ufc_nm_solver_state, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_nm_solver_algo), INTENT(IN) :: algo
+      TYPE(ufc_nm_solver_state), INTENT(INOUT) :: state
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_j2_init(ctx, props, st) BIND(C, NAME='ufc_ph_j2_init')
+      IMPORT :: C_INT, C_PTR, ufc_ph_j2_props, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_ph_j2_props), INTENT(IN) :: props
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_j2_update(ctx, u, u_n, n_yield, st) &
+        BIND(C, NAME='ufc_ph_j2_update')
+      IMPORT :: C_INT, C_INT64_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx, u, u_n
+      INTEGER(C_INT64_T), INTENT(OUT) :: n_yield
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_j2_commit(ctx, st) BIND(C, NAME='ufc_ph_j2_commit')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+  ! message(512) of a ufc_status as a Fortran string (ErrorStatusType%message)
+  FUNCTION ufc_status_message(st) RESULT(msg)
+    TYPE(ufc_status), INTENT(IN) :: st
+    CHARACTER(LEN=512) :: msg
+    INTEGER :: i
+    msg = ' '
+    DO i = 1, 512
+      IF (st%message(i) == C_NULL_CHAR) EXIT
+      msg(i:i) = st%message(i)
+    END DO
+  END FUNCTION ufc_status_message
+
+END MODULE ufc_hip_iface
