@@ -48,6 +48,7 @@ class StaticModel:
         self.n_dof = self.ctx.n_dof
         self.n_elems = self.ctx.n_elems
         self._key = None
+        self._young = None
         self.last_forward = None
         self.last_adjoint = None
 
@@ -59,11 +60,16 @@ class StaticModel:
                     pr.load_value[0] if pr.load_kind == "pressure" else pr.load_value)
         return cls(pr.coords, pr.conn, pr.nu, pr.bc_eq, pr.bc_val, load, **kw)
 
-    def assemble(self, young: torch.Tensor):
+    def assemble(self, young: torch.Tensor, reuse: bool = False):
+        """Assemble K(young); with reuse, skip it when the operator on the
+        device already is K(young).  The model keeps a reference to the field
+        it assembled, so no other tensor can occupy its storage meanwhile."""
         key = (young.data_ptr(), young._version)
-        if key != self._key:
-            self.ctx.assemble_k(young.detach(), self.nu, precision=self.precision)
-            self._key = key
+        if reuse and self._key == key and self._young is not None:
+            return
+        self.ctx.assemble_k(young.detach(), self.nu, precision=self.precision)
+        self._young = young
+        self._key = key
 
     def solve(self, young: torch.Tensor) -> torch.Tensor:
         return StaticSolve.apply(young, self)
@@ -79,7 +85,7 @@ class StaticSolve(torch.autograd.Function):
     def forward(ctx, young: torch.Tensor, model: StaticModel):
         assert young.is_cuda and young.dtype == torch.float64 and young.numel() == model.n_elems
         young = young.detach().contiguous()
-        model.assemble(young)
+        model.assemble(young, reuse=False)
         u = torch.zeros(model.n_dof, dtype=torch.float64, device=young.device)
         _, info = model.ctx.pcg(precision=model.precision, rtol=model.rtol,
                                 inner_rtol=model.inner_rtol, x=u)
@@ -94,7 +100,7 @@ class StaticSolve(torch.autograd.Function):
     def backward(ctx, grad_u: torch.Tensor):
         young, u = ctx.saved_tensors
         model: StaticModel = ctx.model
-        model.assemble(young)  # the operator of the forward solve
+        model.assemble(young, reuse=True)  # the operator of the forward solve
         lam = torch.zeros_like(u)
         _, info = model.ctx.adjoint_solve(grad_u.detach().contiguous(), precision=model.precision,
                                           rtol=model.adjoint_rtol, inner_rtol=model.inner_rtol,
