@@ -106,6 +106,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fd-check", type=int, default=0,
+                    help="central-difference check of the adjoint gradient on N elements")
+    ap.add_argument("--j2", action="store_true", help="also run the J2 Newton workload")
+    ap.add_argument("--j2-size", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -196,37 +200,93 @@ def main():
 
     if not args.no_adjoint:
         na = args.adjoint_size
-        log(f"adjoint {na}^3")
+        log(f"adjoint {na}^3 (torch.autograd.Function over the C-ABI)")
+        from ufc_hip.autograd import StaticModel, compliance
         pa = P.simp_cube(na)
-        with ufc_hip.Context(0) as c2:
-            P.setup_context(c2, pa, ufc_hip.PREC_MIXED)
-            yd = torch.as_tensor(pa.young, device=dev)
-            rho = torch.as_tensor(pa.extra["rho"], device=dev)
-            # compliance c = f^T u with the external load (ubar = 0: no penalty terms)
-            fext = torch.as_tensor(c2.get_rhs(), device=dev)
-            u = torch.zeros(pa.n_dof, dtype=torch.float64, device=dev)
-            lam = torch.zeros_like(u)
-            g = torch.zeros(pa.n_elems, dtype=torch.float64, device=dev)
-            dedrho = 3 * rho ** 2 * 1e9
-            for w in range(args.warmup + 1):
-                torch.cuda.synchronize()
-                ta = time.perf_counter()
-                c2.assemble_k(yd, pa.nu, precision=ufc_hip.PREC_MIXED)
-                _, fi = c2.pcg(precision=ufc_hip.PREC_MIXED, rtol=1e-10, x=u)
-                torch.cuda.synchronize()
-                tb = time.perf_counter()
-                _, ai = c2.adjoint_solve(fext, precision=ufc_hip.PREC_MIXED, rtol=1e-10, x=lam)
-                c2.sensitivity(u, lam, dedrho, pa.nu, out=g)
-                torch.cuda.synchronize()
-                tc = time.perf_counter()
-            fwd_ms = (tb - ta) * 1e3
-            adj_ms = (tc - tb) * 1e3
-            out["adjoint"] = {"workload": f"{na}^3 SIMP cube, rho~U[0.3,1] seed {P.SEED + 1}",
-                              "n_dof": pa.n_dof, "forward_ms": fwd_ms, "grad_ms": adj_ms,
-                              "grad_over_forward": adj_ms / fwd_ms,
-                              "fwd_iters": fi.niter, "adj_iters": ai.niter,
-                              "compliance": float(torch.dot(fext, u))}
+        model = StaticModel.from_problem(pa, precision=ufc_hip.PREC_MIXED, rtol=1e-10)
+        rho0 = torch.as_tensor(pa.extra["rho"], device=dev)
+        for w in range(args.warmup + 1):
+            rho = rho0.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            c, u = compliance(model, 1e-3 + rho ** 3 * 1e9)
+            torch.cuda.synchronize()
+            tb = time.perf_counter()
+            c.backward()
+            torch.cuda.synchronize()
+            tc = time.perf_counter()
+        fwd_ms = (tb - ta) * 1e3
+        adj_ms = (tc - tb) * 1e3
+        out["adjoint"] = {"workload": (f"{na}^3 SIMP cube, rho~U[0.3,1] seed {P.SEED + 1}, "
+                                       "E = 1e-3 + rho^3 1e9; c = f.u; dc/drho by "
+                                       "torch.autograd (adjoint PCG + element kernel)"),
+                          "n_dof": pa.n_dof, "precision": "mixed", "forward_ms": fwd_ms,
+                          "grad_ms": adj_ms, "grad_over_forward": adj_ms / fwd_ms,
+                          "fwd_iters": model.last_forward.niter,
+                          "adj_iters": model.last_adjoint.niter,
+                          "compliance": float(c)}
         log(f"adjoint: forward {fwd_ms:.1f} ms, grad {adj_ms:.1f} ms")
+        if args.fd_check:
+            # central differences h = 1e-6 on seeded elements, FP64 solves to 1e-14
+            log(f"finite-difference check on {args.fd_check} elements")
+            m64 = StaticModel.from_problem(pa, precision=ufc_hip.PREC_FP64, rtol=1e-14,
+                                           adjoint_rtol=1e-14)
+            rho = rho0.clone().requires_grad_(True)
+            c64, _ = compliance(m64, 1e-3 + rho ** 3 * 1e9)
+            c64.backward()
+            g = rho.grad.detach().cpu().numpy()
+            gm = rho0.clone().requires_grad_(True)
+            cm, _ = compliance(model, 1e-3 + gm ** 3 * 1e9)
+            cm.backward()
+            gmx = gm.grad.detach().cpu().numpy()
+            picks = np.random.default_rng(P.SEED + 3).choice(pa.n_elems, args.fd_check,
+                                                             replace=False)
+            errs = []
+            with torch.no_grad():
+                for e in picks:
+                    rp_, rm_ = rho0.clone(), rho0.clone()
+                    rp_[e] += 1e-6
+                    rm_[e] -= 1e-6
+                    cp, _ = compliance(m64, 1e-3 + rp_ ** 3 * 1e9)
+                    cn, _ = compliance(m64, 1e-3 + rm_ ** 3 * 1e9)
+                    fd = (float(cp) - float(cn)) / 2e-6
+                    errs.append(abs(fd - g[e]) / abs(g[e]))
+                    if len(errs) % 4 == 0:
+                        log(f"fd {len(errs)}/{len(picks)}: max rel err {max(errs):.2e}")
+            out["adjoint"]["fd_check"] = {
+                "elements": int(args.fd_check), "h": 1e-6, "solve_rtol": 1e-14,
+                "max_rel_err_fp64_adjoint_vs_fd": float(max(errs)),
+                "median_rel_err": float(np.median(errs)),
+                "mixed_vs_fp64_grad_rel": float(np.abs(gmx - g).max() / np.abs(g).max())}
+            log(f"fd check: max rel err {max(errs):.2e}")
+            m64.close()
+        model.close()
+
+    if args.j2:
+        from ufc_hip.nonlinear import j2_static
+        nj = args.j2_size
+        log(f"J2 {nj}^3 jittered, 10 increments to 0.5% strain")
+        pj = P.j2_cube(nj)
+        target = np.zeros(pj.bc_eq.size)
+        target[pj.extra["n_clamp"]:] = pj.extra["strain"]
+        with ufc_hip.Context(0) as cj:
+            cj.build_pattern(pj.coords, pj.conn)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
+                            precision=ufc_hip.PREC_MIXED)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        out["j2"] = {"workload": (f"{nj}^3 C3D8 cube, interior nodes jittered +-10% h, "
+                                  "E 200e9, nu 0.3, sigma_y 250e6, H 2e9, u_z on z=1 to "
+                                  "0.5% strain in 10 increments; Newton on the consistent "
+                                  "tangent, inner solve FP32-in-FP64 to 1e-10"),
+                     "n_dof": pj.n_dof, "converged": res.converged, "total_ms": (t1 - t0) * 1e3,
+                     "newton_iters": [i.newton_iters for i in res.increments],
+                     "linear_iters": int(sum(i.linear_iters for i in res.increments)),
+                     "yielded_gauss_points_final": int(res.increments[-1].n_yield)
+                     if res.increments else 0}
+        log(f"J2: {(t1 - t0):.2f} s, newton {out['j2']['newton_iters']}")
 
     if not args.no_cpu:
         log("cpu baseline")

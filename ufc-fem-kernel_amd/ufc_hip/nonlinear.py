@@ -1,0 +1,99 @@
+"""Small-strain static Newton-Raphson with the J2 material over the C-ABI.
+
+Mirrors the increment loop of RT_StepDriver_Execute (RT_Step_Exec.f90:453-600)
+and RT_NLSolver_NewtonRaph (RT_Solv_Nonlin.f90:807-1000), paths relative to the
+reference root, with the hot parts on the GPU:
+
+  per Newton iteration
+    ufc_ph_j2_update     strain increments B(u - u_n), radial return from the
+                         committed state, consistent tangent  (PH_J2_ComputeStress)
+    ufc_rt_residual      R = F + P ubar - F_int(sigma) - P u   (RT_Asm_ComputeResidual)
+    converged when ||R|| <= tol_r ||F_int|| (after the first iteration)
+    ufc_rt_asm_assemble_k(UFC_MAT_J2)  tangent K_t into the fixed CSR pattern
+    ufc_nm_pcg           K_t du = R   (FP64 or FP32-in-FP64 refinement)
+    u += du
+  per increment: ufc_ph_j2_commit on convergence; a failed increment
+  (local-Newton failure or no convergence) is cut back by 0.25
+  (pnewdt, PH_Mat_Plast_J2_Iso_Core.f90:419-427) from the committed state.
+
+The reference converges on absolute norms ||R|| < 1e-6 and ||du|| < 1e-5
+(RT_Step_Exec.f90:380-382); in SI units those are unattainable, so the test
+here is relative to the internal force.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+from .static import Context
+
+
+@dataclass
+class IncrementInfo:
+    load_factor: float
+    newton_iters: int
+    linear_iters: int
+    rnorm_rel: float
+    n_yield: int
+
+
+@dataclass
+class NewtonResult:
+    converged: bool
+    increments: list = field(default_factory=list)
+    u: np.ndarray = None
+
+
+def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
+              tol_r=1e-8, lin_rtol=1e-10, max_newton=25, penalty=1.0e12, max_cutbacks=4):
+    """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
+    in n_increments equal increments.  ctx must hold the mesh (build_pattern)."""
+    bc_eq = np.ascontiguousarray(bc_eq, np.int32)
+    bc_target = np.asarray(bc_target, np.float64)
+    ctx.j2_init(*props)
+    n = ctx.n_dof
+    u = np.zeros(n)
+    lam = 0.0
+    step = 1.0 / n_increments
+    res = NewtonResult(converged=True)
+    cutbacks = 0
+    while lam < 1.0 - 1e-12:
+        dlam = min(step, 1.0 - lam)
+        target = lam + dlam
+        ctx.apply_bc(bc_eq, bc_target * target, penalty)
+        u_n = u.copy()
+        ok, its, lin, rrel, ny = False, 0, 0, np.inf, 0
+        u_try = u_n.copy()
+        for it in range(1, max_newton + 1):
+            try:
+                ny = ctx.j2_update(u_try, u_n)
+            except A.UfcError:
+                break  # local Newton failure: cut back
+            r, norms = ctx.residual(u_try)
+            its = it
+            rrel = norms[0] / norms[1] if norms[1] > 0.0 else float("inf")
+            if it > 1 and norms[0] <= tol_r * max(norms[1], 1e-300):
+                ok = True
+                break
+            ctx.assemble_k(None, props[1], precision=precision, material=A.MAT_J2)
+            du, info = ctx.pcg(precision=precision, rtol=lin_rtol, b=r)
+            lin += info.niter
+            if not info.converged:
+                break
+            u_try = u_try + du
+        if ok:
+            ctx.j2_commit()
+            u = u_try
+            lam = target
+            res.increments.append(IncrementInfo(lam, its, lin, rrel, ny))
+            cutbacks = 0
+        else:
+            cutbacks += 1
+            if cutbacks > max_cutbacks:
+                res.converged = False
+                break
+            step *= 0.25
+    res.u = u
+    return res
