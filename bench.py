@@ -52,6 +52,16 @@ def spmv64_bytes(n_dof, nnz):
     return nnz * (8 + 4) + (n_dof + 1) * 4 + 2 * n_dof * 8
 
 
+def measured_traffic(n_dof, kernel="k_spmv<float, true>"):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary, when it was taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "spmv_pcg128_hbm_traffic.json")
+    if n_dof != 6440067 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["kernels"][kernel]["traffic_bytes"]
+
+
 def run_step(ctx, pr, precision, rtol, torch, x):
     import ufc_hip
     torch.cuda.synchronize()
@@ -110,6 +120,9 @@ def main():
                     help="central-difference check of the adjoint gradient on N elements")
     ap.add_argument("--j2", action="store_true", help="also run the J2 Newton workload")
     ap.add_argument("--j2-size", type=int, default=64)
+    ap.add_argument("--pmc-sample", action="store_true",
+                    help="short run for hardware-counter passes: 10 PCG iterations in each "
+                         "precision plus a 2 GiB device copy for calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -137,6 +150,19 @@ def main():
     log(f"mesh {t1 - t0:.2f} s, pattern+slot map+loads+BC {t2 - t1:.2f} s, nnz {ctx.nnz}")
     x = torch.zeros(pr.n_dof, dtype=torch.float64, device=dev)
     ctx.enable_timing(True)
+    if args.pmc_sample:
+        # calibration: a plain copy of 2 GiB (reads 2 GiB, writes 2 GiB)
+        a = torch.ones(2 ** 28, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        for prec in (ufc_hip.PREC_MIXED, ufc_hip.PREC_FP64):
+            ctx.assemble_k(pr.young_dev, pr.nu, precision=prec)
+            ctx.pcg(precision=prec, rtol=1e-10, max_iter=10, max_outer=1, x=x)
+        torch.cuda.synchronize()
+        log("pmc sample done")
+        return
 
     res = {}
     sols = {}
@@ -192,9 +218,11 @@ def main():
     b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes)
     ach = b32 / (spmv_ms * 1e6)
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                       "kernel": "k_spmv_dot<float> (FP32 node-block SpMV + p.Ap)",
-                       "bytes_per_launch": b32, "ms_per_launch": spmv_ms}
+                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(pr.n_dof),
+                       "kernel": "k_spmv<float,true> (FP32 node-block SpMV + p.Ap partials)",
+                       "bytes_per_launch": b32, "ms_per_launch": spmv_ms,
+                       "traffic_source": "profiles/spmv_pcg128_hbm_traffic.md (rocprofv3 "
+                                         "FETCH_SIZE / WRITE_SIZE, copy-calibrated)"}
     ctx.close()
     del pr.young_dev
 
