@@ -1,8 +1,8 @@
-"""The Fortran side of the boundary: ufc_hip_iface.f90 (ISO_C_BINDING) and the
-static-step driver ufc_static_step.f90, built with flang against
-libufc_hip.so.  CPU: it builds and reports a status code without a device.
-GPU: its solution matches the oracle and its exported 1-based pattern the
-reference's own CSR build."""
+"""The Fortran side of the boundary: ufc_hip_iface.f90 (ISO_C_BINDING), the
+static-step driver ufc_static_step.f90 and the J2 Newton step driver
+ufc_j2_step.f90, built with flang against libufc_hip.so.  CPU: they build and
+report a status code without a device.  GPU: their solutions match the oracle
+and the exported 1-based pattern the reference's own CSR build."""
 import os
 import subprocess
 
@@ -12,12 +12,13 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FDIR = os.path.join(ROOT, "ufc-fem-kernel_amd", "fortran")
 EXE = os.path.join(FDIR, "ufc_static_step")
+EXE_J2 = os.path.join(FDIR, "ufc_j2_step")
 
 
-def _exe():
-    if not os.path.exists(EXE):
+def _exe(path=EXE):
+    if not os.path.exists(path):
         subprocess.run(["make", "-s", "-C", FDIR], check=True)
-    return EXE
+    return path
 
 
 def test_driver_reports_status_without_gpu(tmp_path):
@@ -25,6 +26,9 @@ def test_driver_reports_status_without_gpu(tmp_path):
     if torch.cuda.device_count() > 0:
         pytest.skip("a GPU is present")
     r = subprocess.run([_exe(), "cantilever", "20", "4", "4", "fp64", str(tmp_path / "u.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "ufc_ctx_create failed, status" in r.stdout
+    r = subprocess.run([_exe(EXE_J2), "2", "fp64", "2", str(tmp_path / "u.bin")],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "ufc_ctx_create failed, status" in r.stdout
 
@@ -55,3 +59,33 @@ def test_driver_cantilever_matches_oracle(tmp_path, orc, golden_dir, prec):
     assert int(fields[fields.index("nnz") + 1]) == g["ja"].size
     assert int(fields[fields.index("colsum") + 1]) == int(g["ja"].astype(np.int64).sum())
     assert int(fields[fields.index("row_ptr_end") + 1]) == int(g["ia"][-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_j2_driver_matches_oracle(tmp_path, orc, prec):
+    """J2 cube 4^3 (jittered), 10 increments to 0.5% strain, through the
+    Fortran Newton driver; the oracle runs the same increments."""
+    from ufc_hip import problems as P
+    out = tmp_path / "u.bin"
+    print(f"[gpu-test] fortran j2 driver 4^3 {prec}", flush=True)
+    r = subprocess.run([_exe(EXE_J2), "4", prec, "10", str(out)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("increment ") == 10
+    u = np.fromfile(out, dtype=np.float64)
+    pr = P.j2_cube(4)
+    target = np.zeros(pr.bc_eq.size)
+    target[pr.extra["n_clamp"]:] = pr.extra["strain"]
+    state = np.zeros(pr.n_elems * 8 * 13)
+    uo = np.zeros(pr.n_dof)
+    for k in range(1, 11):
+        st, uo, state, _, _ = orc.j2_newton_step(pr.coords.shape[0], pr.conn, pr.coords,
+                                                 pr.extra["props"], pr.bc_eq, target * (k / 10.0),
+                                                 1e12, state, uo)
+        assert st == 0
+    assert u.size == pr.n_dof
+    err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
+    print(f"[gpu-test] fortran j2 u err {err:.2e}", flush=True)
+    assert err < 1e-7

@@ -1,0 +1,203 @@
+! ufc_j2_step.f90 -- Fortran J2 static step over libufc_hip.so: the increment
+! loop of RT_StepDriver_Execute (L5_RT/StepDriver/RT_Step_Exec.f90:453-600) and
+! the Newton-Raphson iteration of RT_NLSolver_NewtonRaph
+! (L5_RT/Solver/RT_Solv_Nonlin.f90:807-1000), paths relative to the reference
+! root, with every per-Gauss-point and per-DOF operation on the GPU:
+!
+!   per increment  prescribed values lam * target (RT_Asm_ApplyBC, penalty)
+!     per Newton iteration
+!       ufc_ph_j2_update       strains B(u - u_n), radial return from the
+!                              committed state, consistent tangent
+!       ufc_rt_residual        R = F + P ubar - F_int - P u
+!       converged when ||R|| <= 1e-8 ||F_int|| (after the first iteration)
+!       ufc_rt_asm_assemble_k  tangent K_t (UFC_MAT_J2) into the fixed CSR
+!       ufc_nm_pcg             K_t du = R, u += du
+!     ufc_ph_j2_commit on convergence; otherwise the increment is cut back by
+!     0.25 from the committed state (pnewdt, PH_Mat_Plast_J2_Iso_Core.f90:419-427).
+!
+! The reference converges on absolute norms (RT_Step_Exec.f90:380-382); the
+! test here is relative to the internal force, which is scale-free.
+!
+! Usage: ufc_j2_step <n> <precision> <increments> <u_out>
+!   unit cube of n^3 C3D8, interior nodes jittered by U[-0.1, 0.1] h (seeded),
+!   E = 200e9, nu = 0.3, sigma_y0 = 250e6, H = 2e9; z = 0 clamped, u_z on
+!   z = 1 raised to 0.005 (0.5% strain) in <increments> equal increments.
+!   precision fp64 | mixed (the linear solve); u_out: n_dof doubles, stream.
+! Prints one line per increment and a summary; exits 0 when every increment
+! converged, 1 otherwise, 2 on a C-ABI error.
+PROGRAM ufc_j2_step
+  USE, INTRINSIC :: ISO_C_BINDING
+  USE ufc_hip_iface
+  IMPLICIT NONE
+
+  INTEGER(C_INT64_T), PARAMETER :: mesh_seed = 20251205_C_INT64_T
+  REAL(C_DOUBLE), PARAMETER :: strain = 0.005_C_DOUBLE, tol_r = 1.0E-8_C_DOUBLE
+  INTEGER, PARAMETER :: max_newton = 25, max_cutbacks = 4
+
+  CHARACTER(LEN=64) :: arg, prec_name
+  CHARACTER(LEN=512) :: outfile
+  INTEGER :: n, n_inc, i, it, cutbacks, n_done, u_unit, total_newton, total_linear, lin
+  INTEGER(C_INT32_T) :: nn, ne, prec, rc
+  INTEGER(C_INT64_T) :: n_yield
+  INTEGER(C_INT64_T) :: ndof
+  REAL(C_DOUBLE) :: lam, dlam, step, target, rrel
+  REAL(C_DOUBLE), TARGET :: norms(2)
+  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: coords(:, :), u(:), u_n(:), u_try(:), r(:), du(:)
+  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: bc_target(:), bc_val(:)
+  INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: conn(:, :), bc_eq(:)
+  LOGICAL :: ok, failed
+  TYPE(C_PTR) :: ctx
+  TYPE(ufc_status) :: st
+  TYPE(ufc_mesh_desc) :: mesh
+  TYPE(ufc_rt_asm_info) :: info
+  TYPE(ufc_rt_asm_algo) :: aalgo
+  TYPE(ufc_rt_asm_args) :: aargs
+  TYPE(ufc_rt_bc_args) :: bc
+  TYPE(ufc_nm_solver_algo) :: salgo
+  TYPE(ufc_nm_solver_state) :: sstate
+  TYPE(ufc_ph_j2_props) :: props
+
+  CALL GET_COMMAND_ARGUMENT(1, arg); READ(arg, *) n
+  CALL GET_COMMAND_ARGUMENT(2, prec_name)
+  CALL GET_COMMAND_ARGUMENT(3, arg); READ(arg, *) n_inc
+  CALL GET_COMMAND_ARGUMENT(4, outfile)
+  prec = UFC_PREC_FP64
+  IF (TRIM(prec_name) == 'mixed') prec = UFC_PREC_MIXED
+
+  nn = INT((n + 1)**3, C_INT32_T)
+  ne = INT(n**3, C_INT32_T)
+  ndof = 3_C_INT64_T * nn
+  ALLOCATE(coords(3, nn), conn(8, ne))
+  ALLOCATE(u(ndof), u_n(ndof), u_try(ndof), r(ndof), du(ndof))
+  rc = ufc_mesh_box(INT(n, C_INT32_T), INT(n, C_INT32_T), INT(n, C_INT32_T), 1.0_C_DOUBLE, &
+                    1.0_C_DOUBLE, 1.0_C_DOUBLE, 0.1_C_DOUBLE, mesh_seed, 1_C_INT32_T, &
+                    C_LOC(coords), C_LOC(conn), st)
+  CALL check('ufc_mesh_box')
+
+  ! z = 0: all three equations clamped; z = 1: u_z prescribed (1-based eqs)
+  ALLOCATE(bc_eq(3 * 2 * (n + 1)**2), bc_target(3 * 2 * (n + 1)**2))
+  bc%n_bc = 0
+  DO i = 1, nn
+    IF (coords(3, i) == 0.0_C_DOUBLE) THEN
+      bc_eq(bc%n_bc + 1:bc%n_bc + 3) = [3 * (i - 1) + 1, 3 * (i - 1) + 2, 3 * (i - 1) + 3]
+      bc_target(bc%n_bc + 1:bc%n_bc + 3) = 0.0_C_DOUBLE
+      bc%n_bc = bc%n_bc + 3
+    END IF
+  END DO
+  DO i = 1, nn
+    IF (coords(3, i) == 1.0_C_DOUBLE) THEN
+      bc%n_bc = bc%n_bc + 1
+      bc_eq(bc%n_bc) = 3 * (i - 1) + 3
+      bc_target(bc%n_bc) = strain
+    END IF
+  END DO
+  ALLOCATE(bc_val(bc%n_bc))
+
+  rc = ufc_ctx_create(0_C_INT32_T, ctx, st)
+  CALL check('ufc_ctx_create')
+  mesh%n_nodes = nn
+  mesh%n_elems = ne
+  mesh%index_base = 1
+  mesh%coords = C_LOC(coords)
+  mesh%conn = C_LOC(conn)
+  rc = ufc_rt_asm_build_pattern(ctx, mesh, info, st)
+  CALL check('ufc_rt_asm_build_pattern')
+  props%young = 200.0E9_C_DOUBLE
+  props%nu = 0.3_C_DOUBLE
+  props%sigma_y0 = 250.0E6_C_DOUBLE
+  props%hardening = 2.0E9_C_DOUBLE
+  rc = ufc_ph_j2_init(ctx, props, st)
+  CALL check('ufc_ph_j2_init')
+
+  aalgo%precision = prec
+  aalgo%material = UFC_MAT_J2
+  aargs%young = C_NULL_PTR
+  aargs%nu = props%nu
+  bc%index_base = 1
+  bc%eq = C_LOC(bc_eq)
+  bc%value = C_LOC(bc_val)
+  bc%penalty = 1.0E12_C_DOUBLE
+  salgo%precision = prec
+  salgo%rtol = 1.0E-10_C_DOUBLE
+  salgo%inner_rtol = 1.0E-5_C_DOUBLE
+
+  u = 0.0_C_DOUBLE
+  lam = 0.0_C_DOUBLE
+  step = 1.0_C_DOUBLE / REAL(n_inc, C_DOUBLE)
+  cutbacks = 0
+  n_done = 0
+  total_newton = 0
+  total_linear = 0
+  failed = .FALSE.
+  DO WHILE (lam < 1.0_C_DOUBLE - 1.0E-12_C_DOUBLE)
+    dlam = MIN(step, 1.0_C_DOUBLE - lam)
+    target = lam + dlam
+    bc_val = bc_target * target
+    rc = ufc_rt_asm_apply_bc(ctx, bc, st)
+    CALL check('ufc_rt_asm_apply_bc')
+    u_n = u
+    u_try = u_n
+    ok = .FALSE.
+    lin = 0
+    rrel = HUGE(1.0_C_DOUBLE)
+    DO it = 1, max_newton
+      rc = ufc_ph_j2_update(ctx, C_LOC(u_try), C_LOC(u_n), n_yield, st)
+      IF (rc /= UFC_STATUS_OK) EXIT            ! local Newton failure: cut back
+      rc = ufc_rt_residual(ctx, C_LOC(u_try), C_LOC(r), C_LOC(norms), st)
+      CALL check('ufc_rt_residual')
+      IF (norms(2) > 0.0_C_DOUBLE) rrel = norms(1) / norms(2)
+      IF (it > 1 .AND. norms(1) <= tol_r * MAX(norms(2), TINY(1.0_C_DOUBLE))) THEN
+        ok = .TRUE.
+        EXIT
+      END IF
+      rc = ufc_rt_asm_assemble_k(ctx, aalgo, aargs, st)
+      CALL check('ufc_rt_asm_assemble_k')
+      sstate%x = C_LOC(du)
+      sstate%b = C_LOC(r)
+      sstate%x0_zero = 1
+      rc = ufc_nm_pcg(ctx, salgo, sstate, st)
+      lin = lin + sstate%niter
+      IF (rc /= UFC_STATUS_OK) EXIT
+      u_try = u_try + du
+    END DO
+    total_linear = total_linear + lin
+    IF (ok) THEN
+      rc = ufc_ph_j2_commit(ctx, st)
+      CALL check('ufc_ph_j2_commit')
+      u = u_try
+      lam = target
+      n_done = n_done + 1
+      total_newton = total_newton + it
+      cutbacks = 0
+      WRITE(*, '(A,I0,A,F8.5,A,I0,A,I0,A,ES10.3,A,I0)') 'increment ', n_done, ' lambda ', lam, &
+           ' newton ', it, ' linear ', lin, ' rrel ', rrel, ' n_yield ', n_yield
+    ELSE
+      cutbacks = cutbacks + 1
+      IF (cutbacks > max_cutbacks) THEN
+        failed = .TRUE.
+        EXIT
+      END IF
+      step = step * 0.25_C_DOUBLE
+    END IF
+  END DO
+
+  OPEN(NEWUNIT=u_unit, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', &
+       STATUS='REPLACE', ACTION='WRITE')
+  WRITE(u_unit) u
+  CLOSE(u_unit)
+  WRITE(*, '(A,I0,A,I0,A,I0,A,I0,A,L1)') 'ufc_j2_step n_dof ', info%n_dof, ' increments ', &
+       n_done, ' newton ', total_newton, ' linear ', total_linear, ' converged ', .NOT. failed
+  rc = ufc_ctx_destroy(ctx, st)
+  IF (failed) CALL EXIT(1)
+
+CONTAINS
+
+  SUBROUTINE check(what)
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    IF (rc /= UFC_STATUS_OK) THEN
+      WRITE(*, '(A,A,I0,A,A)') what, ' failed, status ', rc, ': ', TRIM(ufc_status_message(st))
+      CALL EXIT(2)
+    END IF
+  END SUBROUTINE check
+
+END PROGRAM ufc_j2_step
