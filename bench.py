@@ -62,25 +62,28 @@ def measured_traffic(n_dof, kernel="k_spmv<float, true>"):
         return json.load(f)["kernels"][kernel]["traffic_bytes"]
 
 
-def run_step(ctx, pr, precision, rtol, torch, x):
+def run_step(ctx, pr, precision, rtol, torch, x, replace_rtol=0.0):
     import ufc_hip
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.assemble_k(pr.young_dev, pr.nu, precision=precision)
     t1 = time.perf_counter()
-    u, info = ctx.pcg(precision=precision, rtol=rtol, inner_rtol=1e-5, x=x)
+    u, info = ctx.pcg(precision=precision, rtol=rtol, inner_rtol=1e-5, x=x,
+                      replace_rtol=replace_rtol)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     t = ctx.timing()
     return dict(wall_ms=(t2 - t0) * 1e3, assemble_wall_ms=(t1 - t0) * 1e3,
                 assemble_ms=t.assemble_ms, solve_ms=t.solve_ms, spmv_ms=t.spmv_ms_avg,
                 spmv_calls=t.spmv_calls, niter=info.niter, n_outer=info.n_outer,
+                n_replace=t.n_replace,
                 rnorm=info.rnorm, converged=info.converged,
                 status=info.status), u
 
 
-def cpu_baseline(pr, fp64_iters, n_sample_iters=20):
-    """FP64 oracle on the same problem; returns DOF/s projected to fp64_iters."""
+def cpu_baseline(pr, fp64_iters, n_sample_iters=20, full_solve=False):
+    """FP64 oracle on the same problem; returns DOF/s projected to fp64_iters
+    (and, with full_solve, the oracle's full solution for the error check)."""
     import numpy as np
     from oracle import oracle as O
     cores = min(16, len(os.sched_getaffinity(0)))
@@ -96,14 +99,67 @@ def cpu_baseline(pr, fp64_iters, n_sample_iters=20):
     _, st, it, rn = O.pcg(rp, ci, val, F, rtol=1e-10, max_iter=n_sample_iters, nthreads=cores)
     t4 = time.perf_counter()
     per_it = (t4 - t3) / max(it, 1)
+    _, _, it1, _ = O.pcg(rp, ci, val, F, rtol=1e-10, max_iter=3, nthreads=1)
+    t5 = time.perf_counter()
+    per_it1 = (t5 - t4) / max(it1, 1)
     t_solve = per_it * fp64_iters
     dofs = pr.n_dof / ((t2 - t1) + t_solve)
-    return {"value": dofs, "unit": "DOF/s", "cores": cores, "kind": "port",
-            "sample": (f"FP64 C oracle (OpenMP) on the same {pr.nx}^3 problem: CSR pattern "
-                       f"{t1 - t0:.2f} s (setup, not counted), assembly {t2 - t1:.2f} s in full, "
-                       f"{it} Jacobi-PCG iterations timed ({per_it * 1e3:.1f} ms each), solve "
-                       f"projected to the {fp64_iters} iterations the FP64 GPU solve needed"),
-            "assemble_s": t2 - t1, "pcg_iter_ms": per_it * 1e3}
+    out = {"value": dofs, "unit": "DOF/s", "cores": cores, "kind": "port",
+           "sample": (f"FP64 C oracle (OpenMP) on the same {pr.nx}^3 problem: CSR pattern "
+                      f"{t1 - t0:.2f} s (setup, not counted), assembly {t2 - t1:.2f} s in full, "
+                      f"{it} Jacobi-PCG iterations timed ({per_it * 1e3:.1f} ms each), solve "
+                      f"projected to the {fp64_iters} iterations the FP64 GPU solve needed"),
+           "assemble_s": t2 - t1, "pcg_iter_ms": per_it * 1e3,
+           "pcg_iter_ms_1core": per_it1 * 1e3}
+    u_full = None
+    if full_solve:
+        t6 = time.perf_counter()
+        u_full, st, itf, rnf = O.pcg(rp, ci, val, F, rtol=1e-12, nthreads=cores)
+        out["full_solve"] = {"rtol": 1e-12, "iters": int(itf), "status": int(st),
+                             "s": time.perf_counter() - t6}
+    return out, u_full
+
+
+def oracle_check(ufc_hip, P, n=32):
+    """GPU mixed and FP64 solutions against the FP64 oracle driven to 1e-13
+    on an n^3 lognormal cube (a size the oracle solves in seconds)."""
+    import numpy as np
+    from oracle import oracle as O
+    pr = P.lognormal_cube(n)
+    rp, ci = O.csr_pattern(pr.coords.shape[0], pr.conn)
+    val = O.assemble(pr.conn, pr.coords, pr.young, pr.nu, rp, ci,
+                     nthreads=min(16, len(os.sched_getaffinity(0))))
+    F = np.zeros(pr.n_dof)
+    O.face_loads_pressure(pr.load_elem, pr.load_face, pr.conn, pr.coords, pr.load_value[0], F)
+    O.apply_penalty(pr.bc_eq, pr.bc_val, 1e12, rp, ci, val, F)
+    uo, st, it, _ = O.pcg(rp, ci, val, F, rtol=1e-13, nthreads=min(16, len(os.sched_getaffinity(0))))
+    res = {"workload": f"{n}^3 lognormal cube (same generator and seed), oracle PCG to 1e-13 "
+                       f"({it} iterations), GPU solves to 1e-10"}
+    with ufc_hip.Context(0) as ctx:
+        P.setup_context(ctx, pr, ufc_hip.PREC_MIXED)
+        for name, prec in (("mixed", ufc_hip.PREC_MIXED), ("fp64", ufc_hip.PREC_FP64)):
+            ctx.assemble_k(pr.young, pr.nu, precision=prec)
+            u, info = ctx.pcg(precision=prec, rtol=1e-10)
+            res[f"{name}_rel"] = float(np.linalg.norm(u - uo) / np.linalg.norm(uo))
+    return res
+
+
+def copy_bandwidth(torch, dev, nbytes=1 << 31):
+    """Device-to-device copy of nbytes (reads and writes nbytes): the
+    attainable-bandwidth denominator beside the 8 TB/s peak."""
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del a, b
+    return 2.0 * nbytes / (ms * 1e6)
 
 
 def main():
@@ -114,8 +170,16 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--adjoint-size", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also solve the full problem with the CPU oracle (minutes) and "
+                         "report the GPU solutions' error against it")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--replace-rtol", type=float, default=0.1,
+                    help="mixed: FP64 residual replacement inside the inner solve every this "
+                         "reduction, search direction kept (>= 1: restarted refinement steps)")
+    ap.add_argument("--no-restarted", action="store_true",
+                    help="skip the comparison run of the restarted refinement form")
     ap.add_argument("--fd-check", type=int, default=0,
                     help="central-difference check of the adjoint gradient on N elements")
     ap.add_argument("--j2", action="store_true", help="also run the J2 Newton workload")
@@ -166,14 +230,18 @@ def main():
 
     res = {}
     sols = {}
-    modes = [("mixed", ufc_hip.PREC_MIXED)] + ([] if args.no_fp64 else [("fp64", ufc_hip.PREC_FP64)])
-    for name, prec in modes:
+    modes = [("mixed", ufc_hip.PREC_MIXED, args.replace_rtol)]
+    if not args.no_fp64:
+        modes.append(("fp64", ufc_hip.PREC_FP64, 0.0))
+    if not args.no_restarted:
+        modes.append(("mixed_restarted", ufc_hip.PREC_MIXED, 1.0))
+    for name, prec, repl in modes:
         for w in range(args.warmup):
-            r, _ = run_step(ctx, pr, prec, 1e-10, torch, x)
+            r, _ = run_step(ctx, pr, prec, 1e-10, torch, x, repl)
             log(f"warmup {name}: {r['wall_ms']:.1f} ms, it {r['niter']}, outer {r['n_outer']}")
         runs = []
         for k in range(args.steps):
-            r, u = run_step(ctx, pr, prec, 1e-10, torch, x)
+            r, u = run_step(ctx, pr, prec, 1e-10, torch, x, repl)
             runs.append(r)
             log(f"step {name} {k}: {r['wall_ms']:.1f} ms (assemble {r['assemble_ms']:.2f} ms, "
                 f"solve {r['solve_ms']:.1f} ms), it {r['niter']}, outer {r['n_outer']}, "
@@ -191,15 +259,26 @@ def main():
            "config": {"workload": (f"{n}^3 C3D8 unit cube, E lognormal(ln 1e9, 0.5) seed "
                                    f"{P.SEED}, nu 0.3, z=0 clamped (penalty 1e12), p=1e6 on "
                                    "z=1; assemble + Jacobi-PCG to 1e-10: FP32 element "
-                                   "integration, values and inner PCG (tol 1e-5), FP64 "
-                                   "refinement residual, update and dots"),
+                                   "integration, values and inner PCG (tol 1e-5, FP64 "
+                                   "residual replacement every 0.1 with the search "
+                                   "direction kept), FP64 refinement residual, update and "
+                                   "dots"),
                       "n_dof": pr.n_dof, "nnz": ctx.nnz, "n_elems": pr.n_elems,
                       "precision": "mixed"},
            "mixed": {"dof_per_s": value, "ms": ms_step,
                      "assemble_ms": float(np.mean([r["assemble_ms"] for r in mixed])),
                      "solve_ms": float(np.mean([r["solve_ms"] for r in mixed])),
                      "inner_iters": mixed[-1]["niter"], "outer_iters": mixed[-1]["n_outer"],
+                     "fp64_residual_replacements": mixed[-1]["n_replace"],
+                     "replace_rtol": args.replace_rtol, "inner_rtol": 1e-5,
                      "rnorm": mixed[-1]["rnorm"]}}
+    if "mixed_restarted" in res:
+        mr = res["mixed_restarted"]
+        msr = float(np.mean([r["wall_ms"] for r in mr]))
+        out["mixed_restarted"] = {
+            "form": "each refinement step restarts the FP32 PCG (inner tol 1e-5, no replacement)",
+            "dof_per_s": pr.n_dof / (msr / 1e3), "ms": msr, "inner_iters": mr[-1]["niter"],
+            "outer_iters": mr[-1]["n_outer"], "rnorm": mr[-1]["rnorm"]}
     if "fp64" in res:
         f64 = res["fp64"]
         ms64 = float(np.mean([r["wall_ms"] for r in f64]))
@@ -210,15 +289,17 @@ def main():
                        "spmv_ms": f64[-1]["spmv_ms"],
                        "spmv_gbs": spmv64_bytes(pr.n_dof, ctx.nnz) / (f64[-1]["spmv_ms"] * 1e6)}
         out["mixed_over_fp64"] = ms64 / ms_step
-        d = sols["mixed"] - sols["fp64"]
-        out["errors"] = {"mixed_vs_fp64_gpu_rel": float(np.linalg.norm(d) /
-                                                        np.linalg.norm(sols["fp64"]))}
+        out["errors"] = {f"{k}_vs_fp64_gpu_rel": float(np.linalg.norm(v - sols["fp64"]) /
+                                                        np.linalg.norm(sols["fp64"]))
+                         for k, v in sols.items() if k != "fp64"}
     # roofline of the dominant kernel (FP32 SpMV + p.Ap inside the mixed solve)
     spmv_ms = mixed[-1]["spmv_ms"]
     b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes)
     ach = b32 / (spmv_ms * 1e6)
+    copy_gbs = copy_bandwidth(torch, dev)
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(pr.n_dof),
+                       "copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs,
                        "kernel": "k_spmv<float,true> (FP32 node-block SpMV + p.Ap partials)",
                        "bytes_per_launch": b32, "ms_per_launch": spmv_ms,
                        "traffic_source": "profiles/spmv_pcg128_hbm_traffic.md (rocprofv3 "
@@ -319,8 +400,15 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         iters = out.get("fp64", {}).get("iters", mixed[-1]["niter"])
-        out["cpu_baseline"] = cpu_baseline(pr, iters)
+        out["cpu_baseline"], u_cpu = cpu_baseline(pr, iters, full_solve=args.cpu_full)
         log(f"cpu {out['cpu_baseline']['value']:.3e} DOF/s")
+        errs = out.setdefault("errors", {})
+        if u_cpu is not None:
+            for name, u in sols.items():
+                errs[f"{name}_vs_cpu_oracle_rel"] = float(np.linalg.norm(u - u_cpu) /
+                                                          np.linalg.norm(u_cpu))
+        errs["vs_cpu_oracle_small"] = oracle_check(ufc_hip, P)
+        log(f"oracle check {errs['vs_cpu_oracle_small']}")
     print(json.dumps(out), flush=True)
 
 

@@ -274,6 +274,11 @@ typedef struct ufc_nm_solver_algo {
   int32_t max_outer;   /* MIXED: refinement steps                            */
   int32_t check_every; /* host poll interval in iterations (0 = 32); the    */
                        /* stopping iteration is decided on the device        */
+  double replace_rtol; /* MIXED: inside each inner solve the FP32 residual  */
+                       /* is replaced by the FP64 b - K u every time it has  */
+                       /* dropped by replace_rtol (<= 0: 0.1), and the inner */
+                       /* PCG keeps its search direction across refinement   */
+                       /* steps; >= 1: no replacement, each step restarts    */
 } ufc_nm_solver_algo;
 
 /* NM_Solver_State (NM_Solv_Def.f90:102-120). */
@@ -311,7 +316,8 @@ typedef struct ufc_timing {
   double solve_ms;
   double spmv_ms_avg;  /* mean SpMV kernel time of the last solve (events)   */
   int32_t spmv_calls;
-  int32_t reserved;
+  int32_t n_replace;   /* MIXED: FP64 residual replacements inside the inner */
+                       /* solves of the last solve                           */
 } ufc_timing;
 int ufc_ctx_timing(ufc_ctx* ctx, ufc_timing* t, ufc_status* st);
 /* per_kernel = 1 brackets every SpMV launch with a HIP event pair so that

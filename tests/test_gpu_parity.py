@@ -149,15 +149,18 @@ def test_pcg_fp64_vs_oracle(ctx, orc, kind):
     assert np.linalg.norm(u2 - uo2) / np.linalg.norm(uo2) < 1e-10
 
 
-def test_pcg_mixed_refinement_vs_oracle(ctx, orc):
+@pytest.mark.parametrize("replace_rtol", [0.0, 0.1])
+def test_pcg_mixed_refinement_vs_oracle(ctx, orc, replace_rtol):
+    """FP32 inner PCG in FP64 refinement, both forms: restarted steps to
+    inner_rtol, and residual replacement every 0.1 with the direction kept."""
     from ufc_hip import problems as P
     pr = P.cube_pressure(12, name="cube12")
     rp, ci, val, F, _ = _oracle_system(orc, pr)
     uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
     P.setup_context(ctx, pr, 2)
     ctx.assemble_k(pr.young, pr.nu, precision=2)
-    say("pcg mixed cube12")
-    u, info = ctx.pcg(precision=2, rtol=1e-10, inner_rtol=1e-5)
+    say("pcg mixed cube12 replace_rtol", replace_rtol)
+    u, info = ctx.pcg(precision=2, rtol=1e-10, inner_rtol=1e-5, replace_rtol=replace_rtol)
     assert info.status == 0 and info.rnorm < 1e-10 and info.n_outer >= 2
     # true FP64 residual of the oracle's operator
     K_u = orc.spmv(rp, ci, val, u)

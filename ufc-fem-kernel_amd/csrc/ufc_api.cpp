@@ -254,7 +254,7 @@ int ufc_ctx_timing(ufc_ctx* c, ufc_timing* t, ufc_status* st) {
     t->solve_ms = c->solve_ms;
     t->spmv_ms_avg = c->spmv_ms_avg;
     t->spmv_calls = c->spmv_calls;
-    t->reserved = 0;
+    t->n_replace = c->n_replace;
     return UFC_STATUS_OK;
   });
 }
@@ -877,11 +877,23 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
       UFC_HIP_CHECK(hipMemcpyAsync(u, s->x, n * 8, hipMemcpyDefault, c->stream));
     const double bnorm0 = ufc::norm2_64(c, b);
     const double bnorm = bnorm0 < 1.0e-30 ? 1.0 : bnorm0;
-    const int max_outer = algo->max_outer > 0 ? algo->max_outer : 20;
     const double inner = algo->inner_rtol > 0.0 ? algo->inner_rtol : 1.0e-5;
+    // residual replacement interval inside the inner solve; >= 1 switches it
+    // off (each refinement step then restarts the FP32 PCG from p = z)
+    const double delta = algo->replace_rtol > 0.0 ? algo->replace_rtol : 0.1;
+    const bool reliable = delta < 1.0;
+    const int max_outer = algo->max_outer > 0 ? algo->max_outer : 20;
+    // bound on FP64 residual evaluations: a few per decade of the target
+    const int max_eval = reliable ? max_outer * (2 + int(std::ceil(std::log(inner) /
+                                                                    std::log(delta))))
+                                  : max_outer;
     double* r = c->w_r.get();
+    bool have_dir = false;
+    double step_start = 0.0;  // ||r||/||b|| where the current refinement step began
+    int nrepl = 0;
     code = UFC_STATUS_INVALID;
-    for (int k = 0; k <= max_outer; ++k) {
+    for (int k = 0; k <= max_eval; ++k) {
+      // r = b - K u in FP64, element by element (the refinement residual)
       apply_fp64_operator(c, u, c->w_t.get());
       ufc::residual64(c, b, c->w_t.get(), u, r);
       rnorm = ufc::norm2_64(c, r) / bnorm;
@@ -893,33 +905,60 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
         code = UFC_STATUS_OK;
         break;
       }
-      if (k == max_outer) break;
-      // Inner tolerance of this step: inner_rtol, loosened to 0.1 rtol/||r||
-      // for a step that can finish the solve (no inner iterations spent below
-      // what the outer target needs).  Tightening below inner_rtol does not
-      // pay: an FP32 inner solve contracts the FP64 residual by ~1e-5 at best
-      // on these problems, whatever its own tolerance.
-      double tol_k = inner;
-      const double loose = 0.1 * algo->rtol / rnorm;
-      if (loose > inner) tol_k = std::min(loose, 0.1);
+      // a refinement step begins at the first residual and whenever the
+      // current one has reached inner_rtol of the step's starting residual
+      const bool new_step = k == 0 || !reliable || rnorm <= inner * step_start;
+      if (new_step) {
+        if (nouter == max_outer) break;
+        ++nouter;
+        step_start = rnorm;
+      } else {
+        ++nrepl;
+      }
+      if (k == max_eval) break;
+      double tol_k;
+      if (reliable) {
+        // FP32 PCG with residual replacement (reliable updates): the inner
+        // CG runs until its FP32 recurrence residual has dropped by delta,
+        // its residual is then replaced by the FP64 one and it continues
+        // with the same search direction, re-orthogonalised against the new
+        // residual.  With delta ~0.1 the replaced residual differs from the
+        // recurrence one by ~1e-4 of its size, small enough to keep CG's
+        // conjugacy, so the whole solve costs ~FP64 PCG's iteration count.
+        // A segment stops early where the refinement step (inner_rtol) or
+        // the solve (rtol) ends.
+        const double need = std::max(inner * step_start, 0.5 * algo->rtol) / rnorm;
+        tol_k = need > delta ? std::min(need, 0.5) : delta;
+      } else {
+        // Restarted refinement: inner tolerance inner_rtol, loosened to
+        // 0.1 rtol/||r|| for a step that can finish the solve.  Tightening
+        // below inner_rtol does not pay: a restarted FP32 inner solve
+        // contracts the FP64 residual by ~1e-5 at best on these problems.
+        tol_k = inner;
+        const double loose = 0.1 * algo->rtol / rnorm;
+        if (loose > inner) tol_k = std::min(loose, 0.1);
+      }
       int it = 0;
       double rin = 0.0;
-      // d = PCG32(K32, r) from d = 0, accumulated straight into u (u += d in
-      // FP64).  Each step restarts the inner Krylov space: continuing the old
-      // search direction after a replacement of this size (the FP64 residual
-      // sits ~10x above the inner one) stalls CG.
-      const int ic = ufc::pcg32(c, r, u, false, false, algo->max_iter, tol_k, algo->check_every,
+      // d = PCG32(K32, r), accumulated straight into u (u += d in FP64).
+      // Without replacement each step restarts from p = z: continuing the old
+      // direction after a replacement as large as inner_rtol allows (the FP64
+      // residual sits ~10x above the FP32 recurrence one at 1e-5) stalls CG.
+      const bool resume = reliable && have_dir;
+      const int ic = ufc::pcg32(c, r, u, false, resume, algo->max_iter, tol_k, algo->check_every,
                                 &it, &rin);
+      have_dir = have_dir || it > 0;
       if (verbose())
-        std::fprintf(stderr, "[ufc_hip] refinement %d: ||r||/||b|| %.3e, inner tol %.2e, %d it\n",
-                     k, rnorm, tol_k, it);
+        std::fprintf(stderr,
+                     "[ufc_hip] refinement %d%s: ||r||/||b|| %.3e, segment tol %.2e, %d it\n",
+                     nouter, new_step ? "" : " (replacement)", rnorm, tol_k, it);
       niter += it;
-      ++nouter;
       if (ic == UFC_STATUS_ERROR) {
         code = UFC_STATUS_ERROR;
         break;
       }
     }
+    c->n_replace = nrepl;
   }
   UFC_HIP_CHECK(hipEventRecord(c->ev3, c->stream));
   UFC_HIP_CHECK(hipMemcpyAsync(s->x, c->w_x.get(), n * 8, hipMemcpyDefault, c->stream));

@@ -69,6 +69,7 @@ struct PcgScalars {
   double bnorm;   // ||b||
   double rnorm;   // ||r|| / ||b||
   double rtol;
+  double gamma;   // (r.p)/(r.z) after a residual replacement
   int32_t iter;   // completed iterations
   int32_t done;   // 0 running, 1 converged, 2 breakdown, 3 max_iter
   int32_t max_iter;
@@ -79,7 +80,7 @@ constexpr int kSolveBlocks = 2048;  // vector kernels: 8 workgroups per CU on 25
 constexpr int kSolveThreads = 256;  // vector kernels: 4 wave64 per workgroup
 constexpr int kSpmvThreads = 512;   // SpMV: 8 wave64 = 16 node groups per workgroup
 constexpr int kSpmvBlocks64 = 32768;  // FP64 CSR SpMV grid (measured best on gfx950)
-constexpr int kSpmvBlocks32 = 8192;   // FP32 SpMV grid
+constexpr int kSpmvBlocks32 = 16384;  // FP32 SpMV grid (measured best on gfx950)
 constexpr int kMaxPartials = 32768;   // per-workgroup partial sums
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
@@ -167,6 +168,7 @@ struct ufc_ctx {
   std::vector<char> spmv_active;    // per pair: 1 = the launch did work
   double assemble_ms = 0.0, solve_ms = 0.0, spmv_ms_avg = 0.0;
   int32_t spmv_calls = 0;
+  int32_t n_replace = 0;  // FP64 residual replacements of the last MIXED solve
 };
 
 namespace ufc {

@@ -87,6 +87,7 @@ MODULE ufc_hip_iface
     REAL(C_DOUBLE) :: inner_rtol = 1.0E-5_C_DOUBLE
     INTEGER(C_INT32_T) :: max_outer = 20
     INTEGER(C_INT32_T) :: check_every = 0
+    REAL(C_DOUBLE) :: replace_rtol = 0.0_C_DOUBLE
   END TYPE ufc_nm_solver_algo
 
   TYPE, BIND(C), PUBLIC :: ufc_nm_solver_state
@@ -111,13 +112,39 @@ MODULE ufc_hip_iface
     REAL(C_DOUBLE) :: solve_ms = 0.0_C_DOUBLE
     REAL(C_DOUBLE) :: spmv_ms_avg = 0.0_C_DOUBLE
     INTEGER(C_INT32_T) :: spmv_calls = 0
-    INTEGER(C_INT32_T) :: reserved = 0
+    INTEGER(C_INT32_T) :: n_replace = 0
   END TYPE ufc_timing
 
-  PUBLIC :: ufc_ctx_create, ufc_ctx_destroy, ufc_ctx_timing, ufc_rt_asm_build_pattern, &
-            ufc_rt_asm_get_pattern, ufc_rt_asm_assemble_k, ufc_rt_asm_apply_bc, &
-            ufc_rt_asm_load, ufc_rt_get_rhs, ufc_rt_residual, ufc_nm_spmv, ufc_nm_pcg, &
-            ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_status_message
+  ! PH_Elem_Eval_Ke_Arg (PH_Elem_Def.f90:218-222) for a batch of C3D8
+  TYPE, BIND(C), PUBLIC :: ufc_ph_ke_algo
+    INTEGER(C_INT32_T) :: precision = UFC_PREC_FP64
+    INTEGER(C_INT32_T) :: reserved = 0
+  END TYPE ufc_ph_ke_algo
+
+  TYPE, BIND(C), PUBLIC :: ufc_ph_ke_args
+    INTEGER(C_INT32_T) :: n_elems = 0
+    INTEGER(C_INT32_T) :: reserved = 0
+    TYPE(C_PTR) :: coords = C_NULL_PTR     ! (3, 8, n_elems)
+    TYPE(C_PTR) :: young = C_NULL_PTR      ! (n_elems)
+    REAL(C_DOUBLE) :: nu = 0.3_C_DOUBLE
+    TYPE(C_PTR) :: ke = C_NULL_PTR         ! (24, 24, n_elems)
+  END TYPE ufc_ph_ke_args
+
+  TYPE, BIND(C), PUBLIC :: ufc_ph_sens_args
+    TYPE(C_PTR) :: u = C_NULL_PTR          ! (n_dof)
+    TYPE(C_PTR) :: lambda = C_NULL_PTR     ! (n_dof)
+    TYPE(C_PTR) :: dedrho = C_NULL_PTR     ! (n_elems)
+    REAL(C_DOUBLE) :: nu = 0.3_C_DOUBLE
+    TYPE(C_PTR) :: grad = C_NULL_PTR       ! (n_elems) out
+  END TYPE ufc_ph_sens_args
+
+  PUBLIC :: ufc_ctx_create, ufc_ctx_destroy, ufc_ctx_info, ufc_ctx_timing, &
+            ufc_ctx_enable_timing, ufc_ph_c3d8_eval_ke, ufc_ph_c3d8_sensitivity, &
+            ufc_rt_asm_build_pattern, ufc_rt_asm_get_pattern, ufc_rt_asm_assemble_k, &
+            ufc_rt_asm_get_values, ufc_rt_asm_apply_bc, ufc_rt_asm_load, ufc_rt_set_rhs, &
+            ufc_rt_get_rhs, ufc_rt_residual, ufc_nm_spmv, ufc_nm_pcg, ufc_nm_adjoint_solve, &
+            ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_ph_j2_get_state, &
+            ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_status_message
 
   INTERFACE
     INTEGER(C_INT) FUNCTION ufc_ctx_create(device, ctx, st) BIND(C, NAME='ufc_ctx_create')
@@ -227,6 +254,95 @@ MODULE ufc_hip_iface
     INTEGER(C_INT) FUNCTION ufc_ph_j2_commit(ctx, st) BIND(C, NAME='ufc_ph_j2_commit')
       IMPORT :: C_INT, C_PTR, ufc_status
       TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_j2_get_state(ctx, state, st) BIND(C, NAME='ufc_ph_j2_get_state')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx, state
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_j2_set_state(ctx, state, st) BIND(C, NAME='ufc_ph_j2_set_state')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx, state
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ctx_info(ctx, info, st) BIND(C, NAME='ufc_ctx_info')
+      IMPORT :: C_INT, C_PTR, ufc_rt_asm_info, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_rt_asm_info), INTENT(INOUT) :: info
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ctx_enable_timing(ctx, per_kernel, st) &
+        BIND(C, NAME='ufc_ctx_enable_timing')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: per_kernel
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_c3d8_eval_ke(ctx, algo, args, st) &
+        BIND(C, NAME='ufc_ph_c3d8_eval_ke')
+      IMPORT :: C_INT, C_PTR, ufc_ph_ke_algo, ufc_ph_ke_args, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_ph_ke_algo), INTENT(IN) :: algo
+      TYPE(ufc_ph_ke_args), INTENT(INOUT) :: args
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ph_c3d8_sensitivity(ctx, args, st) &
+        BIND(C, NAME='ufc_ph_c3d8_sensitivity')
+      IMPORT :: C_INT, C_PTR, ufc_ph_sens_args, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_ph_sens_args), INTENT(IN) :: args
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_get_values(ctx, precision, values, st) &
+        BIND(C, NAME='ufc_rt_asm_get_values')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: precision
+      TYPE(C_PTR), VALUE :: values
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_set_rhs(ctx, f, st) BIND(C, NAME='ufc_rt_set_rhs')
+      IMPORT :: C_INT, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx, f
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_nm_adjoint_solve(ctx, algo, dJdu, state, st) &
+        BIND(C, NAME='ufc_nm_adjoint_solve')
+      IMPORT :: C_INT, C_PTR, ufc_nm_solver_algo, ufc_nm_solver_state, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_nm_solver_algo), INTENT(IN) :: algo
+      TYPE(C_PTR), VALUE :: dJdu
+      TYPE(ufc_nm_solver_state), INTENT(INOUT) :: state
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_mesh_box(nx, ny, nz, lx, ly, lz, jitter, seed, index_base, &
+                                         coords, conn, st) BIND(C, NAME='ufc_mesh_box')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR, ufc_status
+      INTEGER(C_INT32_T), VALUE :: nx, ny, nz
+      REAL(C_DOUBLE), VALUE :: lx, ly, lz, jitter
+      INTEGER(C_INT64_T), VALUE :: seed
+      INTEGER(C_INT32_T), VALUE :: index_base
+      TYPE(C_PTR), VALUE :: coords, conn
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_field(kind, n, seed, a, b, out, st) BIND(C, NAME='ufc_field')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR, ufc_status
+      INTEGER(C_INT32_T), VALUE :: kind
+      INTEGER(C_INT64_T), VALUE :: n, seed
+      REAL(C_DOUBLE), VALUE :: a, b
+      TYPE(C_PTR), VALUE :: out
       TYPE(ufc_status), INTENT(INOUT) :: st
     END FUNCTION
   END INTERFACE

@@ -115,7 +115,6 @@ PROGRAM ufc_j2_step
   aargs%nu = props%nu
   bc%index_base = 1
   bc%eq = C_LOC(bc_eq)
-  bc%value = C_LOC(bc_val)
   bc%penalty = 1.0E12_C_DOUBLE
   salgo%precision = prec
   salgo%rtol = 1.0E-10_C_DOUBLE
@@ -132,7 +131,10 @@ PROGRAM ufc_j2_step
   DO WHILE (lam < 1.0_C_DOUBLE - 1.0E-12_C_DOUBLE)
     dlam = MIN(step, 1.0_C_DOUBLE - lam)
     target = lam + dlam
-    bc_val = bc_target * target
+    ! element-wise into the fixed array: a whole-array assignment from the
+    ! longer bc_target would reallocate bc_val and leave bc%value dangling
+    bc_val(1:bc%n_bc) = bc_target(1:bc%n_bc) * target
+    bc%value = C_LOC(bc_val)
     rc = ufc_rt_asm_apply_bc(ctx, bc, st)
     CALL check('ufc_rt_asm_apply_bc')
     u_n = u
@@ -142,7 +144,10 @@ PROGRAM ufc_j2_step
     rrel = HUGE(1.0_C_DOUBLE)
     DO it = 1, max_newton
       rc = ufc_ph_j2_update(ctx, C_LOC(u_try), C_LOC(u_n), n_yield, st)
-      IF (rc /= UFC_STATUS_OK) EXIT            ! local Newton failure: cut back
+      IF (rc /= UFC_STATUS_OK) THEN            ! local Newton failure: cut back
+        WRITE(*, '(A,I0,A,A)') 'ufc_ph_j2_update status ', rc, ': ', TRIM(ufc_status_message(st))
+        EXIT
+      END IF
       rc = ufc_rt_residual(ctx, C_LOC(u_try), C_LOC(r), C_LOC(norms), st)
       CALL check('ufc_rt_residual')
       IF (norms(2) > 0.0_C_DOUBLE) rrel = norms(1) / norms(2)
@@ -157,7 +162,10 @@ PROGRAM ufc_j2_step
       sstate%x0_zero = 1
       rc = ufc_nm_pcg(ctx, salgo, sstate, st)
       lin = lin + sstate%niter
-      IF (rc /= UFC_STATUS_OK) EXIT
+      IF (rc /= UFC_STATUS_OK) THEN
+        WRITE(*, '(A,I0,A,A)') 'ufc_nm_pcg status ', rc, ': ', TRIM(ufc_status_message(st))
+        EXIT
+      END IF
       u_try = u_try + du
     END DO
     total_linear = total_linear + lin
@@ -172,6 +180,8 @@ PROGRAM ufc_j2_step
       WRITE(*, '(A,I0,A,F8.5,A,I0,A,I0,A,ES10.3,A,I0)') 'increment ', n_done, ' lambda ', lam, &
            ' newton ', it, ' linear ', lin, ' rrel ', rrel, ' n_yield ', n_yield
     ELSE
+      WRITE(*, '(A,F8.5,A,I0,A,ES10.3)') 'cutback at lambda ', target, ' after newton ', it, &
+           ' rrel ', rrel
       cutbacks = cutbacks + 1
       IF (cutbacks > max_cutbacks) THEN
         failed = .TRUE.

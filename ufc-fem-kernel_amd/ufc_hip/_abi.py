@@ -90,7 +90,7 @@ class LoadArgs(C.Structure):
 class SolverAlgo(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_iter", C.c_int32), ("rtol", C.c_double),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("replace_rtol", C.c_double)]
 
 
 class SolverState(C.Structure):
@@ -102,7 +102,7 @@ class SolverState(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("assemble_ms", C.c_double), ("solve_ms", C.c_double),
                 ("spmv_ms_avg", C.c_double), ("spmv_calls", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("n_replace", C.c_int32)]
 
 
 P = C.POINTER

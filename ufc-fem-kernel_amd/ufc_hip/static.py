@@ -286,7 +286,7 @@ class Context:
         return t
 
     def _solve(self, fname, extra, precision, rtol, max_iter, inner_rtol, max_outer, x, b,
-               x0, check_every):
+               x0, check_every, replace_rtol):
         arg = _Arg()
         out = np.empty(self.n_dof) if x is None else x
         xp = arg.out(out, np.float64)
@@ -298,7 +298,8 @@ class Context:
             else:
                 out[...] = np.asarray(x0)
             x0_zero = 0
-        algo = A.SolverAlgo(precision, max_iter, rtol, inner_rtol, max_outer, check_every)
+        algo = A.SolverAlgo(precision, max_iter, rtol, inner_rtol, max_outer, check_every,
+                            replace_rtol)
         s = A.SolverState(xp, arg.ptr(b, np.float64) if b is not None else None, 0, 0, 0.0, 0,
                           x0_zero)
         args = [C.byref(algo)] + ([arg.ptr(extra, np.float64)] if extra is not None else []) + \
@@ -310,17 +311,18 @@ class Context:
                               t.solve_ms, t.spmv_ms_avg, t.spmv_calls)
 
     def pcg(self, precision=A.PREC_FP64, rtol=1e-10, max_iter=100000, inner_rtol=1e-5,
-            max_outer=20, x=None, b=None, x0=None, check_every=0):
+            max_outer=20, x=None, b=None, x0=None, check_every=0, replace_rtol=0.0):
         """Jacobi-PCG on the assembled K; returns (x, SolveInfo).  status is
         OK, INVALID (max_iter reached, NM_Solv_Core.f90:419-422) or ERROR
-        (breakdown)."""
+        (breakdown).  MIXED: replace_rtol > 0 selects residual replacement
+        with the search direction kept instead of restarted refinement steps."""
         return self._solve("ufc_nm_pcg", None, precision, rtol, max_iter, inner_rtol,
-                           max_outer, x, b, x0, check_every)
+                           max_outer, x, b, x0, check_every, replace_rtol)
 
     def adjoint_solve(self, dJdu, precision=A.PREC_FP64, rtol=1e-10, max_iter=100000,
-                      inner_rtol=1e-5, max_outer=20, x=None, check_every=0):
+                      inner_rtol=1e-5, max_outer=20, x=None, check_every=0, replace_rtol=0.0):
         return self._solve("ufc_nm_adjoint_solve", dJdu, precision, rtol, max_iter, inner_rtol,
-                           max_outer, x, None, None, check_every)
+                           max_outer, x, None, None, check_every, replace_rtol)
 
     def info(self):
         i = A.AsmInfo()
