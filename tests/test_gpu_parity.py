@@ -28,7 +28,40 @@ def _problem_small(kind):
         return P.cube_pressure(8, name="cube8")
     if kind == "jitter6":
         return P.cube_pressure(6, name="jitter6", jitter=0.1)
+    if kind == "irregular":
+        return _irregular()
     raise KeyError(kind)
+
+
+def _irregular():
+    """Notched, jittered 8x6x5 block with shuffled node and element numbering:
+    node degrees from 1 to 8 elements, ragged rows, no structure in the ids."""
+    from ufc_hip import problems as P
+    base = P.cube_pressure(1, name="irr")  # for the Problem shape only
+    import ufc_hip
+    nx, ny, nz = 8, 6, 5
+    coords, conn = ufc_hip.mesh_box(nx, ny, nz, 2.0, 1.5, 1.0, 0.1, 77)
+    e = np.arange(nx * ny * nz).reshape(nz, ny, nx)
+    keep = np.ones_like(e, bool)
+    keep[2:, 3:, 4:] = False      # notch through the top corner
+    keep[0, 0, nx - 1] = False    # a missing corner element
+    conn = conn[e[keep].ravel()]
+    used = np.unique(conn)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(used.size)          # old id used[i] -> new id perm[i]
+    new_of = np.full(coords.shape[0], -1, np.int64)
+    new_of[used] = perm
+    coords_n = np.empty((used.size, 3))
+    coords_n[perm] = coords[used]
+    conn_n = new_of[conn].astype(np.int32)[rng.permutation(conn.shape[0])]
+    z0 = np.nonzero(coords_n[:, 2] == 0.0)[0]
+    clamp = (3 * z0[:, None] + np.arange(3)).ravel().astype(np.int32)
+    top = np.nonzero(np.all(coords_n[conn_n[:, 4:8], 2] == 1.0, axis=1))[0].astype(np.int32)
+    young = 1e9 * (1.0 + rng.random(conn_n.shape[0]))
+    base.name, base.coords, base.conn, base.young = "irregular", coords_n, conn_n, young
+    base.bc_eq, base.bc_val = clamp, np.zeros(clamp.size)
+    base.load_elem, base.load_face = top, np.full(top.size, 1, np.int32)
+    return base
 
 
 def _oracle_system(orc, pr, penalty=1e12):
@@ -81,7 +114,7 @@ def test_pattern_bitexact_vs_reference(ctx, golden_dir, mesh):
     assert np.array_equal(rp, g["ia"]) and np.array_equal(ci, g["ja"])
 
 
-@pytest.mark.parametrize("kind", ["cantilever", "cube8", "jitter6"])
+@pytest.mark.parametrize("kind", ["cantilever", "cube8", "jitter6", "irregular"])
 def test_assembly_fp64_fp32_vs_oracle(ctx, orc, kind):
     from ufc_hip import problems as P
     pr = _problem_small(kind)
@@ -128,7 +161,7 @@ def test_spmv_fp64_fp32(ctx, orc):
     assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
 
 
-@pytest.mark.parametrize("kind", ["cantilever", "jitter6"])
+@pytest.mark.parametrize("kind", ["cantilever", "jitter6", "irregular"])
 def test_pcg_fp64_vs_oracle(ctx, orc, kind):
     from ufc_hip import problems as P
     pr = _problem_small(kind)
@@ -149,12 +182,14 @@ def test_pcg_fp64_vs_oracle(ctx, orc, kind):
     assert np.linalg.norm(u2 - uo2) / np.linalg.norm(uo2) < 1e-10
 
 
-@pytest.mark.parametrize("replace_rtol", [0.0, 0.1])
-def test_pcg_mixed_refinement_vs_oracle(ctx, orc, replace_rtol):
+@pytest.mark.parametrize("kind,replace_rtol", [("cube12", 1.0), ("cube12", 0.1),
+                                               ("irregular", 0.1)])
+def test_pcg_mixed_refinement_vs_oracle(ctx, orc, kind, replace_rtol):
     """FP32 inner PCG in FP64 refinement, both forms: restarted steps to
-    inner_rtol, and residual replacement every 0.1 with the direction kept."""
+    inner_rtol (replace_rtol >= 1), and residual replacement every 0.1 with
+    the direction kept."""
     from ufc_hip import problems as P
-    pr = P.cube_pressure(12, name="cube12")
+    pr = P.cube_pressure(12, name="cube12") if kind == "cube12" else _irregular()
     rp, ci, val, F, _ = _oracle_system(orc, pr)
     uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
     P.setup_context(ctx, pr, 2)

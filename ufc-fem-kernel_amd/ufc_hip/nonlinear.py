@@ -10,7 +10,8 @@ reference root, with the hot parts on the GPU:
     ufc_rt_residual      R = F + P ubar - F_int(sigma) - P u   (RT_Asm_ComputeResidual)
     converged when ||R|| <= tol_r ||F_int|| (after the first iteration)
     ufc_rt_asm_assemble_k(UFC_MAT_J2)  tangent K_t into the fixed CSR pattern
-    ufc_nm_pcg           K_t du = R   (FP64 or FP32-in-FP64 refinement)
+    ufc_nm_pcg           K_t du = R   (FP64 or FP32-in-FP64 refinement), to the
+                         inexact-Newton forcing term (Eisenstat-Walker)
     u += du
   per increment: ufc_ph_j2_commit on convergence; a failed increment
   (local-Newton failure or no convergence) is cut back by 0.25
@@ -46,10 +47,28 @@ class NewtonResult:
     u: np.ndarray = None
 
 
+def forcing_term(r_now, r_prev, eta_prev, target, eta_max=0.1, eta_min=1e-10):
+    """Linear-solve tolerance of an inexact Newton step (Eisenstat & Walker
+    1996, choice 2: eta = 0.9 (|R_k|/|R_k-1|)^2, safeguarded by
+    0.9 eta_prev^2 when that is larger), capped at eta_max and never tighter
+    than what reaching `target` from |R_k| needs (0.5 target/|R_k|)."""
+    if r_prev is None:
+        eta = eta_max
+    else:
+        eta = 0.9 * (r_now / r_prev) ** 2
+        if eta_prev is not None and 0.9 * eta_prev ** 2 > 0.1:
+            eta = max(eta, 0.9 * eta_prev ** 2)
+        eta = min(eta, eta_max)
+    eta = max(eta, 0.5 * target / max(r_now, 1e-300), eta_min)
+    return min(eta, eta_max)
+
+
 def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
-              tol_r=1e-8, lin_rtol=1e-10, max_newton=25, penalty=1.0e12, max_cutbacks=4):
+              tol_r=1e-8, lin_rtol=None, max_newton=25, penalty=1.0e12, max_cutbacks=4):
     """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
-    in n_increments equal increments.  ctx must hold the mesh (build_pattern)."""
+    in n_increments equal increments.  ctx must hold the mesh (build_pattern).
+    lin_rtol None: inexact Newton, each linear solve to the forcing term of
+    forcing_term(); a number: every linear solve to that relative tolerance."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
     ctx.j2_init(*props)
@@ -66,6 +85,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
         u_n = u.copy()
         ok, its, lin, rrel, ny = False, 0, 0, np.inf, 0
         u_try = u_n.copy()
+        r_prev = eta = None
         for it in range(1, max_newton + 1):
             try:
                 ny = ctx.j2_update(u_try, u_n)
@@ -77,8 +97,13 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             if it > 1 and norms[0] <= tol_r * max(norms[1], 1e-300):
                 ok = True
                 break
+            if lin_rtol is None:
+                eta = forcing_term(norms[0], r_prev, eta, tol_r * norms[1])
+            else:
+                eta = lin_rtol
+            r_prev = norms[0]
             ctx.assemble_k(None, props[1], precision=precision, material=A.MAT_J2)
-            du, info = ctx.pcg(precision=precision, rtol=lin_rtol, b=r)
+            du, info = ctx.pcg(precision=precision, rtol=eta, b=r)
             lin += info.niter
             if not info.converged:
                 break
