@@ -300,7 +300,7 @@ def main():
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(pr.n_dof),
                        "copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs,
-                       "kernel": "k_spmv<float,true> (FP32 node-block SpMV + p.Ap partials)",
+                       "kernel": "k_spmv<float,true> (FP32 node-block SpMV, difference form)",
                        "bytes_per_launch": b32, "ms_per_launch": spmv_ms,
                        "traffic_source": "profiles/spmv_pcg128_hbm_traffic.md (rocprofv3 "
                                          "FETCH_SIZE / WRITE_SIZE, copy-calibrated)"}

@@ -81,7 +81,7 @@ constexpr int kSolveThreads = 256;  // vector kernels: 4 wave64 per workgroup
 constexpr int kSpmvThreads = 512;   // SpMV: 8 wave64 = 16 node groups per workgroup
 constexpr int kSpmvBlocks64 = 32768;  // FP64 CSR SpMV grid (measured best on gfx950)
 constexpr int kSpmvBlocks32 = 16384;  // FP32 SpMV grid (measured best on gfx950)
-constexpr int kMaxPartials = 32768;   // per-workgroup partial sums
+constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vector kernels)
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
 

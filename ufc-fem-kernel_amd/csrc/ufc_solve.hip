@@ -7,8 +7,9 @@
 // with ||r|| / ||b|| < rtol (||b|| < 1e-30 -> 1), z = r / diag,
 // beta = (r.z)_new / (r.z)_old, p = z + beta p.
 //
-// One iteration is five launches; every scalar stays on the device:
-//   spmv_dot  : Ap = K p and per-workgroup partials of p.Ap
+// One iteration is six launches; every scalar stays on the device:
+//   spmv      : Ap = K p
+//   dot_pap   : per-workgroup partials of p.Ap
 //   fin_alpha : one workgroup adds the partials in index order, alpha
 //   update    : x += alpha p, r -= alpha Ap, partials of r.r and r.(r/diag)
 //   fin_beta  : ||r||/||b||, convergence / breakdown / max_iter, beta
@@ -150,31 +151,39 @@ __device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __res
   for (int d = 0; d < 3; ++d) a[d] = sum_lanes(a[d], GROUP);
 }
 
-// y = K x; with sc != null also per-workgroup partials of x.y (p.Ap in PCG)
-// and an early return once the solve is done
+// y = K x; with sc != null an early return once the solve is done
 template <typename T, bool BLOCK_INDEX>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
     int32_t n_nodes, const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
     const int32_t* __restrict__ col_idx, const T* __restrict__ val, const T* __restrict__ x,
-    T* __restrict__ y, double* __restrict__ partials, const PcgScalars* __restrict__ sc) {
+    T* __restrict__ y, const PcgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
   const int k = threadIdx.x & (GROUP - 1);
   const int64_t g0 = int64_t(blockIdx.x) * GROUPS_PER_BLOCK + threadIdx.x / GROUP;
   const int64_t ng = int64_t(gridDim.x) * GROUPS_PER_BLOCK;
-  double dot = 0.0;
   for (int64_t n = g0; n < n_nodes; n += ng) {
     T a[3];
     node_rows<T, BLOCK_INDEX>(n, k, nb_ptr, nb_idx, col_idx, val, x, a);
-    if (k < 3) {
-      const T yv = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
-      y[3 * n + k] = yv;
-      if (sc) dot += double(yv) * double(x[3 * n + k]);
-    }
+    if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
   }
-  if (sc) {
-    double v[1] = {dot};
-    write_partials<1, kSpmvThreads>(v, partials);
-  }
+}
+
+// partials of p.Ap.  A pass of its own: inside the SpMV the per-workgroup
+// reduction's barrier holds every group of a workgroup until the slowest has
+// finished, which cost 0.04 ms per FP32 SpMV at 128^3; re-reading p and Ap
+// (2 x 26 MB) costs 0.01 ms.
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void k_dot_pap(int64_t n, const T* __restrict__ p,
+                                                          const T* __restrict__ ap,
+                                                          double* __restrict__ partials,
+                                                          const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  double s = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * kSolveThreads)
+    s += double(p[i]) * double(ap[i]);
+  double v[1] = {s};
+  write_partials<1, kSolveThreads>(v, partials);
 }
 
 __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restrict__ partials,
@@ -441,10 +450,10 @@ int spmv_blocks(const ufc_ctx* c) {
 }
 
 template <typename T, bool BLOCK_INDEX>
-void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool with_dot) {
+void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg) {
   hipLaunchKernelGGL((k_spmv<T, BLOCK_INDEX>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads), 0,
                      c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(), c->col_idx.get(),
-                     val, x, y, c->partials.get(), with_dot ? c->scalars.get() : nullptr);
+                     val, x, y, in_pcg ? c->scalars.get() : nullptr);
 }
 
 // resume: continue a CG after a residual replacement -- r := b (the new true
@@ -464,7 +473,6 @@ int pcg_run(ufc_ctx* c, const double* b, double* x, const T* ax_or_null, T* r, T
   if (resume)
     hipLaunchKernelGGL(k_preplace<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                        p, r, diag, sc);
-  const int sblocks = spmv_blocks<T>(c);
   UFC_HIP_CHECK(hipGetLastError());
   const int every = check_every > 0 ? check_every : 32;
   const size_t first_event = c->spmv_active.size();
@@ -474,7 +482,10 @@ int pcg_run(ufc_ctx* c, const double* b, double* x, const T* ax_or_null, T* r, T
       spmv_event_begin(c);
       launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
       spmv_event_end(c);
-      hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part, sblocks, sc);
+      hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
+                         p, ap, part, sc);
+      hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part,
+                         kSolveBlocks, sc);
       hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                          x, r, p, ap, diag, part, sc);
       hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc);

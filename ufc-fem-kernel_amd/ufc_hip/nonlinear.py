@@ -47,20 +47,24 @@ class NewtonResult:
     u: np.ndarray = None
 
 
-def forcing_term(r_now, r_prev, eta_prev, target, eta_max=0.1, eta_min=1e-10):
-    """Linear-solve tolerance of an inexact Newton step (Eisenstat & Walker
-    1996, choice 2: eta = 0.9 (|R_k|/|R_k-1|)^2, safeguarded by
-    0.9 eta_prev^2 when that is larger), capped at eta_max and never tighter
-    than what reaching `target` from |R_k| needs (0.5 target/|R_k|)."""
+def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
+                 eta_min=1e-10):
+    """Linear-solve tolerance of an inexact Newton step.  The first iteration
+    of an increment carries the jump of the prescribed values (a residual on
+    the penalty rows only) and is solved to eta_first, so that the interior
+    sees the new boundary displacement before plasticity is evaluated on it;
+    later iterations use Eisenstat & Walker (1996) choice 2,
+    eta = 0.9 (|R_k|/|R_k-1|)^2 (safeguarded by 0.9 eta_prev^2), capped at
+    eta_max and never tighter than reaching `target` from |R_k| needs."""
     if r_prev is None:
-        eta = eta_max
+        eta = eta_first
     else:
         eta = 0.9 * (r_now / r_prev) ** 2
         if eta_prev is not None and 0.9 * eta_prev ** 2 > 0.1:
             eta = max(eta, 0.9 * eta_prev ** 2)
         eta = min(eta, eta_max)
     eta = max(eta, 0.5 * target / max(r_now, 1e-300), eta_min)
-    return min(eta, eta_max)
+    return min(eta, 0.5)
 
 
 def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
