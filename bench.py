@@ -62,6 +62,18 @@ def measured_traffic(n_dof, kernel="k_spmv<float, true>"):
         return json.load(f)["kernels"][kernel]["traffic_bytes"]
 
 
+def measured_assembly_flop(n_elems):
+    """Executed FLOPs per assembly launch from the committed rocprofv3 VALU
+    counter summary, when it was taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "assembly128_valu_flops.json")
+    if n_elems != 2097152 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"]
+    return {"fp32": k["k_assemble<float, 0>"]["executed_flop"],
+            "fp64": k["k_assemble<double, 0>"]["executed_flop"]}
+
+
 def run_step(ctx, pr, precision, rtol, torch, x, replace_rtol=0.0):
     import ufc_hip
     torch.cuda.synchronize()
@@ -315,6 +327,20 @@ def main():
         asm["fp64_ms"] = out["fp64"]["assemble_ms"]
         asm["fp64_tflops"] = asm["flop"] / (asm["fp64_ms"] * 1e9)
         asm["fp64_frac"] = asm["fp64_tflops"] / asm["peak_vector_tflops"]
+    # the C-ABI also takes host arrays: the same assembly with E in host memory
+    # (the PCIe-inclusive form of the step; the timed steps pass device data)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.assemble_k(pr.young_dev, pr.nu, precision=ufc_hip.PREC_MIXED)
+    t1 = time.perf_counter()
+    ctx.assemble_k(np.ascontiguousarray(pr.young), pr.nu, precision=ufc_hip.PREC_MIXED)
+    t2 = time.perf_counter()
+    asm["host_young_extra_ms"] = ((t2 - t1) - (t1 - t0)) * 1e3
+    ex = measured_assembly_flop(pr.n_elems)
+    if ex is not None:
+        asm["executed_flop"] = ex
+        asm["executed_flop_source"] = ("profiles/assembly128_valu_flops.md (rocprofv3 "
+                                       "SQ_INSTS_VALU_FLOPS_FP32/FP64 x 64 lanes)")
     out["assembly"] = asm
     ctx.close()
     del pr.young_dev

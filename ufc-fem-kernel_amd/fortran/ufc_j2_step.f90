@@ -11,7 +11,9 @@
 !       ufc_rt_residual        R = F + P ubar - F_int - P u
 !       converged when ||R|| <= 1e-8 ||F_int|| (after the first iteration)
 !       ufc_rt_asm_assemble_k  tangent K_t (UFC_MAT_J2) into the fixed CSR
-!       ufc_nm_pcg             K_t du = R, u += du
+!       ufc_nm_pcg             K_t du = R to the inexact-Newton forcing term
+!                              (first iteration 1e-6, then Eisenstat-Walker
+!                              choice 2 capped at 1e-3), u += du
 !     ufc_ph_j2_commit on convergence; otherwise the increment is cut back by
 !     0.25 from the committed state (pnewdt, PH_Mat_Plast_J2_Iso_Core.f90:419-427).
 !
@@ -40,7 +42,7 @@ PROGRAM ufc_j2_step
   INTEGER(C_INT32_T) :: nn, ne, prec, rc
   INTEGER(C_INT64_T) :: n_yield
   INTEGER(C_INT64_T) :: ndof
-  REAL(C_DOUBLE) :: lam, dlam, step, target, rrel
+  REAL(C_DOUBLE) :: lam, dlam, step, target, rrel, eta, eta_prev, r_prev
   REAL(C_DOUBLE), TARGET :: norms(2)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: coords(:, :), u(:), u_n(:), u_try(:), r(:), du(:)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: bc_target(:), bc_val(:)
@@ -117,7 +119,6 @@ PROGRAM ufc_j2_step
   bc%eq = C_LOC(bc_eq)
   bc%penalty = 1.0E12_C_DOUBLE
   salgo%precision = prec
-  salgo%rtol = 1.0E-10_C_DOUBLE
   salgo%inner_rtol = 1.0E-5_C_DOUBLE
 
   u = 0.0_C_DOUBLE
@@ -142,6 +143,8 @@ PROGRAM ufc_j2_step
     ok = .FALSE.
     lin = 0
     rrel = HUGE(1.0_C_DOUBLE)
+    r_prev = -1.0_C_DOUBLE
+    eta_prev = -1.0_C_DOUBLE
     DO it = 1, max_newton
       rc = ufc_ph_j2_update(ctx, C_LOC(u_try), C_LOC(u_n), n_yield, st)
       IF (rc /= UFC_STATUS_OK) THEN            ! local Newton failure: cut back
@@ -155,6 +158,24 @@ PROGRAM ufc_j2_step
         ok = .TRUE.
         EXIT
       END IF
+      ! forcing term: the first iteration carries the jump of the prescribed
+      ! values and is solved to 1e-6; then eta = 0.9 (|R_k|/|R_k-1|)^2
+      ! (Eisenstat & Walker 1996), safeguarded, capped at 1e-3, and never
+      ! tighter than reaching the Newton target needs
+      IF (r_prev < 0.0_C_DOUBLE) THEN
+        eta = 1.0E-6_C_DOUBLE
+      ELSE
+        eta = 0.9_C_DOUBLE * (norms(1) / r_prev)**2
+        IF (eta_prev > 0.0_C_DOUBLE .AND. 0.9_C_DOUBLE * eta_prev**2 > 0.1_C_DOUBLE) &
+          eta = MAX(eta, 0.9_C_DOUBLE * eta_prev**2)
+        eta = MIN(eta, 1.0E-3_C_DOUBLE)
+      END IF
+      eta = MAX(eta, 0.5_C_DOUBLE * tol_r * norms(2) / MAX(norms(1), TINY(1.0_C_DOUBLE)), &
+                1.0E-10_C_DOUBLE)
+      eta = MIN(eta, 0.5_C_DOUBLE)
+      r_prev = norms(1)
+      eta_prev = eta
+      salgo%rtol = eta
       rc = ufc_rt_asm_assemble_k(ctx, aalgo, aargs, st)
       CALL check('ufc_rt_asm_assemble_k')
       sstate%x = C_LOC(du)
