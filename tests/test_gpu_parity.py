@@ -274,3 +274,45 @@ def test_j2_update_and_newton_vs_oracle(ctx, orc):
     sg = ctx.j2_get_state().ravel()
     so = state
     assert np.abs(sg - so).max() < 1e-6 * np.abs(so).max()
+
+
+def test_pcg_zero_rhs_and_converged_start(ctx, orc):
+    """b = 0 gives x = 0 with no iteration (NM_Solver_PCG would form 0/0);
+    a start at the solution returns it unchanged with no iteration."""
+    from ufc_hip import problems as P
+    pr = _problem_small("cube8")
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    zero = np.zeros(pr.n_dof)
+    u, info = ctx.pcg(precision=0, rtol=1e-10, b=zero)
+    assert info.status == 0 and info.converged and info.niter == 0 and not np.any(u)
+    u_ref, info_ref = ctx.pcg(precision=0, rtol=1e-13)
+    u2, info2 = ctx.pcg(precision=0, rtol=1e-10, x0=u_ref)
+    assert info2.converged and info2.niter == 0 and np.array_equal(u2, u_ref)
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    u3, info3 = ctx.pcg(precision=2, rtol=1e-10, b=zero)
+    assert info3.converged and info3.niter == 0 and not np.any(u3)
+
+
+def test_pattern_rejects_invalid_meshes(ctx):
+    """Host-side validation of the mesh: status codes, no device work."""
+    import ufc_hip
+    from ufc_hip import _abi as A
+    coords, conn = ufc_hip.mesh_box(2, 2, 2)
+    bad = conn.copy()
+    bad[3, 5] = coords.shape[0]            # node id out of range
+    with pytest.raises(A.UfcError) as e:
+        ctx.build_pattern(coords, bad)
+    assert e.value.code == A.STATUS_INVALID
+    bad = conn.copy()
+    bad[1, 2] = bad[1, 0]                  # element repeats a node
+    with pytest.raises(A.UfcError) as e:
+        ctx.build_pattern(coords, bad)
+    assert e.value.code == A.STATUS_INVALID
+    many = np.concatenate([conn] + [conn[:1]] * 8)   # node 0 in 9 elements
+    with pytest.raises(A.UfcError) as e:
+        ctx.build_pattern(coords, many)
+    assert e.value.code == A.STATUS_UNSUPPORTED
+    # the context stays usable after the rejections
+    info = ctx.build_pattern(coords, conn)
+    assert info.nnz > 0

@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restr
 }
 
 // r = b - Ax (ax null: x0 = 0), p = z = r/diag (p kept when keep_p);
-// partials of r.z, b.b and, with keep_p, r.p
+// partials of r.z, b.b, r.p (with keep_p) and r.r
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void k_setup(int64_t n, const double* __restrict__ b,
                                                         const T* __restrict__ ax,
@@ -309,11 +309,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_setup(int64_t n, const double
                                                         const T* __restrict__ diag,
                                                         double* __restrict__ partials,
                                                         int keep_p) {
-  double rz = 0.0, bb = 0.0, rp = 0.0;
+  double rz = 0.0, bb = 0.0, rp = 0.0, rr = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * kSolveThreads) {
     const double bi = b[i];
     const T ri = ax ? T(bi - double(ax[i])) : T(bi);
+    rr += double(ri) * double(ri);
     r[i] = ri;
     const T d = diag[i];
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
@@ -324,8 +325,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_setup(int64_t n, const double
     rz += double(ri) * double(zi);
     bb += bi * bi;
   }
-  double v[3] = {rz, bb, rp};
-  write_partials<3, kSolveThreads>(v, partials);
+  double v[4] = {rz, bb, rp, rr};
+  write_partials<4, kSolveThreads>(v, partials);
 }
 
 // replace != 0: residual replacement in a running CG.  The new r.z enters
@@ -336,8 +337,8 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_setup(const double* __restr
                                                             PcgScalars* __restrict__ sc,
                                                             double rtol, int32_t max_iter,
                                                             int replace) {
-  double s[3];
-  sum_partials<3>(s, partials, kSolveBlocks);
+  double s[4];
+  sum_partials<4>(s, partials, kSolveBlocks);
   if (threadIdx.x == 0) {
     const double rz_old = sc->rz;
     sc->rz = s[0];
@@ -353,6 +354,13 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_setup(const double* __restr
     const bool ok = replace && rz_old > 0.0 && isfinite(rz_old) && s[0] > 0.0;
     sc->beta = ok ? s[0] / rz_old : 0.0;
     sc->gamma = ok ? s[2] / s[0] : 0.0;
+    // an initial residual already below rtol (b = 0, or x0 the solution)
+    // converges with 0 iterations; NM_Solver_PCG would form alpha = 0/0 here
+    const double rn0 = sqrt(s[3]) / bn;
+    if (sc->done == 0 && rn0 < rtol) {
+      sc->done = 1;
+      sc->rnorm = rn0;
+    }
   }
 }
 
@@ -553,7 +561,7 @@ void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32) {
     c->s_ap.resize(n);
     c->s_x.resize(n);
   }
-  c->partials.resize(size_t(kMaxPartials) * 3);
+  c->partials.resize(size_t(kMaxPartials) * 4);
   if (!c->scalars.get()) {
     c->scalars.resize(1);
     UFC_HIP_CHECK(hipMemsetAsync(c->scalars.get(), 0, sizeof(PcgScalars), c->stream));
