@@ -940,13 +940,23 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
       }
       int it = 0;
       double rin = 0.0;
-      // d = PCG32(K32, r), accumulated straight into u (u += d in FP64).
-      // Without replacement each step restarts from p = z: continuing the old
-      // direction after a replacement as large as inner_rtol allows (the FP64
-      // residual sits ~10x above the FP32 recurrence one at 1e-5) stalls CG.
+      // Reliable form: the segment's correction d is accumulated in FP32 and
+      // added to u in FP64 at its end (a 0.1 segment tolerates FP32 rounding
+      // in d; it saves 8 bytes per DOF and iteration).  Restarted form: d is
+      // accumulated straight into u in FP64, and each step restarts from
+      // p = z: continuing the old direction after a replacement as large as
+      // inner_rtol allows (the FP64 residual sits ~10x above the FP32
+      // recurrence one at 1e-5) stalls CG.
       const bool resume = reliable && have_dir;
-      const int ic = ufc::pcg32(c, r, u, false, resume, algo->max_iter, tol_k, algo->check_every,
-                                &it, &rin);
+      int ic;
+      if (reliable) {
+        ic = ufc::pcg32_segment(c, r, c->s_x.get(), resume, algo->max_iter, tol_k,
+                                algo->check_every, &it, &rin);
+        ufc::axpy_f32_to_f64(c, c->s_x.get(), u);
+      } else {
+        ic = ufc::pcg32(c, r, u, false, false, algo->max_iter, tol_k, algo->check_every, &it,
+                        &rin);
+      }
       have_dir = have_dir || it > 0;
       if (verbose())
         std::fprintf(stderr,

@@ -203,6 +203,10 @@ int pcg64(ufc_ctx* c, const double* b, double* x, bool x0_zero, int max_iter, do
 // after a residual replacement (b is the new true residual)
 int pcg32(ufc_ctx* c, const double* b64, double* x, bool zero_x, bool resume, int max_iter,
           double rtol, int check_every, int* niter, double* rnorm);
+// FP32 PCG segment whose correction d (FP32, zeroed first) the caller adds to
+// its FP64 iterate; resume as for pcg32
+int pcg32_segment(ufc_ctx* c, const double* b64, float* d, bool resume, int max_iter,
+                  double rtol, int check_every, int* niter, double* rnorm);
 void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u);
 void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o);
 void cast_f32_to_f64(ufc_ctx* c, const float* a, double* o);

@@ -230,12 +230,16 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restr
 }
 
 // x += alpha p, r -= alpha Ap; partials of r.r and r.z with z = r/diag.
-// The iterate x is FP64 in both solvers: an FP32 iterate carries rounding
-// noise of eps32 |x|, which K amplifies to eps32 * lambda_max |x| in the true
-// residual -- for the smooth fields of an elastic solve that stalls the FP64
-// refinement near eps32 * cond(K) instead of the inner tolerance.
-template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, double* __restrict__ x,
+// The iterate X: FP64 for the FP64 solver and for a restarted FP32 inner
+// solve -- an FP32 iterate carries rounding noise of eps32 |x|, which K
+// amplifies to eps32 * lambda_max |x| in the true residual, and for the
+// smooth fields of an elastic solve that stalls a refinement step near
+// eps32 * cond(K) (~1e-3) instead of its 1e-5 tolerance.  FP32 for an inner
+// segment between residual replacements: the segment's correction is flushed
+// into the FP64 u when its 0.1 reduction is reached, so the same noise is
+// ~1e-3 of a residual the segment only has to cut by 10.
+template <typename T, typename X>
+__global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restrict__ x,
                                                          T* __restrict__ r,
                                                          const T* __restrict__ p,
                                                          const T* __restrict__ ap,
@@ -248,7 +252,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, double* __r
   double rr = 0.0, rz = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * kSolveThreads) {
-    x[i] = x[i] + alpha * double(p[i]);
+    if constexpr (sizeof(X) == 8)
+      x[i] = x[i] + alpha * double(p[i]);
+    else
+      x[i] = x[i] + a * p[i];
     const T ri = r[i] - a * ap[i];
     r[i] = ri;
     const T d = diag[i];
@@ -494,8 +501,8 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg) {
 // resume: continue a CG after a residual replacement -- r := b (the new true
 // residual), p := z + beta (p - gamma z) keeps the search direction (and with
 // it the Krylov information) instead of restarting from p = z
-template <typename T, bool BLOCK_INDEX>
-int pcg_run(ufc_ctx* c, const double* b, double* x, const T* ax_or_null, T* r, T* p, T* ap,
+template <typename T, bool BLOCK_INDEX, typename X>
+int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, T* ap,
             const T* diag, const T* val, int max_iter, double rtol, int check_every,
             int* niter, double* rnorm, bool resume = false) {
   const int64_t n = c->n_dof;
@@ -521,7 +528,7 @@ int pcg_run(ufc_ctx* c, const double* b, double* x, const T* ax_or_null, T* r, T
                          p, ap, part, sc);
       hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part,
                          kSolveBlocks, sc);
-      hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
+      hipLaunchKernelGGL((k_update<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                          x, r, p, ap, diag, part, sc);
       hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc);
       hipLaunchKernelGGL(k_pupdate<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
@@ -628,9 +635,17 @@ int pcg32(ufc_ctx* c, const double* b64, double* x, bool zero_x, bool resume, in
   // the iteration starts from 0; without zero_x the correction accumulates
   // into the caller's x (the refinement's u += d)
   if (zero_x) UFC_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * size_t(c->n_dof), c->stream));
-  return pcg_run<float, true>(c, b64, x, nullptr, c->s_r.get(), c->s_p.get(), c->s_ap.get(),
-                              c->diag32.get(), c->val32.get(), max_iter, rtol, check_every,
-                              niter, rnorm, resume);
+  return pcg_run<float, true, double>(c, b64, x, nullptr, c->s_r.get(), c->s_p.get(),
+                                      c->s_ap.get(), c->diag32.get(), c->val32.get(), max_iter,
+                                      rtol, check_every, niter, rnorm, resume);
+}
+
+int pcg32_segment(ufc_ctx* c, const double* b64, float* d, bool resume, int max_iter,
+                  double rtol, int check_every, int* niter, double* rnorm) {
+  UFC_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(float) * size_t(c->n_dof), c->stream));
+  return pcg_run<float, true, float>(c, b64, d, nullptr, c->s_r.get(), c->s_p.get(),
+                                     c->s_ap.get(), c->diag32.get(), c->val32.get(), max_iter,
+                                     rtol, check_every, niter, rnorm, resume);
 }
 
 void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o) {
