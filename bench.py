@@ -41,10 +41,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def spmv32_bytes(n_dof, nnz, n_nodes):
+def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4):
     """Algorithmic bytes of one FP32 node-block SpMV: values + node index view
-    + node pointers + x + y."""
-    return nnz * 4 + (nnz // 9) * 4 + (n_nodes + 1) * 4 + 2 * n_dof * 4
+    (2-byte offsets or 4-byte ids per neighbour) + node pointers + x + y."""
+    return nnz * 4 + (nnz // 9) * index_bytes + (n_nodes + 1) * 4 + 2 * n_dof * 4
 
 
 def spmv64_bytes(n_dof, nnz):
@@ -52,14 +52,15 @@ def spmv64_bytes(n_dof, nnz):
     return nnz * (8 + 4) + (n_dof + 1) * 4 + 2 * n_dof * 8
 
 
-def measured_traffic(n_dof, kernel="k_spmv<float, true>"):
+def measured_traffic(n_dof, kernel):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, when it was taken on this workload."""
     path = os.path.join(ROOT, "profiles", "spmv_pcg128_hbm_traffic.json")
     if n_dof != 6440067 or not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["kernels"][kernel]["traffic_bytes"]
+        k = json.load(f)["kernels"]
+    return k[kernel]["traffic_bytes"] if kernel in k else None
 
 
 def measured_assembly_flop(n_elems):
@@ -306,13 +307,14 @@ def main():
                          for k, v in sols.items() if k != "fp64"}
     # roofline of the dominant kernel (FP32 SpMV + p.Ap inside the mixed solve)
     spmv_ms = mixed[-1]["spmv_ms"]
-    b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes)
+    b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ctx.nb_index_bytes)
+    k32 = "k_spmv<float, true, true>" if ctx.nb_index_bytes == 2 else "k_spmv<float, true, false>"
     ach = b32 / (spmv_ms * 1e6)
     copy_gbs = copy_bandwidth(torch, dev)
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(pr.n_dof),
+                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(pr.n_dof, k32),
                        "copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs,
-                       "kernel": "k_spmv<float,true> (FP32 node-block SpMV, difference form)",
+                       "kernel": k32 + " (FP32 node-block SpMV, difference form)",
                        "bytes_per_launch": b32, "ms_per_launch": spmv_ms,
                        "traffic_source": "profiles/spmv_pcg128_hbm_traffic.md (rocprofv3 "
                                          "FETCH_SIZE / WRITE_SIZE, copy-calibrated)"}

@@ -102,7 +102,8 @@ typedef struct ufc_rt_asm_info {
   int32_t n_nodes;
   int32_t n_elems;
   int32_t max_row_nnz;
-  int32_t reserved;
+  int32_t nb_index_bytes; /* neighbour index width of the FP32 SpMV view: 2  */
+                          /* when every |node - neighbour| < 2^15, else 4    */
 } ufc_rt_asm_info;
 int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
 

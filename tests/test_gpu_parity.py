@@ -313,6 +313,41 @@ def test_pattern_rejects_invalid_meshes(ctx):
     with pytest.raises(A.UfcError) as e:
         ctx.build_pattern(coords, many)
     assert e.value.code == A.STATUS_UNSUPPORTED
+    iso = np.concatenate([coords, [[9.0, 9.0, 9.0]]])   # a node in no element
+    with pytest.raises(A.UfcError) as e:
+        ctx.build_pattern(iso, conn)
+    assert e.value.code == A.STATUS_INVALID
     # the context stays usable after the rejections
     info = ctx.build_pattern(coords, conn)
     assert info.nnz > 0
+
+
+def test_fp32_view_wide_numbering_vs_oracle(ctx, orc):
+    """A 32^3 cube with randomly permuted node ids (bandwidth > 2^15): the FP32
+    view falls back to 4-byte neighbour ids; SpMV and the mixed solve match
+    the oracle as with 16-bit offsets."""
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(32, name="cube32perm")
+    perm = np.random.default_rng(11).permutation(pr.coords.shape[0])
+    coords = np.empty_like(pr.coords)
+    coords[perm] = pr.coords
+    pr.conn = perm[pr.conn].astype(np.int32)
+    pr.coords = coords
+    pr.bc_eq = (3 * perm[pr.bc_eq // 3] + pr.bc_eq % 3).astype(np.int32)
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 2)
+    assert ctx.nb_index_bytes == 4
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    x = np.random.default_rng(2).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    y32 = ctx.spmv(x, precision=1)
+    assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    say("wide numbering mixed solve")
+    u, info = ctx.pcg(precision=2, rtol=1e-10)
+    assert info.converged
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+    # the structured numbering of the same mesh takes the 16-bit path
+    P.setup_context(ctx, P.cube_pressure(8, name="cube8"), 2)
+    assert ctx.nb_index_bytes == 2

@@ -233,7 +233,7 @@ int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
     info->n_nodes = c->n_nodes;
     info->n_elems = c->n_elems;
     info->max_row_nnz = c->max_row_nnz;
-    info->reserved = 0;
+    info->nb_index_bytes = c->have_off16 ? 2 : 4;
     return UFC_STATUS_OK;
   });
 }
@@ -432,9 +432,14 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     // node -> (element, local index), elements ascending
     std::vector<int32_t> cnt(static_cast<size_t>(nn), 0);
     for (int32_t v : c->h_conn) cnt[size_t(v)]++;
-    for (int32_t n = 0; n < nn; ++n)
+    for (int32_t n = 0; n < nn; ++n) {
       UFC_REQUIRE(cnt[size_t(n)] <= ufc::kMaxNodeElems, UFC_STATUS_UNSUPPORTED,
                   "node " + std::to_string(n + base) + " belongs to more than 8 elements");
+      // a node in no element has an empty CSR row and no diagonal: its
+      // equations could be neither solved nor constrained
+      UFC_REQUIRE(cnt[size_t(n)] > 0, UFC_STATUS_INVALID,
+                  "node " + std::to_string(n + base) + " belongs to no element");
+    }
     std::vector<int32_t> s_elem(static_cast<size_t>(nn) * 8, -1);
     std::vector<uint8_t> s_loc(static_cast<size_t>(nn) * 8, 0);
     std::fill(cnt.begin(), cnt.end(), 0);
@@ -508,6 +513,12 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       }
     });
     row_ptr[size_t(nn) * 3] = int32_t(nnz);
+    // 16-bit neighbour offsets when the numbering's bandwidth allows
+    int64_t maxoff = 0;
+    for (int64_t n = 0; n < nn; ++n)
+      for (int32_t q = c->h_nb_ptr[size_t(n)]; q < c->h_nb_ptr[size_t(n) + 1]; ++q)
+        maxoff = std::max<int64_t>(maxoff, std::abs(int64_t(nb_idx[size_t(q)]) - n));
+    c->have_off16 = maxoff <= 32767;
 
     c->n_nodes = nn;
     c->n_elems = ne;
@@ -534,6 +545,18 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     up(c->conn.get(), c->h_conn.data(), c->conn.bytes());
     up(c->nb_ptr.get(), c->h_nb_ptr.data(), c->nb_ptr.bytes());
     up(c->nb_idx.get(), nb_idx.data(), c->nb_idx.bytes());
+    std::vector<int16_t> off16;
+    if (c->have_off16) {
+      off16.resize(size_t(npairs));
+      parallel_for(nn, [&](int64_t n) {
+        for (int32_t q = c->h_nb_ptr[size_t(n)]; q < c->h_nb_ptr[size_t(n) + 1]; ++q)
+          off16[size_t(q)] = int16_t(int64_t(nb_idx[size_t(q)]) - n);
+      });
+      c->nb_off16.resize(size_t(npairs));
+      up(c->nb_off16.get(), off16.data(), c->nb_off16.bytes());
+    } else {
+      c->nb_off16.release();
+    }
     up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
     up(c->diag_pos.get(), diag_pos.data(), c->diag_pos.bytes());
     up(c->slot_elem.get(), s_elem.data(), c->slot_elem.bytes());
@@ -547,7 +570,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       info->n_nodes = nn;
       info->n_elems = ne;
       info->max_row_nnz = c->max_row_nnz;
-      info->reserved = 0;
+      info->nb_index_bytes = c->have_off16 ? 2 : 4;
     }
     msg = "pattern built: nnz " + std::to_string(nnz);
     return UFC_STATUS_OK;

@@ -131,9 +131,10 @@ __device__ __forceinline__ void sum_partials(double (&out)[NV], const double* pa
 
 // ---------------------------------------------------------------- SpMV body
 // Row sums of node n in a[0..2] (valid in every lane of the 32-lane group).
-template <typename T, bool BLOCK_INDEX>
+template <typename T, bool BLOCK_INDEX, bool OFF16>
 __device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __restrict__ nb_ptr,
                                           const int32_t* __restrict__ nb_idx,
+                                          const int16_t* __restrict__ nb_off,
                                           const int32_t* __restrict__ col_idx,
                                           const T* __restrict__ val, const T* __restrict__ x,
                                           T (&a)[3]) {
@@ -152,7 +153,7 @@ __device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __res
       // Difference form: the self block holds the penalty diagonal and
       // multiplies x_n; every other block multiplies x_m - x_n (s = 0 leaves
       // x_m exact, branch-free).
-      const int64_t m = nb_idx[q0 + k];
+      const int64_t m = OFF16 ? n + int64_t(nb_off[q0 + k]) : int64_t(nb_idx[q0 + k]);
       T v[9];
 #pragma unroll
       for (int d = 0; d < 3; ++d)
@@ -178,19 +179,22 @@ __device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __res
   for (int d = 0; d < 3; ++d) a[d] = sum_group32(a[d]);
 }
 
-// y = K x; with sc != null an early return once the solve is done
-template <typename T, bool BLOCK_INDEX>
+// y = K x; with sc != null an early return once the solve is done.  OFF16:
+// the FP32 view's neighbours come as 16-bit offsets m - n (2 bytes instead of
+// 4 per neighbour, measured 0.456 -> 0.440 ms at 128^3)
+template <typename T, bool BLOCK_INDEX, bool OFF16>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
     int32_t n_nodes, const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
-    const int32_t* __restrict__ col_idx, const T* __restrict__ val, const T* __restrict__ x,
-    T* __restrict__ y, const PcgScalars* __restrict__ sc) {
+    const int16_t* __restrict__ nb_off, const int32_t* __restrict__ col_idx,
+    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+    const PcgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
   const int k = threadIdx.x & (GROUP - 1);
   const int64_t g0 = int64_t(blockIdx.x) * GROUPS_PER_BLOCK + threadIdx.x / GROUP;
   const int64_t ng = int64_t(gridDim.x) * GROUPS_PER_BLOCK;
   for (int64_t n = g0; n < n_nodes; n += ng) {
     T a[3];
-    node_rows<T, BLOCK_INDEX>(n, k, nb_ptr, nb_idx, col_idx, val, x, a);
+    node_rows<T, BLOCK_INDEX, OFF16>(n, k, nb_ptr, nb_idx, nb_off, col_idx, val, x, a);
     if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
   }
 }
@@ -493,9 +497,18 @@ int spmv_blocks(const ufc_ctx* c) {
 
 template <typename T, bool BLOCK_INDEX>
 void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg) {
-  hipLaunchKernelGGL((k_spmv<T, BLOCK_INDEX>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads), 0,
-                     c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(), c->col_idx.get(),
-                     val, x, y, in_pcg ? c->scalars.get() : nullptr);
+  const PcgScalars* sc = in_pcg ? c->scalars.get() : nullptr;
+  if constexpr (BLOCK_INDEX) {
+    if (c->have_off16) {
+      hipLaunchKernelGGL((k_spmv<T, true, true>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads),
+                         0, c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(),
+                         c->nb_off16.get(), c->col_idx.get(), val, x, y, sc);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_spmv<T, BLOCK_INDEX, false>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads),
+                     0, c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(),
+                     c->nb_off16.get(), c->col_idx.get(), val, x, y, sc);
 }
 
 // resume: continue a CG after a residual replacement -- r := b (the new true

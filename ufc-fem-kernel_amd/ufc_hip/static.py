@@ -160,6 +160,7 @@ class Context:
         self._call("ufc_rt_asm_build_pattern", C.byref(m), C.byref(info))
         self.n_dof, self.nnz = info.n_dof, info.nnz
         self.n_nodes, self.n_elems = info.n_nodes, info.n_elems
+        self.nb_index_bytes = info.nb_index_bytes
         return info
 
     def get_pattern(self, index_base=0):
