@@ -342,6 +342,51 @@ int ufc_mesh_box(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, doubl
 int ufc_field(int32_t kind, int64_t n, uint64_t seed, double a, double b, double* out,
               ufc_status* st);
 
+/* ====================================================================== */
+/* L6_AP: input deck                                                       */
+/* ====================================================================== */
+
+/* A static C3D8 step read from an Abaqus-style input deck: replaces the
+ * keyword reading of AP_Input_Domain_ParseKeyword (L6_AP/Input/Parser/
+ * AP_Inp_Mgr.f90:99-125, keyword dispatch AP_Inp_KWBrg.f90:249) for the
+ * keywords of this path.  Read: *NODE, *ELEMENT (TYPE=C3D8), *NSET, *ELSET
+ * (GENERATE), *SURFACE (TYPE=ELEMENT, S1..S6), *MATERIAL + *ELASTIC,
+ * *SOLID SECTION, *BOUNDARY (dof ranges, ENCASTRE, PINNED, X/Y/ZSYMM),
+ * *STEP + *STATIC, *CLOAD, *DLOAD (P1..P6), *DSLOAD, one *PART with one
+ * *INSTANCE (labels "INSTANCE.label"); output requests and other keywords are
+ * skipped.  Other element types, more than one part or step, and several
+ * Poisson ratios return UFC_STATUS_UNSUPPORTED.  Pressures keep the
+ * reference's sign (positive along the outward normal, the opposite of
+ * Abaqus).  All arrays belong to the model until ufc_ap_free; indices are
+ * 0-based.  Face label Sk (Pk) is column k of PH_ELEM_C3D8_FACE_NODES
+ * (PH_Elem_C3D8.f90:120-121), as the reference maps *SURFACE labels
+ * (MD_KW_Mapper.f90:2137-2155), returned as k - 1.  Errors carry the deck
+ * line in st->line_number. */
+typedef struct ufc_ap_model {
+  int32_t n_nodes;
+  int32_t n_elems;
+  const int32_t* node_label; /* (n_nodes) labels of the deck                 */
+  const int32_t* elem_label; /* (n_elems)                                    */
+  const double* coords;      /* (n_nodes, 3)                                 */
+  const int32_t* conn;       /* (n_elems, 8) node indices                    */
+  const double* young;       /* (n_elems) E of each element's section        */
+  double nu;
+  int32_t n_bc;              /* prescribed equations (last value wins)       */
+  int32_t n_cload;
+  const int32_t* bc_eq;      /* (n_bc) eq = 3 node + dof                     */
+  const double* bc_val;      /* (n_bc)                                       */
+  const int32_t* cload_eq;   /* (n_cload)                                    */
+  const double* cload_val;   /* (n_cload)                                    */
+  int32_t n_face;            /* pressure-loaded element faces                */
+  int32_t reserved;
+  const int32_t* face_elem;  /* (n_face)                                     */
+  const int32_t* face_id;    /* (n_face) 0..5                                */
+  const double* face_p;      /* (n_face) pressure                            */
+  void* impl;                /* storage owned by the library                 */
+} ufc_ap_model;
+int ufc_ap_read_inp(const char* path, ufc_ap_model* model, ufc_status* st);
+int ufc_ap_free(ufc_ap_model* model, ufc_status* st);
+
 #ifdef __cplusplus
 }
 #endif

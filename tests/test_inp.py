@@ -1,0 +1,183 @@
+"""Input-deck reader (ufc_ap_read_inp through the C-ABI), CPU only.
+
+Golden decks: tests/golden/decks/sample_*.inp are the reference's own sample
+decks (tests/samples/, read by tests/test_integration_framework.f90:38-40):
+sample 1 is one C3D8 in a part with one material; samples 2 and 3 hold S4
+shells and B21 beams, which this path does not model.  The block deck and its
+expected arrays come from tests/deck_cases.py.  Comparisons are exact.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import deck_cases
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECKS = os.path.join(HERE, "golden", "decks")
+FORTRAN = os.path.join(os.path.dirname(HERE), "ufc-fem-kernel_amd", "fortran")
+
+
+def _read(path):
+    from ufc_hip import inp
+    return inp.read_inp(path)
+
+
+def _assert_matches(d, want):
+    for key, v in want.items():
+        got = getattr(d, key)
+        if key == "nu":
+            assert got == v
+        else:
+            assert got.shape == np.asarray(v).shape, key
+            assert np.array_equal(got, v), key
+
+
+def test_reference_sample_1_single_c3d8():
+    d = _read(os.path.join(DECKS, "sample_1_simple_part.inp"))
+    unit = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                     [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    assert np.array_equal(d.coords, unit)
+    assert np.array_equal(d.node_label, np.arange(1, 9))
+    assert np.array_equal(d.conn, np.arange(8)[None])
+    assert np.array_equal(d.elem_label, [1])
+    assert np.array_equal(d.young, [210000.0]) and d.nu == 0.3
+    assert d.bc_eq.size == 0 and d.cload_eq.size == 0 and d.face_elem.size == 0
+
+
+@pytest.mark.parametrize("name,what", [("sample_2_assembly_contact.inp", "'S4'"),
+                                       ("sample_3_step_static.inp", "'B21'")])
+def test_reference_samples_other_elements_unsupported(name, what):
+    from ufc_hip import UfcError
+    from ufc_hip._abi import STATUS_UNSUPPORTED
+    with pytest.raises(UfcError) as e:
+        _read(os.path.join(DECKS, name))
+    assert e.value.code == STATUS_UNSUPPORTED
+    assert "line 9" in e.value.message and what in e.value.message
+
+
+def test_block_deck_every_keyword_form(tmp_path):
+    path = str(tmp_path / "block.inp")
+    want = deck_cases.block_deck(path)
+    _assert_matches(_read(path), want)
+
+
+def test_op_new_and_symmetry_conditions(tmp_path):
+    path = str(tmp_path / "opnew.inp")
+    want = deck_cases.block_deck(path, extra_step="\r\n".join(
+        ["*Boundary, op=NEW", "Blk-1.Bottom, ZSYMM", "Blk-1.Tip, PINNED",
+         "*Cload, OP=NEW", "Blk-1.101, 1, 7.5", "Blk-1.101, 1, 2.5"]))
+    d = _read(path)
+    idx = {lab: q for q, lab in enumerate(want["node_label"])}
+    bottom = [deck_cases.node_label(i, j, 0) for j in range(deck_cases.SY)
+              for i in range(deck_cases.SX)]
+    tip = [deck_cases.node_label(deck_cases.NX, j, deck_cases.NZ) for j in range(deck_cases.SY)]
+    eqs = sorted([3 * idx[b] + 2 for b in bottom] + [3 * idx[t] + d for t in tip for d in range(3)])
+    assert np.array_equal(d.bc_eq, eqs) and np.all(d.bc_val == 0.0)
+    assert np.array_equal(d.cload_eq, [3 * idx[101]] * 2)
+    assert np.array_equal(d.cload_val, [7.5, 2.5])
+    assert np.array_equal(d.face_p, want["face_p"])  # loads of other kinds stay
+
+
+def test_boundary_dofs_beyond_three_are_ignored(tmp_path):
+    """dofs 4-6 (rotations) do not exist on C3D8 nodes: 'Tip, 2, 6' fixes 2, 3."""
+    path = str(tmp_path / "rot.inp")
+    want = deck_cases.block_deck(path, replace={"Blk-1.Tip, 1, 1, 0.001": "Blk-1.Tip, 2, 6"})
+    d = _read(path)
+    idx = {lab: q for q, lab in enumerate(want["node_label"])}
+    tip = [deck_cases.node_label(deck_cases.NX, j, deck_cases.NZ) for j in range(deck_cases.SY)]
+    bc = {e: v for e, v in zip(want["bc_eq"].tolist(), want["bc_val"].tolist())}
+    for t in tip:
+        del bc[3 * idx[t]]
+        bc[3 * idx[t] + 1] = bc[3 * idx[t] + 2] = 0.0
+    assert np.array_equal(d.bc_eq, sorted(bc))
+    assert np.array_equal(d.bc_val, [bc[e] for e in sorted(bc)])
+
+
+BAD = [
+    ("two steps", {"*End Step": "*End Step\r\n*Step\r\n*Static\r\n*End Step"}, 11, "*STEP"),
+    ("two Poisson ratios", {" 3.0e9, 0.3": " 3.0e9, 0.25"}, 11, "Poisson"),
+    ("nlgeom", {"nlgeom=NO": "nlgeom=YES"}, 11, "NLGEOM"),
+    ("other element", {"type=C3D8": "type=C3D20R"}, 11, "C3D20R"),
+    ("face label", {"Top, S2": "Top, S7"}, 11, "S7"),
+    ("traction", {"Blk-1.TopS, P, 1.0e5": "Blk-1.TopS, TRVEC, 1.0, 0, 0, 1"}, 11, "TRVEC"),
+    ("moment", {"Blk-1.Tip, 3, -500.": "Blk-1.Tip, 5, -500."}, 11, "moment"),
+    ("instance offset", {"*End Instance": "1.0, 0.0, 0.0\r\n*End Instance"}, 11, "translation"),
+    ("unknown set", {"Blk-1.Tip, 3": "Blk-1.Tipp, 3"}, 1, "TIPP"),
+    ("unknown instance", {"Blk-1.Bottom": "Blk-2.Bottom"}, 1, "Blk-2"),
+    ("undefined node", {"*Element, type=C3D8, elset=Everything\r\n1001, 101,":
+                        "*Element, type=C3D8, elset=Everything\r\n1001, 999,"}, 1, "999"),
+    ("duplicate node", {"*NODE, NSET=second\r\n": "*NODE, NSET=second\r\n  101, 0, 0, 0\r\n"},
+     1, "twice"),
+    ("no section", {"*Solid Section, elset=Soft, material=Rubberish\r\n,\r\n": ""}, 1,
+     "no section"),
+    ("bad number", {" 1.0e9, 0.3": " 1.0e9x, 0.3"}, 1, "1.0e9x"),
+    ("short element", {"1002, ": "1002, 1, "}, 1, "8 nodes"),
+]
+
+
+@pytest.mark.parametrize("case,repl,code,what", BAD, ids=[b[0] for b in BAD])
+def test_rejected_decks_name_the_problem(tmp_path, case, repl, code, what):
+    from ufc_hip import UfcError
+    path = str(tmp_path / "bad.inp")
+    deck_cases.block_deck(path, replace=repl)
+    with pytest.raises(UfcError) as e:
+        _read(path)
+    assert e.value.code == code, str(e.value)
+    assert what.upper() in e.value.message.upper(), str(e.value)
+
+
+def test_missing_file_is_invalid(tmp_path):
+    from ufc_hip import UfcError
+    with pytest.raises(UfcError) as e:
+        _read(str(tmp_path / "nope.inp"))
+    assert e.value.code == 1 and "cannot open" in e.value.message
+
+
+@pytest.mark.parametrize("kind", ["cube", "irregular_young"])
+def test_write_read_round_trip(tmp_path, kind):
+    from ufc_hip import inp, problems as P
+    pr = P.cube_pressure(4, jitter=0.1, p=3.5e5)
+    if kind == "irregular_young":
+        pr.young = np.random.default_rng(1).choice([1e9, 2e9, 5e8], pr.n_elems)
+        pr.bc_val = np.linspace(0.0, 1e-3, pr.bc_eq.size)
+    path = str(tmp_path / "rt.inp")
+    inp.write_inp(path, pr)
+    d = inp.read_inp(path)
+    assert np.array_equal(d.coords, pr.coords) and np.array_equal(d.conn, pr.conn)
+    assert np.array_equal(d.young, pr.young) and d.nu == pr.nu
+    order = np.argsort(pr.bc_eq)
+    assert np.array_equal(d.bc_eq, pr.bc_eq[order]) and np.array_equal(d.bc_val, pr.bc_val[order])
+    assert np.array_equal(d.face_elem, pr.load_elem) and np.array_equal(d.face_id, pr.load_face)
+    assert np.all(d.face_p == 3.5e5)
+
+
+def test_write_rejects_traction(tmp_path):
+    from ufc_hip import inp, problems as P
+    with pytest.raises(ValueError):
+        inp.write_inp(str(tmp_path / "c.inp"), P.cantilever())
+
+
+def test_fortran_deck_driver_reports_reader_errors():
+    exe = os.path.join(FORTRAN, "ufc_inp_step")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drivers not built (no Fortran compiler)")
+    r = subprocess.run([exe, os.path.join(DECKS, "sample_3_step_static.inp"), "fp64", os.devnull],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "ufc_ap_read_inp failed, status 11: line 9" in r.stdout
+
+
+def test_fortran_deck_driver_reads_then_needs_a_device(tmp_path):
+    import torch
+    exe = os.path.join(FORTRAN, "ufc_inp_step")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drivers not built (no Fortran compiler)")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    path = str(tmp_path / "block.inp")
+    deck_cases.block_deck(path)
+    r = subprocess.run([exe, path, "fp64", str(tmp_path / "u.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "ufc_ctx_create failed, status" in r.stdout

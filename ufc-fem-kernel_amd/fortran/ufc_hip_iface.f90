@@ -138,13 +138,39 @@ MODULE ufc_hip_iface
     TYPE(C_PTR) :: grad = C_NULL_PTR       ! (n_elems) out
   END TYPE ufc_ph_sens_args
 
+  ! ufc_ap_model: a static C3D8 step read from an input deck (0-based arrays
+  ! owned by the library until ufc_ap_free)
+  TYPE, BIND(C), PUBLIC :: ufc_ap_model
+    INTEGER(C_INT32_T) :: n_nodes = 0
+    INTEGER(C_INT32_T) :: n_elems = 0
+    TYPE(C_PTR) :: node_label = C_NULL_PTR ! (n_nodes)
+    TYPE(C_PTR) :: elem_label = C_NULL_PTR ! (n_elems)
+    TYPE(C_PTR) :: coords = C_NULL_PTR     ! (3, n_nodes)
+    TYPE(C_PTR) :: conn = C_NULL_PTR       ! (8, n_elems), 0-based nodes
+    TYPE(C_PTR) :: young = C_NULL_PTR      ! (n_elems)
+    REAL(C_DOUBLE) :: nu = 0.0_C_DOUBLE
+    INTEGER(C_INT32_T) :: n_bc = 0
+    INTEGER(C_INT32_T) :: n_cload = 0
+    TYPE(C_PTR) :: bc_eq = C_NULL_PTR      ! (n_bc), eq = 3 node + dof, 0-based
+    TYPE(C_PTR) :: bc_val = C_NULL_PTR
+    TYPE(C_PTR) :: cload_eq = C_NULL_PTR   ! (n_cload)
+    TYPE(C_PTR) :: cload_val = C_NULL_PTR
+    INTEGER(C_INT32_T) :: n_face = 0
+    INTEGER(C_INT32_T) :: reserved = 0
+    TYPE(C_PTR) :: face_elem = C_NULL_PTR  ! (n_face), 0-based
+    TYPE(C_PTR) :: face_id = C_NULL_PTR    ! (n_face), 0..5
+    TYPE(C_PTR) :: face_p = C_NULL_PTR     ! (n_face)
+    TYPE(C_PTR) :: impl = C_NULL_PTR
+  END TYPE ufc_ap_model
+
   PUBLIC :: ufc_ctx_create, ufc_ctx_destroy, ufc_ctx_info, ufc_ctx_timing, &
             ufc_ctx_enable_timing, ufc_ph_c3d8_eval_ke, ufc_ph_c3d8_sensitivity, &
             ufc_rt_asm_build_pattern, ufc_rt_asm_get_pattern, ufc_rt_asm_assemble_k, &
             ufc_rt_asm_get_values, ufc_rt_asm_apply_bc, ufc_rt_asm_load, ufc_rt_set_rhs, &
             ufc_rt_get_rhs, ufc_rt_residual, ufc_nm_spmv, ufc_nm_pcg, ufc_nm_adjoint_solve, &
             ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_ph_j2_get_state, &
-            ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_status_message
+            ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_ap_read_inp, ufc_ap_free, &
+            ufc_status_message
 
   INTERFACE
     INTEGER(C_INT) FUNCTION ufc_ctx_create(device, ctx, st) BIND(C, NAME='ufc_ctx_create')
@@ -343,6 +369,20 @@ MODULE ufc_hip_iface
       INTEGER(C_INT64_T), VALUE :: n, seed
       REAL(C_DOUBLE), VALUE :: a, b
       TYPE(C_PTR), VALUE :: out
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    ! path: NUL-terminated (TRIM(name) // C_NULL_CHAR)
+    INTEGER(C_INT) FUNCTION ufc_ap_read_inp(path, model, st) BIND(C, NAME='ufc_ap_read_inp')
+      IMPORT :: C_INT, C_CHAR, ufc_ap_model, ufc_status
+      CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: path
+      TYPE(ufc_ap_model), INTENT(INOUT) :: model
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_ap_free(model, st) BIND(C, NAME='ufc_ap_free')
+      IMPORT :: C_INT, ufc_ap_model, ufc_status
+      TYPE(ufc_ap_model), INTENT(INOUT) :: model
       TYPE(ufc_status), INTENT(INOUT) :: st
     END FUNCTION
   END INTERFACE

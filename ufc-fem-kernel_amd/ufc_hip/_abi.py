@@ -106,6 +106,18 @@ class Timing(C.Structure):
                 ("n_replace", C.c_int32)]
 
 
+class ApModel(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("n_elems", C.c_int32),
+                ("node_label", C.c_void_p), ("elem_label", C.c_void_p),
+                ("coords", C.c_void_p), ("conn", C.c_void_p), ("young", C.c_void_p),
+                ("nu", C.c_double), ("n_bc", C.c_int32), ("n_cload", C.c_int32),
+                ("bc_eq", C.c_void_p), ("bc_val", C.c_void_p),
+                ("cload_eq", C.c_void_p), ("cload_val", C.c_void_p),
+                ("n_face", C.c_int32), ("reserved", C.c_int32),
+                ("face_elem", C.c_void_p), ("face_id", C.c_void_p), ("face_p", C.c_void_p),
+                ("impl", C.c_void_p)]
+
+
 P = C.POINTER
 _SIGS = {
     "ufc_ctx_create": [C.c_int32, P(C.c_void_p), P(Status)],
@@ -136,6 +148,8 @@ _SIGS = {
                      C.c_double, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_field": [C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_void_p,
                   P(Status)],
+    "ufc_ap_read_inp": [C.c_char_p, P(ApModel), P(Status)],
+    "ufc_ap_free": [P(ApModel), P(Status)],
 }
 
 _lib = None
