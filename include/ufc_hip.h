@@ -343,6 +343,56 @@ int ufc_field(int32_t kind, int64_t n, uint64_t seed, double a, double b, double
               ufc_status* st);
 
 /* ====================================================================== */
+/* L5_RT: restart                                                          */
+/* ====================================================================== */
+
+/* The step data of MD_RestartData as RT_Out_RestartSave writes it
+ * (L5_RT/Output/RT_Out_Restart.f90:189-306; saved per converged increment by
+ * RT_Step_Exec.f90:551-558). */
+typedef struct ufc_rt_restart_data {
+  double time;           /* step time at the end of the increment          */
+  double lambda;         /* load factor                                    */
+  double residual_norm;  /* last Newton residual norm                      */
+  int32_t increment;     /* converged increments so far                    */
+  int32_t iterations;    /* Newton iterations of the last increment        */
+  int32_t converged;     /* 1 / 0                                          */
+  int32_t n_dof;         /* see below; set by a load                       */
+  double* u;             /* (n_dof) host or device; v = a = 0 (static)     */
+} ufc_rt_restart_data;
+/* With a context, u holds the context's n_dof values (host or device) and
+ * data->n_dof is ignored on entry.  ctx may be NULL when with_state is 0:
+ * u is then a host array of data->n_dof doubles (save) or with room for
+ * data->n_dof, which must equal the file's (load). */
+
+/* Writes a restart file: the record sequence of RT_Out_RestartSave as
+ * Fortran unformatted sequential records (4-byte length markers, the files
+ * IF_IO_FORMAT_BINARY opens, L1_IF/IO/IF_IO_File.f90:219-221):
+ *   [version=1, 20260204, n_dof] (3 x int32), u, v, a (n_dof doubles each),
+ *   [time, increment, lambda] (f64, i32, f64), [residual_norm, iterations,
+ *   converged] (f64, i32, i32), [n_dof] (the reference's checksum),
+ * so RT_Out_RestartRestore reads it; v and a are written as zeros (the
+ * reference writes them only when allocated but always reads them back).
+ * with_state = 1 appends the committed J2 state of ctx after those records:
+ *   [n_elems, 8, 6, 7, n_chunks] (int32), then per chunk of at most 2^21
+ *   elements one record of 13 doubles per Gauss point (stress(6), then the
+ *   UMAT statev: eq. plastic strain, plastic strain(6),
+ *   PH_Mat_Plast_J2_UMAT_Core.f90:521-531), elements in order.
+ * The reference's reader stops before these records.  Errors as
+ * RT_Out_RestartSave: empty path or no u -> UFC_STATUS_INVALID, the file
+ * cannot be written -> UFC_STATUS_ERROR. */
+int ufc_rt_restart_save(ufc_ctx* ctx, const char* path, const ufc_rt_restart_data* data,
+                        int32_t with_state, ufc_status* st);
+/* Reads a restart file written as above (RT_Out_RestartRestore,
+ * RT_Out_Restart.f90:45-187): fills data (u must hold ctx's n_dof) and, with
+ * with_state = 1, loads the J2 state into ctx (committed and trial copies;
+ * ufc_ph_j2_init must have been called on the same mesh).  Missing file,
+ * version != 1, n_dof <= 0 or != ctx's -> UFC_STATUS_INVALID; checksum
+ * mismatch, truncated records or a missing state block ->
+ * UFC_STATUS_ERROR. */
+int ufc_rt_restart_load(ufc_ctx* ctx, const char* path, ufc_rt_restart_data* data,
+                        int32_t with_state, ufc_status* st);
+
+/* ====================================================================== */
 /* L6_AP: input deck                                                       */
 /* ====================================================================== */
 

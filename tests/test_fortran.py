@@ -89,3 +89,53 @@ def test_j2_driver_matches_oracle(tmp_path, orc, prec):
     err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
     print(f"[gpu-test] fortran j2 u err {err:.2e}", flush=True)
     assert err < 1e-7
+
+
+@pytest.mark.gpu
+def test_j2_driver_checkpoint_and_resume(tmp_path, orc):
+    """The Fortran J2 driver checkpoints after every increment
+    (ufc_rt_restart_save with the J2 state); a job stopped after 5 of 10
+    increments and resumed from its checkpoint ends bit-identical to the job
+    run straight through, and the final checkpoint's stress and eq. plastic
+    strain match the oracle's."""
+    import struct
+    from ufc_hip import problems as P, restart as R
+    ck = str(tmp_path / "j2.rst")
+    full, part = tmp_path / "full.bin", tmp_path / "part.bin"
+    runs = [([str(full)], "full"), ([str(part), ck, "new", "5"], "first half"),
+            ([str(part), ck, "resume"], "resumed")]
+    outs = []
+    for extra, what in runs:
+        print(f"[gpu-test] fortran j2 driver 4^3 fp64 restart: {what}", flush=True)
+        r = subprocess.run([_exe(EXE_J2), "4", "fp64", "10"] + extra, capture_output=True,
+                           text=True, timeout=120)
+        print(r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append(r.stdout)
+    assert outs[1].count("increment ") == 5 and "resumed after increment 5" in outs[2]
+    u_full = np.fromfile(full, dtype=np.float64)
+    assert np.array_equal(np.fromfile(part, dtype=np.float64), u_full)
+    pr = P.j2_cube(4)
+    u_ck, s_ck = R.load(ck, n_dof=pr.n_dof)
+    assert np.array_equal(u_ck, u_full) and s_ck.increment == 10 and s_ck.converged
+    # state block after the reference's records: (n_elems, 8, 6, 7, chunks), then the state
+    b = open(ck, "rb").read()
+    p, recs = 0, []
+    while p < len(b):
+        (n,) = struct.unpack_from("<i", b, p)
+        recs.append(b[p + 4:p + 4 + n])
+        p += 8 + n
+    assert struct.unpack("<5i", recs[7]) == (pr.n_elems, 8, 6, 7, 1)
+    sg = np.frombuffer(recs[8], "<f8").reshape(-1, 13)
+    target = np.zeros(pr.bc_eq.size)
+    target[pr.extra["n_clamp"]:] = pr.extra["strain"]
+    state = np.zeros(pr.n_elems * 8 * 13)
+    uo = np.zeros(pr.n_dof)
+    for k in range(1, 11):
+        st, uo, state, _, _ = orc.j2_newton_step(pr.coords.shape[0], pr.conn, pr.coords,
+                                                 pr.extra["props"], pr.bc_eq, target * (k / 10.0),
+                                                 1e12, state, uo)
+        assert st == 0
+    so = state.reshape(-1, 13)
+    assert np.abs(sg[:, :6] - so[:, :6]).max() < 1e-6 * np.abs(so[:, :6]).max()
+    assert np.abs(sg[:, 6] - so[:, 6]).max() < 1e-6 * so[:, 6].max()

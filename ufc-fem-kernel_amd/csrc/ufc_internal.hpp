@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -84,6 +85,36 @@ constexpr int kSpmvBlocks32 = 16384;  // FP32 SpMV grid (measured best on gfx950
 constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vector kernels)
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
+
+// ---------------------------------------------------------------- C-ABI status
+inline void set_status(ufc_status* st, int code, const std::string& msg, const char* src) {
+  if (!st) return;
+  st->status_code = code;
+  st->severity = code == UFC_STATUS_OK ? 0 : (code == UFC_STATUS_WARN ? 1 : 3);
+  st->category = code;
+  st->line_number = 0;
+  std::snprintf(st->message, sizeof st->message, "%s", msg.c_str());
+  std::snprintf(st->source, sizeof st->source, "%s", src);
+}
+
+template <typename F>
+int guarded(ufc_status* st, const char* src, F&& body) {
+  try {
+    std::string msg;
+    int code = body(msg);
+    set_status(st, code, msg, src);
+    return code;
+  } catch (const Error& e) {
+    set_status(st, e.code, e.msg, src);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_status(st, UFC_STATUS_MEM_ERROR, "host allocation failed", src);
+    return UFC_STATUS_MEM_ERROR;
+  } catch (...) {
+    set_status(st, UFC_STATUS_FATAL, "unexpected exception", src);
+    return UFC_STATUS_FATAL;
+  }
+}
 
 }  // namespace ufc
 

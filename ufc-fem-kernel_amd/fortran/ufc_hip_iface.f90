@@ -138,6 +138,18 @@ MODULE ufc_hip_iface
     TYPE(C_PTR) :: grad = C_NULL_PTR       ! (n_elems) out
   END TYPE ufc_ph_sens_args
 
+  ! ufc_rt_restart_data: the step data of MD_RestartData (RT_Out_Restart.f90)
+  TYPE, BIND(C), PUBLIC :: ufc_rt_restart_data
+    REAL(C_DOUBLE) :: time = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: lambda = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: residual_norm = 0.0_C_DOUBLE
+    INTEGER(C_INT32_T) :: increment = 0
+    INTEGER(C_INT32_T) :: iterations = 0
+    INTEGER(C_INT32_T) :: converged = 1
+    INTEGER(C_INT32_T) :: n_dof = 0
+    TYPE(C_PTR) :: u = C_NULL_PTR          ! (n_dof)
+  END TYPE ufc_rt_restart_data
+
   ! ufc_ap_model: a static C3D8 step read from an input deck (0-based arrays
   ! owned by the library until ufc_ap_free)
   TYPE, BIND(C), PUBLIC :: ufc_ap_model
@@ -170,6 +182,7 @@ MODULE ufc_hip_iface
             ufc_rt_get_rhs, ufc_rt_residual, ufc_nm_spmv, ufc_nm_pcg, ufc_nm_adjoint_solve, &
             ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_ph_j2_get_state, &
             ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_ap_read_inp, ufc_ap_free, &
+            ufc_rt_restart_save, ufc_rt_restart_load, &
             ufc_status_message
 
   INTERFACE
@@ -373,6 +386,26 @@ MODULE ufc_hip_iface
     END FUNCTION
 
     ! path: NUL-terminated (TRIM(name) // C_NULL_CHAR)
+    INTEGER(C_INT) FUNCTION ufc_rt_restart_save(ctx, path, data, with_state, st) &
+        BIND(C, NAME='ufc_rt_restart_save')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, C_CHAR, ufc_rt_restart_data, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: path
+      TYPE(ufc_rt_restart_data), INTENT(IN) :: data
+      INTEGER(C_INT32_T), VALUE :: with_state
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_restart_load(ctx, path, data, with_state, st) &
+        BIND(C, NAME='ufc_rt_restart_load')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, C_CHAR, ufc_rt_restart_data, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: path
+      TYPE(ufc_rt_restart_data), INTENT(INOUT) :: data
+      INTEGER(C_INT32_T), VALUE :: with_state
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
     INTEGER(C_INT) FUNCTION ufc_ap_read_inp(path, model, st) BIND(C, NAME='ufc_ap_read_inp')
       IMPORT :: C_INT, C_CHAR, ufc_ap_model, ufc_status
       CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: path

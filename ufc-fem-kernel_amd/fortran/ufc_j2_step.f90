@@ -20,11 +20,16 @@
 ! The reference converges on absolute norms (RT_Step_Exec.f90:380-382); the
 ! test here is relative to the internal force, which is scale-free.
 !
-! Usage: ufc_j2_step <n> <precision> <increments> <u_out>
+! Usage: ufc_j2_step <n> <precision> <increments> <u_out> [restart [mode [stop]]]
 !   unit cube of n^3 C3D8, interior nodes jittered by U[-0.1, 0.1] h (seeded),
 !   E = 200e9, nu = 0.3, sigma_y0 = 250e6, H = 2e9; z = 0 clamped, u_z on
 !   z = 1 raised to 0.005 (0.5% strain) in <increments> equal increments.
 !   precision fp64 | mixed (the linear solve); u_out: n_dof doubles, stream.
+!   restart: checkpoint file written after every converged increment with u,
+!   the step data and the J2 state (ufc_rt_restart_save; RT_Step_Exec.f90:
+!   551-558); '-' for none.  mode: new (default) | resume (continue from the
+!   checkpoint at its load factor; increments resume at the nominal size).
+!   stop: end after that many converged increments in total (0: run to 1).
 ! Prints one line per increment and a summary; exits 0 when every increment
 ! converged, 1 otherwise, 2 on a C-ABI error.
 PROGRAM ufc_j2_step
@@ -37,7 +42,10 @@ PROGRAM ufc_j2_step
   INTEGER, PARAMETER :: max_newton = 25, max_cutbacks = 4
 
   CHARACTER(LEN=64) :: arg, prec_name
-  CHARACTER(LEN=512) :: outfile
+  CHARACTER(LEN=512) :: outfile, rstfile
+  CHARACTER(LEN=16) :: rstmode
+  INTEGER :: stop_after
+  TYPE(ufc_rt_restart_data) :: rst
   INTEGER :: n, n_inc, i, it, cutbacks, n_done, u_unit, total_newton, total_linear, lin
   INTEGER(C_INT32_T) :: nn, ne, prec, rc
   INTEGER(C_INT64_T) :: n_yield
@@ -63,6 +71,15 @@ PROGRAM ufc_j2_step
   CALL GET_COMMAND_ARGUMENT(2, prec_name)
   CALL GET_COMMAND_ARGUMENT(3, arg); READ(arg, *) n_inc
   CALL GET_COMMAND_ARGUMENT(4, outfile)
+  rstfile = '-'
+  rstmode = 'new'
+  stop_after = 0
+  IF (COMMAND_ARGUMENT_COUNT() >= 5) CALL GET_COMMAND_ARGUMENT(5, rstfile)
+  IF (COMMAND_ARGUMENT_COUNT() >= 6) CALL GET_COMMAND_ARGUMENT(6, rstmode)
+  IF (COMMAND_ARGUMENT_COUNT() >= 7) THEN
+    CALL GET_COMMAND_ARGUMENT(7, arg)
+    READ(arg, *) stop_after
+  END IF
   prec = UFC_PREC_FP64
   IF (TRIM(prec_name) == 'mixed') prec = UFC_PREC_MIXED
 
@@ -129,7 +146,16 @@ PROGRAM ufc_j2_step
   total_newton = 0
   total_linear = 0
   failed = .FALSE.
-  DO WHILE (lam < 1.0_C_DOUBLE - 1.0E-12_C_DOUBLE)
+  IF (TRIM(rstmode) == 'resume') THEN
+    rst%u = C_LOC(u)
+    rc = ufc_rt_restart_load(ctx, TRIM(rstfile) // C_NULL_CHAR, rst, 1_C_INT32_T, st)
+    CALL check('ufc_rt_restart_load')
+    lam = rst%lambda
+    n_done = rst%increment
+    WRITE(*, '(A,I0,A,F8.5)') 'resumed after increment ', n_done, ' lambda ', lam
+  END IF
+  DO WHILE (lam < 1.0_C_DOUBLE - 1.0E-12_C_DOUBLE .AND. &
+            (stop_after <= 0 .OR. n_done < stop_after))
     dlam = MIN(step, 1.0_C_DOUBLE - lam)
     target = lam + dlam
     ! element-wise into the fixed array: a whole-array assignment from the
@@ -200,6 +226,17 @@ PROGRAM ufc_j2_step
       cutbacks = 0
       WRITE(*, '(A,I0,A,F8.5,A,I0,A,I0,A,ES10.3,A,I0)') 'increment ', n_done, ' lambda ', lam, &
            ' newton ', it, ' linear ', lin, ' rrel ', rrel, ' n_yield ', n_yield
+      IF (TRIM(rstfile) /= '-') THEN
+        rst%time = lam
+        rst%lambda = lam
+        rst%increment = INT(n_done, C_INT32_T)
+        rst%iterations = INT(it, C_INT32_T)
+        rst%residual_norm = norms(1)
+        rst%converged = 1
+        rst%u = C_LOC(u)
+        rc = ufc_rt_restart_save(ctx, TRIM(rstfile) // C_NULL_CHAR, rst, 1_C_INT32_T, st)
+        CALL check('ufc_rt_restart_save')
+      END IF
     ELSE
       WRITE(*, '(A,F8.5,A,I0,A,ES10.3)') 'cutback at lambda ', target, ' after newton ', it, &
            ' rrel ', rrel

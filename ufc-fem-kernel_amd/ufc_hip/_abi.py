@@ -118,6 +118,12 @@ class ApModel(C.Structure):
                 ("impl", C.c_void_p)]
 
 
+class RestartData(C.Structure):
+    _fields_ = [("time", C.c_double), ("lambda_", C.c_double), ("residual_norm", C.c_double),
+                ("increment", C.c_int32), ("iterations", C.c_int32), ("converged", C.c_int32),
+                ("n_dof", C.c_int32), ("u", C.c_void_p)]
+
+
 P = C.POINTER
 _SIGS = {
     "ufc_ctx_create": [C.c_int32, P(C.c_void_p), P(Status)],
@@ -148,6 +154,8 @@ _SIGS = {
                      C.c_double, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_field": [C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_void_p,
                   P(Status)],
+    "ufc_rt_restart_save": [C.c_void_p, C.c_char_p, P(RestartData), C.c_int32, P(Status)],
+    "ufc_rt_restart_load": [C.c_void_p, C.c_char_p, P(RestartData), C.c_int32, P(Status)],
     "ufc_ap_read_inp": [C.c_char_p, P(ApModel), P(Status)],
     "ufc_ap_free": [P(ApModel), P(Status)],
 }

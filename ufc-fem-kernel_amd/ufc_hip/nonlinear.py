@@ -28,6 +28,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _abi as A
+from . import restart as R
 from .static import Context
 
 
@@ -68,26 +69,38 @@ def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
 
 
 def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
-              tol_r=1e-8, lin_rtol=None, max_newton=25, penalty=1.0e12, max_cutbacks=4):
+              tol_r=1e-8, lin_rtol=None, max_newton=25, penalty=1.0e12, max_cutbacks=4,
+              restart_file=None, checkpoint_freq=1, resume_from=None, stop_after=None):
     """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
     in n_increments equal increments.  ctx must hold the mesh (build_pattern).
     lin_rtol None: inexact Newton, each linear solve to the forcing term of
-    forcing_term(); a number: every linear solve to that relative tolerance."""
+    forcing_term(); a number: every linear solve to that relative tolerance.
+
+    restart_file: u, the step data and the committed J2 state are saved there
+    after every checkpoint_freq-th converged increment (RT_Step_Exec.f90:551-558).
+    resume_from: a restart file to continue from (load factor, increment count,
+    u and the J2 state); increments then resume at the nominal size
+    1/n_increments.  stop_after: return after that many converged increments
+    in total (a job that ends early, for restarts)."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
     ctx.j2_init(*props)
     n = ctx.n_dof
     u = np.zeros(n)
     lam = 0.0
+    count = 0
+    if resume_from is not None:
+        u, rs = R.load(resume_from, ctx=ctx, with_state=True)
+        lam, count = rs.lambda_, rs.increment
     step = 1.0 / n_increments
     res = NewtonResult(converged=True)
     cutbacks = 0
-    while lam < 1.0 - 1e-12:
+    while lam < 1.0 - 1e-12 and (stop_after is None or count < stop_after):
         dlam = min(step, 1.0 - lam)
         target = lam + dlam
         ctx.apply_bc(bc_eq, bc_target * target, penalty)
         u_n = u.copy()
-        ok, its, lin, rrel, ny = False, 0, 0, np.inf, 0
+        ok, its, lin, rrel, ny, rabs = False, 0, 0, np.inf, 0, np.inf
         u_try = u_n.copy()
         r_prev = eta = None
         for it in range(1, max_newton + 1):
@@ -97,6 +110,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
                 break  # local Newton failure: cut back
             r, norms = ctx.residual(u_try)
             its = it
+            rabs = norms[0]
             rrel = norms[0] / norms[1] if norms[1] > 0.0 else float("inf")
             if it > 1 and norms[0] <= tol_r * max(norms[1], 1e-300):
                 ok = True
@@ -116,8 +130,12 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             ctx.j2_commit()
             u = u_try
             lam = target
+            count += 1
             res.increments.append(IncrementInfo(lam, its, lin, rrel, ny))
             cutbacks = 0
+            if restart_file is not None and count % checkpoint_freq == 0:
+                R.save(restart_file, u, R.StepState(lam, count, lam, rabs, its, True), ctx=ctx,
+                       with_state=True)
         else:
             cutbacks += 1
             if cutbacks > max_cutbacks:
