@@ -111,9 +111,24 @@ int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
 /* L4_PH: element kernels                                                 */
 /* ====================================================================== */
 
+/* C3D8 variants: the PH_ELEM_C3D8_VARIANT_* values (PH_Elem_C3D8.f90:111-116)
+ * as PH_Elem_C3D8_StiffMatrixByVariant (:400-417) computes them.  0, 1 and 5
+ * are the standard 2x2x2 element; 3 (the reference's HYBRID) is the
+ * selective (B-bar) element.  REDUCED uses the hourglass base vectors
+ * xi.eta.zeta, eta.zeta, xi.zeta, xi.eta projected off the linear fields
+ * (the reference's 4th vector is the linear mode -zeta, which stiffens rigid
+ * rotations); INCOMPAT uses the modes 1 - xi_k^2 of each displacement
+ * component with J0-scaled gradients (the reference's three scalar modes
+ * fail the patch test). */
+#define UFC_VARIANT_STANDARD 1
+#define UFC_VARIANT_REDUCED 2   /* C3D8R: 1 point + hourglass stiffness    */
+#define UFC_VARIANT_SELECTIVE 3 /* B-bar (reference: VARIANT_HYBRID)       */
+#define UFC_VARIANT_INCOMPAT 4  /* C3D8I: incompatible modes, condensed    */
+#define UFC_VARIANT_27PT 6      /* 3x3x3 Gauss                             */
+
 typedef struct ufc_ph_ke_algo {
   int32_t precision; /* UFC_PREC_FP64 or UFC_PREC_FP32 (Ke written in it)  */
-  int32_t reserved;
+  int32_t variant;   /* UFC_VARIANT_* (0: standard)                         */
 } ufc_ph_ke_algo;
 
 /* PH_Elem_Eval_Ke_Arg (PH_Elem_Def.f90:218-222) for a batch of C3D8. */
@@ -129,13 +144,16 @@ typedef struct ufc_ph_ke_args {
 /* Batched Ke = sum_gp detJ w B^T D B, 2x2x2 Gauss: replaces
  * PH_Elem_Eval_Ke(desc,algo,arg,status) (PH_Elem_Eval.f90:35-81) ->
  * Compute_Ke_C3D8 (PH_ElemKeDispatch.f90:207-237) ->
- * PH_Elem_C3D8_StiffMatrix (PH_Elem_C3D8.f90:759-780). */
+ * PH_Elem_C3D8_StiffMatrix (PH_Elem_C3D8.f90:759-780); algo->variant
+ * selects PH_Elem_C3D8_StiffMatrixByVariant's element (:400-417). */
 int ufc_ph_c3d8_eval_ke(ufc_ctx* ctx, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* args,
                         ufc_status* st);
 
 /* Adjoint element sensitivities for E_e = E_e(rho_e):
  *   grad[e] = -dEdrho[e] * lambda_e^T K_e(E=1) u_e,
- * recomputing K_e(E=1) instead of storing it.  No counterpart in the
+ * recomputing K_e(E=1) instead of storing it (after an elastic assembly
+ * with another variant: that element, as lambda_e^T K_e u_e / E_e with the
+ * assembly's E_e and nu, which must then be non-zero).  No counterpart in the
  * reference (it has no sensitivity kernel); consumes the output of
  * ufc_nm_adjoint_solve (NM_Adjoint_Solve, NM_Solv_GMRESTranspose.f90:229-291). */
 typedef struct ufc_ph_sens_args {
@@ -197,6 +215,11 @@ int ufc_rt_asm_get_pattern(ufc_ctx* ctx, int32_t index_base, int32_t* row_ptr,
 typedef struct ufc_rt_asm_algo {
   int32_t precision; /* FP64 values; FP32 values; MIXED = FP32 values        */
   int32_t material;  /* UFC_MAT_ELASTIC or UFC_MAT_J2                        */
+  int32_t variant;   /* UFC_VARIANT_* for UFC_MAT_ELASTIC (0: standard); the  */
+                     /* FP64 refinement residual, ufc_rt_residual and         */
+                     /* ufc_ph_c3d8_sensitivity use the same element. J2:     */
+                     /* standard only (others: UFC_STATUS_UNSUPPORTED).       */
+  int32_t reserved;
 } ufc_rt_asm_algo;
 
 typedef struct ufc_rt_asm_args {

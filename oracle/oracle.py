@@ -51,6 +51,8 @@ def lib():
         L.orc_c3d8_dndx.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp, vp]
         L.orc_c3d8_ke.restype = C.c_int
         L.orc_c3d8_ke.argtypes = [vp, C.c_double, C.c_double, vp]
+        L.orc_c3d8_ke_variant.restype = C.c_int
+        L.orc_c3d8_ke_variant.argtypes = [vp, C.c_double, C.c_double, C.c_int, vp]
         L.orc_c3d8_ke_dgp.restype = C.c_int
         L.orc_c3d8_ke_dgp.argtypes = [vp, vp, vp]
         L.orc_c3d8_strain.argtypes = [vp, vp, vp]
@@ -150,6 +152,23 @@ def ke(xe, E, nu) -> np.ndarray:
     return K.T.copy()  # stored column-major
 
 
+VARIANT_STANDARD, VARIANT_REDUCED, VARIANT_SELECTIVE, VARIANT_INCOMPAT, VARIANT_27PT = 1, 2, 3, 4, 6
+
+
+def ke_variant(xe, E, nu, variant) -> np.ndarray:
+    """24x24 stiffness of a C3D8 variant (PH_Elem_C3D8_StiffMatrixByVariant,
+    PH_Elem_C3D8.f90:400-417): 1 standard, 2 reduced + hourglass, 3 selective
+    (B-bar), 4 incompatible modes, 6 27-point."""
+    xe = _f64(xe)
+    K = np.empty((24, 24))
+    rc = lib().orc_c3d8_ke_variant(_p(xe), E, nu, int(variant), _p(K))
+    if rc < 0:
+        raise ValueError(f"unknown variant {variant}")
+    if rc == 0:
+        raise ArithmeticError("incompatible-mode block not positive definite")
+    return K.T.copy()
+
+
 def ke_dgp(xe, Dgp) -> np.ndarray:
     """Ke with one 6x6 D per Gauss point; Dgp is (8,6,6) with Dgp[g][i][j]=D_ij."""
     xe = _f64(xe)
@@ -189,6 +208,25 @@ def csr_pattern(n_nodes: int, conn: np.ndarray):
     ci = np.empty(nnz, np.int32)
     lib().orc_csr_pattern(n_nodes, conn.shape[0], _p(conn), _p(rp), _p(ci))
     return rp, ci
+
+
+def assemble_variant(conn, coords, E, nu, variant, rp, ci):
+    """CSR values on (rp, ci) from the per-element ke_variant matrices, merged
+    as RT_CSR_FromTripletMerged does (element order within each entry)."""
+    conn = np.asarray(conn)
+    n_rows = rp.size - 1
+    ti, tj, tv = [], [], []
+    for e in range(conn.shape[0]):
+        K = ke_variant(coords[conn[e]], float(E[e]), nu, variant)
+        d = (3 * conn[e][:, None] + np.arange(3)).ravel()
+        ti.append(np.repeat(d, 24))
+        tj.append(np.tile(d, 24))
+        tv.append(K.ravel())
+    rp2, ci2, val = csr_from_triplets(n_rows, np.concatenate(ti), np.concatenate(tj),
+                                      np.concatenate(tv))
+    if not (np.array_equal(rp2, rp) and np.array_equal(ci2, ci)):
+        raise ValueError("pattern differs from the element triplets")
+    return val
 
 
 def csr_from_triplets(n_rows, ti, tj, tv=None):

@@ -176,9 +176,9 @@ int orc_c3d8_dndx(const double* coords, double xi, double eta, double zeta,
 
 /* PH_Elem_C3D8_BMatrix, PH_Elem_C3D8.f90:550-573: rows [xx,yy,zz,gxy,gyz,gxz],
  * B is (6,24) column-major. */
-static void bmatrix(const double dNdx[24], double B[144]) {
-  memset(B, 0, 144 * sizeof(double));
-  for (int a = 0; a < 8; ++a) {
+static void bmatrix_n(const double* dNdx, int n, double* B) {
+  memset(B, 0, 18 * n * sizeof(double));
+  for (int a = 0; a < n; ++a) {
     const double gx = dNdx[3 * a], gy = dNdx[3 * a + 1], gz = dNdx[3 * a + 2];
     const int c = 3 * a;
     B[0 + 6 * c] = gx;
@@ -192,6 +192,7 @@ static void bmatrix(const double dNdx[24], double B[144]) {
     B[5 + 6 * (c + 2)] = gx;
   }
 }
+static void bmatrix(const double dNdx[24], double B[144]) { bmatrix_n(dNdx, 8, B); }
 
 /* PH_Elem_C3D8_ConstMatrix, PH_Elem_C3D8.f90:603-625 ((6,6) column-major) */
 static void const_matrix(double E, double nu, double D[36]) {
@@ -251,6 +252,229 @@ int orc_c3d8_ke(const double* coords, double E, double nu, double* Ke) {
   const_matrix(E, nu, D);
   for (int g = 0; g < 8; ++g) memcpy(Dgp + 36 * g, D, sizeof D);
   return orc_c3d8_ke_dgp(coords, Dgp, Ke);
+}
+
+/* ---------------------------------------------------------------- variants
+ * PH_Elem_C3D8_StiffMatrixByVariant (PH_Elem_C3D8.f90:400-417): 1 standard
+ * (0 and 5, which the reference routes to the standard element, too),
+ * 2 reduced (C3D8R), 3 selective (the reference's VARIANT_HYBRID routes to
+ * PH_Elem_C3D8_StiffMatrixSelective), 4 incompatible modes, 6 27-point. */
+
+/* Ke += w B^T D B for one (6,ncol) B column-major into Ke (ncol,ncol) */
+static void add_btdb(const double* Ba, int na, const double* Bb, int nb, const double* D,
+                     double w, double* K, int ld) {
+  for (int j = 0; j < nb; ++j) {
+    double db[6];
+    for (int m = 0; m < 6; ++m) {
+      double s = 0.0;
+      for (int q = 0; q < 6; ++q) s += D[m + 6 * q] * Bb[q + 6 * j];
+      db[m] = s;
+    }
+    for (int i = 0; i < na; ++i) {
+      double s = 0.0;
+      for (int m = 0; m < 6; ++m) s += Ba[m + 6 * i] * db[m];
+      K[i + ld * j] += w * s;
+    }
+  }
+}
+
+/* J (j,k) = dx_j/dxi_k at (xi,eta,zeta), its determinant and inverse */
+static double jacobian(const double* coords, double xi, double eta, double zeta,
+                       double J[3][3], double Ji[3][3]) {
+  double N[8], dN[24];
+  orc_c3d8_shape(xi, eta, zeta, N, dN);
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) {
+      J[j][k] = 0.0;
+      for (int a = 0; a < 8; ++a) J[j][k] += dN[3 * a + k] * coords[3 * a + j];
+    }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][0] * J[2][2] - J[1][2] * J[2][0];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double d = J[0][0] * c00 - J[0][1] * c01 + J[0][2] * c02;
+  if (fabs(d) > 1.0e-20) {
+    Ji[0][0] = c00 / d;
+    Ji[0][1] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]) / d;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / d;
+    Ji[1][0] = -c01 / d;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / d;
+    Ji[1][2] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]) / d;
+    Ji[2][0] = c02 / d;
+    Ji[2][1] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]) / d;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / d;
+  } else {
+    memset(Ji, 0, 9 * sizeof(double));
+  }
+  return d;
+}
+
+/* PH_Elem_C3D8_StiffMatrixReduced (:466-498): one point at the centre with
+ * weight 8, plus k_hg sum_m gamma_m gamma_m^T (x) I3, k_hg = 0.1 mu V / V^(2/3),
+ * V the 8-point volume (PH_ELEM_C3D8_Volume_8pt, :291-304).  The reference's
+ * GAMMA8 (:123-127) holds xi.eta.zeta, eta.zeta, xi.zeta and, as its 4th
+ * column, the linear mode -zeta in place of xi.eta; here the 4th vector is
+ * xi.eta, and each base vector h is projected off the linear fields,
+ * gamma = h - sum_i (h . x_i) b_i with b_i the centre gradients
+ * (Flanagan & Belytschko 1981), which leaves h unchanged on parallelepipeds. */
+static int ke_reduced(const double* coords, double E, double nu, double* Ke) {
+  double D[36], g0[24], B[144], dj0;
+  const_matrix(E, nu, D);
+  memset(Ke, 0, 576 * sizeof(double));
+  if (orc_c3d8_dndx(coords, 0.0, 0.0, 0.0, g0, &dj0)) {
+    bmatrix(g0, B);
+    add_btdb(B, 24, B, 24, D, 8.0 * dj0, Ke, 24);
+  }
+  double V = 0.0;
+  for (int g = 0; g < 8; ++g) {
+    double xi, eta, zeta, J[3][3], Ji[3][3];
+    gauss_point(g, &xi, &eta, &zeta);
+    V += jacobian(coords, xi, eta, zeta, J, Ji);
+  }
+  if (V <= 1.0e-20) return 1;
+  const double mu = E / (2.0 * (1.0 + nu));
+  const double L = cbrt(V);
+  const double k = 0.1 * mu * V / (L * L);
+  for (int m = 0; m < 4; ++m) {
+    double h[8], gam[8], hx[3] = {0, 0, 0};
+    for (int a = 0; a < 8; ++a) {
+      const double x = XI_N[a], y = ETA_N[a], z = ZETA_N[a];
+      h[a] = m == 0 ? -x * y * z : m == 1 ? y * z : m == 2 ? x * z : x * y;
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int a = 0; a < 8; ++a) hx[i] += h[a] * coords[3 * a + i];
+    for (int a = 0; a < 8; ++a)
+      gam[a] = h[a] - (hx[0] * g0[3 * a] + hx[1] * g0[3 * a + 1] + hx[2] * g0[3 * a + 2]);
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b)
+        for (int d = 0; d < 3; ++d) Ke[(3 * a + d) + 24 * (3 * b + d)] += k * gam[a] * gam[b];
+  }
+  return 1;
+}
+
+/* PH_Elem_C3D8_StiffMatrixSelective (:500-540): B-bar, the volumetric rows
+ * (B1+B2+B3)/3 replaced by their volume average over the element. */
+static int ke_selective(const double* coords, double E, double nu, double* Ke) {
+  double D[36], g[24], B[144], bvol[24], bavg[24] = {0}, V = 0.0;
+  const_matrix(E, nu, D);
+  memset(Ke, 0, 576 * sizeof(double));
+  for (int p = 0; p < 8; ++p) {
+    double xi, eta, zeta, dj;
+    gauss_point(p, &xi, &eta, &zeta);
+    if (!orc_c3d8_dndx(coords, xi, eta, zeta, g, &dj)) continue;
+    bmatrix(g, B);
+    V += dj;
+    for (int c = 0; c < 24; ++c) bavg[c] += dj * (B[0 + 6 * c] + B[1 + 6 * c] + B[2 + 6 * c]) / 3.0;
+  }
+  if (V <= 1.0e-20) return 1;
+  for (int c = 0; c < 24; ++c) bavg[c] /= V;
+  for (int p = 0; p < 8; ++p) {
+    double xi, eta, zeta, dj;
+    gauss_point(p, &xi, &eta, &zeta);
+    if (!orc_c3d8_dndx(coords, xi, eta, zeta, g, &dj)) continue;
+    bmatrix(g, B);
+    for (int c = 0; c < 24; ++c) {
+      bvol[c] = (B[0 + 6 * c] + B[1 + 6 * c] + B[2 + 6 * c]) / 3.0;
+      for (int r = 0; r < 3; ++r) B[r + 6 * c] += bavg[c] - bvol[c];
+    }
+    add_btdb(B, 24, B, 24, D, dj, Ke, 24);
+  }
+  return 1;
+}
+
+/* Incompatible modes.  The reference's PH_Elem_C3D8_StiffMatrixIncompatible
+ * (:419-464) enriches u by three scalar modes along (1,1,1), two of which
+ * (xi(1-eta^2)(1-zeta^2), eta(1-xi^2)(1-zeta^2)) have a non-zero mean
+ * gradient, so the condensed element fails the patch test (2% soft in
+ * uniform strain on the unit cube).  Here: the incompatible modes
+ * P_k = 1 - xi_k^2 for each displacement component (9 modes, Wilson et al.
+ * 1973) with gradients J0^-T dP/dxi detJ0/detJ (J0 at the centre; Taylor,
+ * Beresford & Wilson 1976), whose integral over the element vanishes, and
+ * Ke = Kuu - Kua Kaa^-1 Kua^T. */
+static int ke_incompat(const double* coords, double E, double nu, double* Ke) {
+  double D[36], g[24], B[144], Be[54], J0[3][3], J0i[3][3];
+  double Kua[24 * 9] = {0}, Kaa[81] = {0};
+  const_matrix(E, nu, D);
+  memset(Ke, 0, 576 * sizeof(double));
+  const double dj0 = jacobian(coords, 0.0, 0.0, 0.0, J0, J0i);
+  for (int p = 0; p < 8; ++p) {
+    double xc[3], dj;
+    gauss_point(p, &xc[0], &xc[1], &xc[2]);
+    if (!orc_c3d8_dndx(coords, xc[0], xc[1], xc[2], g, &dj)) continue;
+    bmatrix(g, B);
+    double ge[9];  /* gradient of mode k: ge[3k + j] */
+    for (int k = 0; k < 3; ++k)
+      for (int j = 0; j < 3; ++j) ge[3 * k + j] = J0i[k][j] * (-2.0 * xc[k]) * dj0 / dj;
+    bmatrix_n(ge, 3, Be);
+    add_btdb(B, 24, B, 24, D, dj, Ke, 24);
+    add_btdb(B, 24, Be, 9, D, dj, Kua, 24);
+    add_btdb(Be, 9, Be, 9, D, dj, Kaa, 9);
+  }
+  /* Kaa = L L^T, W = L^-1 Kua^T (9,24); Ke -= W^T W */
+  double L[81] = {0};
+  for (int j = 0; j < 9; ++j) {
+    double s = Kaa[j + 9 * j];
+    for (int k = 0; k < j; ++k) s -= L[j + 9 * k] * L[j + 9 * k];
+    if (s <= 0.0) return 0;
+    L[j + 9 * j] = sqrt(s);
+    for (int i = j + 1; i < 9; ++i) {
+      double t = Kaa[i + 9 * j];
+      for (int k = 0; k < j; ++k) t -= L[i + 9 * k] * L[j + 9 * k];
+      L[i + 9 * j] = t / L[j + 9 * j];
+    }
+  }
+  double W[9 * 24];
+  for (int c = 0; c < 24; ++c)
+    for (int i = 0; i < 9; ++i) {
+      double t = Kua[c + 24 * i];
+      for (int k = 0; k < i; ++k) t -= L[i + 9 * k] * W[k + 9 * c];
+      W[i + 9 * c] = t / L[i + 9 * i];
+    }
+  for (int b = 0; b < 24; ++b)
+    for (int a = 0; a < 24; ++a) {
+      double s = 0.0;
+      for (int k = 0; k < 9; ++k) s += W[k + 9 * a] * W[k + 9 * b];
+      Ke[a + 24 * b] -= s;
+    }
+  return 1;
+}
+
+/* PH_Elem_C3D8_StiffMatrix27 (:1976-1995): 3x3x3 Gauss, points GP3 =
+ * -/+0.774596669241483 and 0 (the reference's literal, :130), weights 5/9,
+ * 8/9, 5/9, order of PH_Elem_C3D8_GaussPoints27 (:717-732). */
+static int ke_gauss27(const double* coords, double E, double nu, double* Ke) {
+  static const double P3[3] = {-0.774596669241483, 0.0, 0.774596669241483};
+  const double W3[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  double D[36], g[24], B[144];
+  const_matrix(E, nu, D);
+  memset(Ke, 0, 576 * sizeof(double));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        double dj;
+        if (!orc_c3d8_dndx(coords, P3[i], P3[j], P3[k], g, &dj)) continue;
+        bmatrix(g, B);
+        add_btdb(B, 24, B, 24, D, dj * W3[i] * W3[j] * W3[k], Ke, 24);
+      }
+  return 1;
+}
+
+int orc_c3d8_ke_variant(const double* coords, double E, double nu, int variant, double* Ke) {
+  switch (variant) {
+    case 0:
+    case 1:
+    case 5:
+      return orc_c3d8_ke(coords, E, nu, Ke);
+    case 2:
+      return ke_reduced(coords, E, nu, Ke);
+    case 3:
+      return ke_selective(coords, E, nu, Ke);
+    case 4:
+      return ke_incompat(coords, E, nu, Ke);
+    case 6:
+      return ke_gauss27(coords, E, nu, Ke);
+    default:
+      return -1;
+  }
 }
 
 /* strain at each Gauss point, eps = B u_e (PH_Elem_C3D8_Strain as used by

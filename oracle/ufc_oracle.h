@@ -42,6 +42,8 @@ int  orc_c3d8_dndx(const double* coords, double xi, double eta, double zeta,
                    double dNdx[24], double* detJ);
 int  orc_c3d8_ke(const double* coords, double E, double nu, double* Ke);
 int  orc_c3d8_ke_dgp(const double* coords, const double* Dgp, double* Ke);
+/* variants 0,1,5 standard, 2 reduced, 3 selective, 4 incompatible, 6 27-point */
+int  orc_c3d8_ke_variant(const double* coords, double E, double nu, int variant, double* Ke);
 void orc_c3d8_strain(const double* coords, const double* ue, double* eps_gp);
 void orc_c3d8_fint(const double* coords, const double* sig_gp, double* Fe);
 void orc_c3d8_face_pressure(const double* coords, int face, double p, double* Fe);

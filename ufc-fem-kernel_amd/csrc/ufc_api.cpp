@@ -249,6 +249,8 @@ int ufc_ph_c3d8_eval_ke(ufc_ctx* c, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* 
     ufc::DBuf<double> tc, ty;
     const double* dc = dev_in(c, a->coords, ne * 24, tc);
     const double* dy = dev_in(c, a->young, ne, ty);
+    const int variant = ufc::canonical_variant(algo->variant);
+    UFC_REQUIRE(variant > 0, UFC_STATUS_INVALID, "unknown element variant");
     const bool fp32 = algo->precision == UFC_PREC_FP32;
     const size_t esz = fp32 ? sizeof(float) : sizeof(double);
     void* out = a->ke;
@@ -257,7 +259,9 @@ int ufc_ph_c3d8_eval_ke(ufc_ctx* c, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* 
       tk.resize(ne * 576 * esz);
       out = tk.get();
     }
-    ufc::launch_eval_ke(a->n_elems, dc, dy, a->nu, out, fp32, c->stream);
+    ufc::DBuf<double> tcf;
+    if (variant == ufc::kVarIncompat) tcf.resize(ne * ufc::kIncompatC);
+    ufc::launch_eval_ke(a->n_elems, dc, dy, a->nu, out, fp32, variant, tcf.get(), c->stream);
     if (out != a->ke)
       UFC_HIP_CHECK(hipMemcpyAsync(a->ke, out, ne * 576 * esz, hipMemcpyDeviceToHost, c->stream));
     UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -589,8 +593,14 @@ int ufc_rt_asm_assemble_k(ufc_ctx* c, const ufc_rt_asm_algo* algo, const ufc_rt_
                 UFC_STATUS_INVALID, "unknown precision");
     UFC_REQUIRE(algo->material == UFC_MAT_ELASTIC || algo->material == UFC_MAT_J2,
                 UFC_STATUS_INVALID, "unknown material");
+    const int variant = ufc::canonical_variant(algo->variant);
+    UFC_REQUIRE(variant > 0, UFC_STATUS_INVALID, "unknown element variant");
+    UFC_REQUIRE(algo->material == UFC_MAT_ELASTIC || variant == ufc::kVarFull,
+                UFC_STATUS_UNSUPPORTED, "J2 assembly is available for the standard element only");
     UFC_HIP_CHECK(hipSetDevice(c->device));
     if (algo->material == UFC_MAT_ELASTIC) {
+      c->variant = variant;
+      if (variant == ufc::kVarIncompat) c->inc_c.resize(size_t(c->n_elems) * ufc::kIncompatC);
       UFC_REQUIRE(a && a->young, UFC_STATUS_INVALID, "elastic assembly needs young[]");
       UFC_REQUIRE(a->nu > -1.0 && a->nu < 0.5, UFC_STATUS_INVALID, "nu out of (-1, 0.5)");
       c->young.resize(size_t(c->n_elems));

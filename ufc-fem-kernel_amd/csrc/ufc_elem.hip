@@ -28,14 +28,14 @@ __device__ __forceinline__ T tabs(T v) {
 // Shape-function derivatives w.r.t. x at Gauss point gp (order of
 // PH_Elem_C3D8_GaussPoints, :694-715: xi outer, zeta inner; weights 1).
 // Returns false when |detJ| <= 1e-12 (the point is skipped, :775).
+constexpr int XS[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+constexpr int YS[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+constexpr int ZS[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
+// dN/dx at (xi, eta, zeta); also J^-1 (Ji[k][j] = dxi_k/dx_j) when Ji != null
 template <typename T>
-__device__ __forceinline__ bool c3d8_dndx(const T (&x)[8][3], int gp, T (&g)[8][3], T& detJ) {
-  constexpr int XS[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
-  constexpr int YS[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
-  constexpr int ZS[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
-  const T xi = (gp & 4) ? T(kGaussPt) : T(-kGaussPt);
-  const T eta = (gp & 2) ? T(kGaussPt) : T(-kGaussPt);
-  const T zeta = (gp & 1) ? T(kGaussPt) : T(-kGaussPt);
+__device__ __forceinline__ bool c3d8_dndx_at(const T (&x)[8][3], T xi, T eta, T zeta,
+                                             T (&g)[8][3], T& detJ, T (*Jinv)[3] = nullptr) {
   T dn[8][3];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -76,7 +76,22 @@ __device__ __forceinline__ bool c3d8_dndx(const T (&x)[8][3], int gp, T (&g)[8][
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int j = 0; j < 3; ++j) g[a][j] = Ji[0][j] * dn[a][0] + Ji[1][j] * dn[a][1] + Ji[2][j] * dn[a][2];
+  if (Jinv) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Jinv[k][j] = Ji[k][j];
+  }
   return true;
+}
+
+// at Gauss point gp of PH_Elem_C3D8_GaussPoints (xi outer, zeta inner)
+template <typename T>
+__device__ __forceinline__ bool c3d8_dndx(const T (&x)[8][3], int gp, T (&g)[8][3], T& detJ) {
+  const T xi = (gp & 4) ? T(kGaussPt) : T(-kGaussPt);
+  const T eta = (gp & 2) ? T(kGaussPt) : T(-kGaussPt);
+  const T zeta = (gp & 1) ? T(kGaussPt) : T(-kGaussPt);
+  return c3d8_dndx_at(x, xi, eta, zeta, g, detJ);
 }
 
 template <typename T>
@@ -120,44 +135,339 @@ __device__ __forceinline__ void bt_d(const T* ga, const double* __restrict__ D, 
   }
 }
 
-// ---------------------------------------------------------------- batch Ke
-// One wave64 per element, lane = (a, b): the 3x3 block K_ab over 8 Gauss
-// points, written column-major into Ke(24,24,e).
+// ga = row a of the (8, 3) register table g without dynamic register
+// indexing.  A macro, not a function: with g behind a reference the select
+// chain is folded into an indexed load before inlining, and g then lives in
+// scratch memory.
+#define UFC_SELECT_ROW(g, a, ga)                                     \
+  do {                                                               \
+    _Pragma("unroll") for (int q_ = 0; q_ < 3; ++q_) {               \
+      auto v_ = (g)[0][q_];                                          \
+      _Pragma("unroll") for (int r_ = 1; r_ < 8; ++r_) v_ = ((a) == r_) ? (g)[r_][q_] : v_; \
+      (ga)[q_] = v_;                                                 \
+    }                                                                \
+  } while (0)
+
+// kb[b] += la ga gb^T + mw gb ga^T + mw (ga . gb) I, the isotropic B_a^T D B_b
 template <typename T>
-__global__ __launch_bounds__(64) void k_eval_ke(int32_t n_elems, const double* __restrict__ xe,
-                                                const double* __restrict__ young, double nu,
-                                                T* __restrict__ ke) {
-  const int64_t e = blockIdx.x;
-  if (e >= n_elems) return;
-  const int a = threadIdx.x >> 3, b = threadIdx.x & 7;
-  T x[8][3];
+__device__ __forceinline__ void add_iso_blocks(const T (&g)[8][3], const T (&ga)[3], T la, T mw,
+                                               T (&kb)[8][3][3]) {
 #pragma unroll
-  for (int q = 0; q < 8; ++q)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[q][d] = T(xe[24 * e + 3 * q + d]);
-  const double E = young[e];
-  const T lam = T(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
-  const T mu = T(E / (2.0 * (1.0 + nu)));
-  T k[3][3] = {};
-#pragma unroll
-  for (int gp = 0; gp < 8; ++gp) {
-    T g[8][3], dj;
-    if (!c3d8_dndx(x, gp, g, dj)) continue;
-    const T ga0 = g[a][0], ga1 = g[a][1], ga2 = g[a][2];
+  for (int b = 0; b < 8; ++b) {
     const T gb0 = g[b][0], gb1 = g[b][1], gb2 = g[b][2];
-    const T ga[3] = {ga0, ga1, ga2}, gb[3] = {gb0, gb1, gb2};
-    const T dotab = ga0 * gb0 + ga1 * gb1 + ga2 * gb2;
+    const T dd = mw * (ga[0] * gb0 + ga[1] * gb1 + ga[2] * gb2);
+    const T gb[3] = {gb0, gb1, gb2};
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        k[i][j] += dj * (lam * ga[i] * gb[j] + mu * ga[j] * gb[i] + (i == j ? mu * dotab : T(0)));
+        kb[b][i][j] += la * ga[i] * gb[j] + mw * ga[j] * gb[i] + (i == j ? dd : T(0));
   }
+}
+
+// Rows of node a of the elastic K_e, kb[b] = K_ab (added to kb), for the
+// variant VAR of PH_Elem_C3D8_StiffMatrixByVariant (PH_Elem_C3D8.f90:400-417):
+//   kVarFull      2x2x2 Gauss (PH_Elem_C3D8_StiffMatrix, :759-780)
+//   kVarGauss27   3x3x3 Gauss, points 0, -/+0.774596669241483, weights 5/9,
+//                 8/9 (PH_Elem_C3D8_StiffMatrix27, :1976-1995)
+//   kVarSelective B-bar (PH_Elem_C3D8_StiffMatrixSelective, :500-540): with
+//                 the volumetric rows replaced by their element average the
+//                 block is sum_gp dV [(lam - kap) ga gb^T + mu (gb ga^T +
+//                 ga.gb I)] + kap s_a s_b^T / V, kap = lam + 2 mu / 3,
+//                 s_a = sum_gp dV g_a, V = sum_gp dV
+//   kVarReduced   one point at the centre, weight 8, plus the hourglass
+//                 stiffness k sum_m gamma_m,a gamma_m,b I, k = 0.1 mu V^(1/3)
+//                 (PH_Elem_C3D8_StiffMatrixReduced, :466-498) with base
+//                 vectors xi.eta.zeta, eta.zeta, xi.zeta, xi.eta projected
+//                 off the linear fields (Flanagan & Belytschko 1981; the
+//                 reference's GAMMA8, :123-127, holds -zeta in place of xi.eta)
+//   kVarIncompat  the full block minus the incompatible-mode condensation
+//                 C_a C_b^T, C (24, 9) per element from k_incompat_c
+template <typename T, int VAR>
+__device__ __forceinline__ void elem_rows(const T (&x)[8][3], int a, T lam, T mu,
+                                          const double* __restrict__ cinc, T (&kb)[8][3][3]) {
+  if constexpr (VAR == kVarFull || VAR == kVarIncompat) {
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      T g[8][3], dj;
+      if (!c3d8_dndx(x, gp, g, dj)) continue;
+      T ga[3];
+      UFC_SELECT_ROW(g, a, ga);
+      add_iso_blocks(g, ga, lam * dj, mu * dj, kb);
+    }
+    if constexpr (VAR == kVarIncompat) {
+      T ca[3][9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) ca[i][k] = T(cinc[9 * (3 * a + i) + k]);
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          T cb[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) cb[k] = T(cinc[9 * (3 * b + j) + k]);
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            T t = T(0);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) t += ca[i][k] * cb[k];
+            kb[b][i][j] -= t;
+          }
+        }
+    }
+  } else if constexpr (VAR == kVarSelective) {
+    const T kap = lam + T(2) * mu / T(3);
+    T sg[8][3] = {}, V = T(0);
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      T g[8][3], dj;
+      if (!c3d8_dndx(x, gp, g, dj)) continue;
+      T ga[3];
+      UFC_SELECT_ROW(g, a, ga);
+      add_iso_blocks(g, ga, (lam - kap) * dj, mu * dj, kb);
+      V += dj;
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sg[b][j] += dj * g[b][j];
+    }
+    if (V <= T(1.0e-20)) {  // the reference returns Ke = 0
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) kb[b][i][j] = T(0);
+      return;
+    }
+    T sa[3];
+    UFC_SELECT_ROW(sg, a, sa);
+    const T f = kap / V;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) kb[b][i][j] += f * sa[i] * sg[b][j];
+  } else if constexpr (VAR == kVarGauss27) {
+#pragma unroll 1
+    for (int p = 0; p < 27; ++p) {
+      const int i = p / 9, j = (p / 3) % 3, k = p % 3;
+      const T c = T(0.774596669241483);
+      const T xi = i == 0 ? -c : (i == 1 ? T(0) : c);
+      const T eta = j == 0 ? -c : (j == 1 ? T(0) : c);
+      const T zeta = k == 0 ? -c : (k == 1 ? T(0) : c);
+      const T wi = i == 1 ? T(8.0 / 9.0) : T(5.0 / 9.0);
+      const T wj = j == 1 ? T(8.0 / 9.0) : T(5.0 / 9.0);
+      const T wk = k == 1 ? T(8.0 / 9.0) : T(5.0 / 9.0);
+      T g[8][3], dj;
+      if (!c3d8_dndx_at(x, xi, eta, zeta, g, dj)) continue;
+      const T w = dj * (wi * wj * wk);
+      T ga[3];
+      UFC_SELECT_ROW(g, a, ga);
+      add_iso_blocks(g, ga, lam * w, mu * w, kb);
+    }
+  } else if constexpr (VAR == kVarReduced) {
+    T g0[8][3], dj0;
+    if (c3d8_dndx_at(x, T(0), T(0), T(0), g0, dj0)) {
+      T ga[3];
+      UFC_SELECT_ROW(g0, a, ga);
+      add_iso_blocks(g0, ga, T(8) * lam * dj0, T(8) * mu * dj0, kb);
+    } else {
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g0[b][j] = T(0);
+    }
+    T V = T(0);
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      T g[8][3], dj;
+      c3d8_dndx(x, gp, g, dj);
+      V += dj;
+    }
+    if (V <= T(1.0e-20)) return;
+    const T L = T(cbrt(double(V)));
+    const T khg = T(0.1) * mu * V / (L * L);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      T h[8], hx[3] = {T(0), T(0), T(0)};
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        const int hv = m == 0 ? -XS[n] * YS[n] * ZS[n]
+                              : (m == 1 ? YS[n] * ZS[n] : (m == 2 ? XS[n] * ZS[n] : XS[n] * YS[n]));
+        h[n] = T(hv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) hx[i] += h[n] * x[n][i];
+      }
+      T gam[8], gma = T(0);
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        gam[n] = h[n] - (hx[0] * g0[n][0] + hx[1] * g0[n][1] + hx[2] * g0[n][2]);
+        gma = (a == n) ? gam[n] : gma;
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const T t = khg * gma * gam[b];
+        kb[b][0][0] += t;
+        kb[b][1][1] += t;
+        kb[b][2][2] += t;
+      }
+    }
+  }
+}
+
+// element coordinates: conn == nullptr reads (8, 3, n) element coordinates
+template <typename T>
+__device__ __forceinline__ void load_x(const double* __restrict__ coords,
+                                       const int32_t* __restrict__ conn, int64_t e,
+                                       T (&x)[8][3]) {
+  if (conn) {
+    gather_coords(coords, conn, e, x);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[q][d] = T(coords[24 * e + 3 * q + d]);
+  }
+}
+
+// ---------------------------------------------------------------- incompatible modes
+// C (24, 9) with K_cond = Kuu - C C^T: the modes P_k = 1 - xi_k^2 of each
+// displacement component (Wilson et al. 1973) with gradients
+// J0^-T dP/dxi detJ0/detJ (Taylor, Beresford & Wilson 1976) in place of the
+// reference's three scalar modes (PH_Elem_C3D8.f90:240-259, which fail the
+// patch test); Kaa = L L^T, C = Kua L^-T.  One wave per element: lanes 0-23
+// form the rows of Kua, lanes 24-32 the rows of Kaa, lane 0 factors Kaa and
+// lanes 0-23 solve their row of C.  A Kaa that is not positive definite
+// (degenerate element) gives C = 0, i.e. Kuu, as the reference's
+// det(Kaa) ~ 0 branch (:457-460).
+__global__ __launch_bounds__(64) void k_incompat_c(int32_t n_elems,
+                                                   const double* __restrict__ coords,
+                                                   const int32_t* __restrict__ conn,
+                                                   const double* __restrict__ young, double nu,
+                                                   double* __restrict__ cout) {
+  __shared__ double kua[24][9];
+  __shared__ double kaa[9][9];
+  __shared__ double lf[9][9];
+  __shared__ int ok;
+  const int64_t e = blockIdx.x;
+  const int r = threadIdx.x;
+  double x[8][3];
+  load_x(coords, conn, e, x);
+  const double E = young[e];
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E / (2.0 * (1.0 + nu));
+  double g0[8][3], dj0, J0i[3][3];
+  const bool c_ok = c3d8_dndx_at(x, 0.0, 0.0, 0.0, g0, dj0, J0i);
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < 33 && c_ok) {
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      const double xc[3] = {(gp & 4) ? kGaussPt : -kGaussPt, (gp & 2) ? kGaussPt : -kGaussPt,
+                            (gp & 1) ? kGaussPt : -kGaussPt};
+      double g[8][3], dj;
+      if (!c3d8_dndx(x, gp, g, dj)) continue;
+      double ge[3][3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ge[k][j] = J0i[k][j] * (-2.0 * xc[k]) * dj0 / dj;
+      // this lane's row: pseudo-node gradient gr, component i
+      double gr[3];
+      int i;
+      if (r < 24) {
+        UFC_SELECT_ROW(g, r / 3, gr);
+        i = r % 3;
+      } else {
+        const int k = (r - 24) / 3;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gr[q] = k == 0 ? ge[0][q] : (k == 1 ? ge[1][q] : ge[2][q]);
+        i = (r - 24) % 3;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double dd = mu * (gr[0] * ge[q][0] + gr[1] * ge[q][1] + gr[2] * ge[q][2]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double gri = i == 0 ? gr[0] : (i == 1 ? gr[1] : gr[2]);
+          const double grj = gr[j];
+          const double gqi = i == 0 ? ge[q][0] : (i == 1 ? ge[q][1] : ge[q][2]);
+          acc[3 * q + j] += dj * (lam * gri * ge[q][j] + mu * grj * gqi + (i == j ? dd : 0.0));
+        }
+      }
+    }
+  }
+  if (r < 24) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) kua[r][k] = acc[k];
+  } else if (r < 33) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) kaa[r - 24][k] = acc[k];
+  }
+  __syncthreads();
+  if (r == 0) {
+    int good = c_ok ? 1 : 0;
+    for (int j = 0; j < 9 && good; ++j) {
+      double sj = kaa[j][j];
+      for (int k = 0; k < j; ++k) sj -= lf[j][k] * lf[j][k];
+      if (!(sj > 0.0)) {
+        good = 0;
+        break;
+      }
+      lf[j][j] = sqrt(sj);
+      for (int i2 = j + 1; i2 < 9; ++i2) {
+        double t = kaa[i2][j];
+        for (int k = 0; k < j; ++k) t -= lf[i2][k] * lf[j][k];
+        lf[i2][j] = t / lf[j][j];
+      }
+    }
+    ok = good;
+  }
+  __syncthreads();
+  if (r < 24) {
+    double w[9];
+#pragma unroll
+    for (int i2 = 0; i2 < 9; ++i2) {
+      double t = kua[r][i2];
+#pragma unroll
+      for (int k = 0; k < i2; ++k) t -= lf[i2][k] * w[k];
+      w[i2] = ok ? t / lf[i2][i2] : 0.0;
+    }
+    double* out = cout + int64_t(kIncompatC) * e + 9 * r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) out[k] = w[k];
+  }
+}
+
+// ---------------------------------------------------------------- batch Ke
+// Eight lanes per element, lane = node a: rows 3a..3a+2 of Ke, written
+// column-major into Ke(24,24,e).
+template <typename T, int VAR>
+__global__ __launch_bounds__(64) void k_eval_ke(int32_t n_elems, const double* __restrict__ xe,
+                                                const double* __restrict__ young, double nu,
+                                                const double* __restrict__ inc_c,
+                                                T* __restrict__ ke) {
+  const int64_t e = int64_t(blockIdx.x) * 8 + (threadIdx.x >> 3);
+  const int a = threadIdx.x & 7;
+  if (e >= n_elems) return;
+  T x[8][3];
+  load_x(xe, nullptr, e, x);
+  const double E = young[e];
+  const T lam = T(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+  const T mu = T(E / (2.0 * (1.0 + nu)));
+  T kb[8][3][3] = {};
+  elem_rows<T, VAR>(x, a, lam, mu, VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * e : nullptr,
+                    kb);
   T* out = ke + 576 * e;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int b = 0; b < 8; ++b)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) out[(3 * a + i) + 24 * (3 * b + j)] = k[i][j];
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out[(3 * a + i) + 24 * (3 * b + j)] = kb[b][i][j];
 }
 
 // ---------------------------------------------------------------- assembly
@@ -169,12 +479,12 @@ __global__ __launch_bounds__(64) void k_eval_ke(int32_t n_elems, const double* _
 // exactly once, without atomics.  The 8 nodes' segments are contiguous in the
 // values array (node n's 3 rows start at 9*nb_ptr[n]), so the write-out is a
 // single coalesced stream.
-template <typename T, int MAT>
+template <typename T, int MAT, int VAR>
 __global__ __launch_bounds__(64) void k_assemble(
     int32_t n_nodes, const double* __restrict__ coords, const int32_t* __restrict__ conn,
     const int32_t* __restrict__ slot_elem, const uint64_t* __restrict__ slot_code,
     const int32_t* __restrict__ nb_ptr, const double* __restrict__ young, double nu,
-    const double* __restrict__ dgp, T* __restrict__ val) {
+    const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val) {
   __shared__ T acc[8 * 9 * kMaxNodeNbrs];
   const int lane = threadIdx.x;
   const int nl = lane >> 3, s = lane & 7;
@@ -206,39 +516,19 @@ __global__ __launch_bounds__(64) void k_assemble(
     const int a = int((code >> 40) & 7);
     T x[8][3];
     gather_coords(coords, conn, elem, x);
-    T lam = T(0), mu = T(0);
-    if (MAT == UFC_MAT_ELASTIC) {
+    if constexpr (MAT == UFC_MAT_ELASTIC) {
       const double E = young[elem];
-      lam = T(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
-      mu = T(E / (2.0 * (1.0 + nu)));
-    }
+      const T lam = T(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+      const T mu = T(E / (2.0 * (1.0 + nu)));
+      elem_rows<T, VAR>(x, a, lam, mu,
+                        VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * elem : nullptr, kb);
+    } else {
 #pragma unroll
-    for (int gp = 0; gp < 8; ++gp) {
-      T g[8][3], dj;
-      if (!c3d8_dndx(x, gp, g, dj)) continue;
-      T ga[3];
-      // select row a of g without dynamic register indexing
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        T v = g[0][q];
-#pragma unroll
-        for (int r = 1; r < 8; ++r) v = (a == r) ? g[r][q] : v;
-        ga[q] = v;
-      }
-      if (MAT == UFC_MAT_ELASTIC) {
-        const T la = lam * dj, mw = mu * dj;
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const T gb0 = g[b][0], gb1 = g[b][1], gb2 = g[b][2];
-          const T dd = mw * (ga[0] * gb0 + ga[1] * gb1 + ga[2] * gb2);
-          const T gb[3] = {gb0, gb1, gb2};
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-              kb[b][i][j] += la * ga[i] * gb[j] + mw * ga[j] * gb[i] + (i == j ? dd : T(0));
-        }
-      } else {
+      for (int gp = 0; gp < 8; ++gp) {
+        T g[8][3], dj;
+        if (!c3d8_dndx(x, gp, g, dj)) continue;
+        T ga[3];
+        UFC_SELECT_ROW(g, a, ga);
         T bd[3][6];
         bt_d(ga, dgp + 36 * (8 * int64_t(elem) + gp), bd);
 #pragma unroll
@@ -412,6 +702,56 @@ __global__ void k_node_gather(int32_t n_nodes, const int32_t* __restrict__ slot_
   y[3 * n + 0] = s0;
   y[3 * n + 1] = s1;
   y[3 * n + 2] = s2;
+}
+
+// ---------------------------------------------------------------- variant apply
+// Eight lanes per element, lane = node a: f_a = sum_b K_ab u_b with the
+// elastic K_e of variant VAR.  SENS = false: w[24 e + 3 a + d] = f_a (element
+// forces for the node gather); SENS = true: grad_e = -dE/drho_e lam_e^T K_e u_e / E_e
+// (K_e is linear in E_e for every variant).
+template <int VAR, bool SENS>
+__global__ __launch_bounds__(64) void k_elem_apply(int32_t n_elems,
+                                                   const double* __restrict__ coords,
+                                                   const int32_t* __restrict__ conn,
+                                                   const double* __restrict__ young, double nu,
+                                                   const double* __restrict__ inc_c,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ lamv,
+                                                   const double* __restrict__ dedrho,
+                                                   double* __restrict__ out) {
+  const int64_t e0 = int64_t(blockIdx.x) * 8 + (threadIdx.x >> 3);
+  const int a = threadIdx.x & 7;
+  const bool live = e0 < n_elems;
+  const int64_t e = live ? e0 : 0;
+  double x[8][3];
+  gather_coords(coords, conn, e, x);
+  const double E = young[e];
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E / (2.0 * (1.0 + nu));
+  double kb[8][3][3] = {};
+  elem_rows<double, VAR>(x, a, lam, mu,
+                         VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * e : nullptr, kb);
+  double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int64_t n = conn[8 * e + b];
+    const double u0 = u[3 * n], u1 = u[3 * n + 1], u2 = u[3 * n + 2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f[i] += kb[b][i][0] * u0 + kb[b][i][1] * u1 + kb[b][i][2] * u2;
+  }
+  if constexpr (!SENS) {
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) out[24 * e + 3 * a + i] = f[i];
+    }
+  } else {
+    const int64_t na = conn[8 * e + a];
+    double t = lamv[3 * na] * f[0] + lamv[3 * na + 1] * f[1] + lamv[3 * na + 2] * f[2];
+    t += __shfl_xor(t, 1, 8);
+    t += __shfl_xor(t, 2, 8);
+    t += __shfl_xor(t, 4, 8);
+    if (live && a == 0) out[e] = -dedrho[e] * t / E;
+  }
 }
 
 // ---------------------------------------------------------------- sensitivity
@@ -588,43 +928,131 @@ __global__ __launch_bounds__(128) void k_j2_update(int32_t n_elems,
 
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
+template <int VAR>
+void eval_ke_var(int32_t n_elems, const double* coords, const double* young, double nu, void* ke,
+                 bool fp32, const double* inc_c, hipStream_t s) {
+  const unsigned grid = blocks_for(n_elems, 8);
+  if (fp32)
+    hipLaunchKernelGGL((k_eval_ke<float, VAR>), dim3(grid), dim3(64), 0, s, n_elems, coords, young,
+                       nu, inc_c, static_cast<float*>(ke));
+  else
+    hipLaunchKernelGGL((k_eval_ke<double, VAR>), dim3(grid), dim3(64), 0, s, n_elems, coords,
+                       young, nu, inc_c, static_cast<double*>(ke));
+}
+
+template <typename T, int VAR>
+void assemble_elastic(ufc_ctx* c, T* val) {
+  hipLaunchKernelGGL((k_assemble<T, UFC_MAT_ELASTIC, VAR>), dim3(blocks_for(c->n_nodes, 8)),
+                     dim3(64), 0, c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
+                     c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), c->young.get(),
+                     c->nu, nullptr, c->inc_c.get(), val);
+}
+
+template <typename T>
+void assemble_variant(ufc_ctx* c, T* val) {
+  switch (c->variant) {
+    case kVarReduced:
+      assemble_elastic<T, kVarReduced>(c, val);
+      break;
+    case kVarSelective:
+      assemble_elastic<T, kVarSelective>(c, val);
+      break;
+    case kVarIncompat:
+      assemble_elastic<T, kVarIncompat>(c, val);
+      break;
+    case kVarGauss27:
+      assemble_elastic<T, kVarGauss27>(c, val);
+      break;
+    default:
+      assemble_elastic<T, kVarFull>(c, val);
+  }
+}
+
+template <int VAR, bool SENS>
+void apply_var(ufc_ctx* c, const double* u, const double* lamv, const double* dedrho,
+               double* out) {
+  hipLaunchKernelGGL((k_elem_apply<VAR, SENS>), dim3(blocks_for(c->n_elems, 8)), dim3(64), 0,
+                     c->stream, c->n_elems, c->coords.get(), c->conn.get(), c->young.get(), c->nu,
+                     c->inc_c.get(), u, lamv, dedrho, out);
+}
+
+template <bool SENS>
+void apply_variant(ufc_ctx* c, const double* u, const double* lamv, const double* dedrho,
+                   double* out) {
+  switch (c->variant) {
+    case kVarReduced:
+      apply_var<kVarReduced, SENS>(c, u, lamv, dedrho, out);
+      break;
+    case kVarSelective:
+      apply_var<kVarSelective, SENS>(c, u, lamv, dedrho, out);
+      break;
+    case kVarIncompat:
+      apply_var<kVarIncompat, SENS>(c, u, lamv, dedrho, out);
+      break;
+    case kVarGauss27:
+      apply_var<kVarGauss27, SENS>(c, u, lamv, dedrho, out);
+      break;
+    default:
+      apply_var<kVarFull, SENS>(c, u, lamv, dedrho, out);
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
-void launch_eval_ke(int32_t n_elems, const double* coords, const double* young, double nu,
-                    void* ke, bool fp32, hipStream_t s) {
+void launch_incompat_c(int32_t n_elems, const double* coords, const int32_t* conn,
+                       const double* young, double nu, double* inc_c, hipStream_t s) {
   if (n_elems <= 0) return;
-  if (fp32)
-    hipLaunchKernelGGL(k_eval_ke<float>, dim3(n_elems), dim3(64), 0, s, n_elems, coords, young,
-                       nu, static_cast<float*>(ke));
-  else
-    hipLaunchKernelGGL(k_eval_ke<double>, dim3(n_elems), dim3(64), 0, s, n_elems, coords, young,
-                       nu, static_cast<double*>(ke));
+  hipLaunchKernelGGL(k_incompat_c, dim3(n_elems), dim3(64), 0, s, n_elems, coords, conn, young,
+                     nu, inc_c);
+  UFC_HIP_CHECK(hipGetLastError());
+}
+
+void launch_eval_ke(int32_t n_elems, const double* coords, const double* young, double nu,
+                    void* ke, bool fp32, int variant, double* inc_c, hipStream_t s) {
+  if (n_elems <= 0) return;
+  switch (variant) {
+    case kVarReduced:
+      eval_ke_var<kVarReduced>(n_elems, coords, young, nu, ke, fp32, nullptr, s);
+      break;
+    case kVarSelective:
+      eval_ke_var<kVarSelective>(n_elems, coords, young, nu, ke, fp32, nullptr, s);
+      break;
+    case kVarIncompat:
+      launch_incompat_c(n_elems, coords, nullptr, young, nu, inc_c, s);
+      eval_ke_var<kVarIncompat>(n_elems, coords, young, nu, ke, fp32, inc_c, s);
+      break;
+    case kVarGauss27:
+      eval_ke_var<kVarGauss27>(n_elems, coords, young, nu, ke, fp32, nullptr, s);
+      break;
+    default:
+      eval_ke_var<kVarFull>(n_elems, coords, young, nu, ke, fp32, nullptr, s);
+  }
   UFC_HIP_CHECK(hipGetLastError());
 }
 
 void launch_assemble(ufc_ctx* c, int precision, int material) {
   const unsigned grid = blocks_for(c->n_nodes, 8);
-  const double* young = c->young.get();
   const double* dgp = c->j2_tangent.get();
+  if (material == UFC_MAT_ELASTIC && c->variant == kVarIncompat)
+    launch_incompat_c(c->n_elems, c->coords.get(), c->conn.get(), c->young.get(), c->nu,
+                      c->inc_c.get(), c->stream);
   if (precision == UFC_PREC_FP64) {
     if (material == UFC_MAT_ELASTIC)
-      hipLaunchKernelGGL((k_assemble<double, UFC_MAT_ELASTIC>), dim3(grid), dim3(64), 0, c->stream,
-                         c->n_nodes, c->coords.get(), c->conn.get(), c->slot_elem.get(),
-                         c->slot_code.get(), c->nb_ptr.get(), young, c->nu, dgp, c->val64.get());
+      assemble_variant<double>(c, c->val64.get());
     else
-      hipLaunchKernelGGL((k_assemble<double, UFC_MAT_J2>), dim3(grid), dim3(64), 0, c->stream,
-                         c->n_nodes, c->coords.get(), c->conn.get(), c->slot_elem.get(),
-                         c->slot_code.get(), c->nb_ptr.get(), young, c->nu, dgp, c->val64.get());
+      hipLaunchKernelGGL((k_assemble<double, UFC_MAT_J2, kVarFull>), dim3(grid), dim3(64), 0,
+                         c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
+                         c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), nullptr, c->nu,
+                         dgp, nullptr, c->val64.get());
   } else {
     if (material == UFC_MAT_ELASTIC)
-      hipLaunchKernelGGL((k_assemble<float, UFC_MAT_ELASTIC>), dim3(grid), dim3(64), 0, c->stream,
-                         c->n_nodes, c->coords.get(), c->conn.get(), c->slot_elem.get(),
-                         c->slot_code.get(), c->nb_ptr.get(), young, c->nu, dgp, c->val32.get());
+      assemble_variant<float>(c, c->val32.get());
     else
-      hipLaunchKernelGGL((k_assemble<float, UFC_MAT_J2>), dim3(grid), dim3(64), 0, c->stream,
-                         c->n_nodes, c->coords.get(), c->conn.get(), c->slot_elem.get(),
-                         c->slot_code.get(), c->nb_ptr.get(), young, c->nu, dgp, c->val32.get());
+      hipLaunchKernelGGL((k_assemble<float, UFC_MAT_J2, kVarFull>), dim3(grid), dim3(64), 0,
+                         c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
+                         c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), nullptr, c->nu,
+                         dgp, nullptr, c->val32.get());
   }
   UFC_HIP_CHECK(hipGetLastError());
 }
@@ -677,7 +1105,9 @@ void launch_penalty(ufc_ctx* c, bool fp32, double sign) {
 void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y) {
   c->elem_w.resize(size_t(c->n_elems) * 24);
   const unsigned grid = blocks_for(c->n_elems, 128);
-  if (mode == 0)
+  if (mode == 0 && c->variant != kVarFull)
+    apply_variant<false>(c, u, nullptr, nullptr, c->elem_w.get());
+  else if (mode == 0)
     hipLaunchKernelGGL(k_elem_force<0>, dim3(grid), dim3(128), 0, c->stream, c->n_elems,
                        c->coords.get(), c->conn.get(), c->young.get(), c->nu, nullptr, nullptr, u,
                        c->elem_w.get());
@@ -697,6 +1127,11 @@ void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y) {
 
 void launch_sensitivity(ufc_ctx* c, const double* u, const double* lam, const double* dedrho,
                         double nu, double* grad) {
+  if (c->variant != kVarFull) {  // the variant and E_e of the last elastic assembly
+    apply_variant<true>(c, u, lam, dedrho, grad);
+    UFC_HIP_CHECK(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_sensitivity, dim3(blocks_for(c->n_elems, 128)), dim3(128), 0, c->stream,
                      c->n_elems, c->coords.get(), c->conn.get(), nu, u, lam, dedrho, grad);
   UFC_HIP_CHECK(hipGetLastError());

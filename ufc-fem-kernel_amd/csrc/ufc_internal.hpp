@@ -86,6 +86,28 @@ constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vecto
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
 
+// C3D8 variants, PH_ELEM_C3D8_VARIANT_* (PH_Elem_C3D8.f90:111-116)
+constexpr int kVarFull = 1, kVarReduced = 2, kVarSelective = 3, kVarIncompat = 4,
+              kVarGauss27 = 6;
+// the element each variant value computes (0 and 5 are the standard element,
+// as PH_Elem_C3D8_StiffMatrixByVariant routes them), -1 if unknown
+inline int canonical_variant(int v) {
+  switch (v) {
+    case 0:
+    case 1:
+    case 5:
+      return kVarFull;
+    case 2:
+    case 3:
+    case 4:
+    case 6:
+      return v;
+    default:
+      return -1;
+  }
+}
+constexpr int kIncompatC = 24 * 9;  // condensation factor per element (doubles)
+
 // ---------------------------------------------------------------- C-ABI status
 inline void set_status(ufc_status* st, int code, const std::string& msg, const char* src) {
   if (!st) return;
@@ -158,6 +180,8 @@ struct ufc_ctx {
   ufc::DBuf<float> kdiag32;  // K(i,i) of the FP32 assembly, no penalty
   bool have64 = false, have32 = false;
   int last_material = UFC_MAT_ELASTIC;
+  int variant = 1;                 // element variant of the last elastic assembly
+  ufc::DBuf<double> inc_c;         // (24, 9, n_elems) incompatible-mode factor
 
   // material of the last elastic assembly
   ufc::DBuf<double> young;
@@ -211,7 +235,11 @@ namespace ufc {
 // ---------------------------------------------------------------- launchers
 // element kernels (ufc_elem.hip)
 void launch_eval_ke(int32_t n_elems, const double* coords, const double* young, double nu,
-                    void* ke, bool fp32, hipStream_t s);
+                    void* ke, bool fp32, int variant, double* inc_c, hipStream_t s);
+// incompatible-mode condensation factor C (Ke = Kuu - C C^T) of every element;
+// conn == nullptr: coords is (8, 3, n_elems) element coordinates
+void launch_incompat_c(int32_t n_elems, const double* coords, const int32_t* conn,
+                       const double* young, double nu, double* inc_c, hipStream_t s);
 void launch_assemble(ufc_ctx* c, int precision, int material);
 void launch_diag(ufc_ctx* c, bool fp32);
 void launch_penalty(ufc_ctx* c, bool fp32, double sign);

@@ -22,6 +22,9 @@ MODULE ufc_hip_iface
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_PREC_FP64 = 0, UFC_PREC_FP32 = 1, &
        UFC_PREC_MIXED = 2
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_MAT_ELASTIC = 0, UFC_MAT_J2 = 1
+  ! C3D8 variants (PH_ELEM_C3D8_VARIANT_*, PH_Elem_C3D8.f90:111-116)
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_VARIANT_STANDARD = 1, UFC_VARIANT_REDUCED = 2, &
+    UFC_VARIANT_SELECTIVE = 3, UFC_VARIANT_INCOMPAT = 4, UFC_VARIANT_27PT = 6
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_LOAD_TRACTION = 0, UFC_LOAD_PRESSURE = 1
 
   TYPE, BIND(C), PUBLIC :: ufc_status
@@ -54,6 +57,8 @@ MODULE ufc_hip_iface
   TYPE, BIND(C), PUBLIC :: ufc_rt_asm_algo
     INTEGER(C_INT32_T) :: precision = UFC_PREC_FP64
     INTEGER(C_INT32_T) :: material = UFC_MAT_ELASTIC
+    INTEGER(C_INT32_T) :: variant = UFC_VARIANT_STANDARD  ! PH_ELEM_C3D8_VARIANT_*
+    INTEGER(C_INT32_T) :: reserved = 0
   END TYPE ufc_rt_asm_algo
 
   TYPE, BIND(C), PUBLIC :: ufc_rt_asm_args
@@ -118,7 +123,7 @@ MODULE ufc_hip_iface
   ! PH_Elem_Eval_Ke_Arg (PH_Elem_Def.f90:218-222) for a batch of C3D8
   TYPE, BIND(C), PUBLIC :: ufc_ph_ke_algo
     INTEGER(C_INT32_T) :: precision = UFC_PREC_FP64
-    INTEGER(C_INT32_T) :: reserved = 0
+    INTEGER(C_INT32_T) :: variant = UFC_VARIANT_STANDARD
   END TYPE ufc_ph_ke_algo
 
   TYPE, BIND(C), PUBLIC :: ufc_ph_ke_args

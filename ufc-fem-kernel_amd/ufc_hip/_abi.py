@@ -29,6 +29,13 @@ PREC_MIXED = 2
 MAT_ELASTIC = 0
 MAT_J2 = 1
 
+# C3D8 variants (PH_ELEM_C3D8_VARIANT_*, PH_Elem_C3D8.f90:111-116)
+VARIANT_STANDARD = 1
+VARIANT_REDUCED = 2
+VARIANT_SELECTIVE = 3
+VARIANT_INCOMPAT = 4
+VARIANT_27PT = 6
+
 LOAD_TRACTION = 0
 LOAD_PRESSURE = 1
 
@@ -51,7 +58,7 @@ class AsmInfo(C.Structure):
 
 
 class KeAlgo(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("precision", C.c_int32), ("variant", C.c_int32)]
 
 
 class KeArgs(C.Structure):
@@ -70,7 +77,8 @@ class J2Props(C.Structure):
 
 
 class AsmAlgo(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("material", C.c_int32)]
+    _fields_ = [("precision", C.c_int32), ("material", C.c_int32), ("variant", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class AsmArgs(C.Structure):

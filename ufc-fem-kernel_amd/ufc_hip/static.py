@@ -170,9 +170,10 @@ class Context:
                    ci.ctypes.data_as(C.c_void_p))
         return rp, ci
 
-    def assemble_k(self, young, nu, precision=A.PREC_FP64, material=A.MAT_ELASTIC):
+    def assemble_k(self, young, nu, precision=A.PREC_FP64, material=A.MAT_ELASTIC,
+                   variant=A.VARIANT_STANDARD):
         arg = _Arg()
-        algo = A.AsmAlgo(precision, material)
+        algo = A.AsmAlgo(precision, material, variant, 0)
         args = A.AsmArgs(arg.ptr(young, np.float64) if young is not None else None, nu)
         self._call("ufc_rt_asm_assemble_k", C.byref(algo), C.byref(args))
 
@@ -227,14 +228,14 @@ class Context:
         return out, norms
 
     # -------------------------------------------------------------- L4 element kernels
-    def eval_ke(self, xe, young, nu, precision=A.PREC_FP64):
-        """Batched Ke: xe (n,8,3), young (n,) -> (n,24,24)."""
+    def eval_ke(self, xe, young, nu, precision=A.PREC_FP64, variant=A.VARIANT_STANDARD):
+        """Batched Ke: xe (n,8,3), young (n,) -> (n,24,24) of the C3D8 variant."""
         xe = np.ascontiguousarray(xe, np.float64)
         young = np.ascontiguousarray(young, np.float64)
         n = xe.shape[0]
         dt = np.float64 if precision == A.PREC_FP64 else np.float32
         ke = np.empty((n, 24, 24), dt)
-        algo = A.KeAlgo(precision, 0)
+        algo = A.KeAlgo(precision, variant)
         args = A.KeArgs(n, 0, xe.ctypes.data_as(C.c_void_p), young.ctypes.data_as(C.c_void_p),
                         nu, ke.ctypes.data_as(C.c_void_p))
         self._call("ufc_ph_c3d8_eval_ke", C.byref(algo), C.byref(args))
