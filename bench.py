@@ -338,6 +338,24 @@ def main():
     ctx.assemble_k(np.ascontiguousarray(pr.young), pr.nu, precision=ufc_hip.PREC_MIXED)
     t2 = time.perf_counter()
     asm["host_young_extra_ms"] = ((t2 - t1) - (t1 - t0)) * 1e3
+    # element variants (UFC_VARIANT_*) through the same fused assembly: mean of
+    # 3 launches after one warm-up, events around the assembly call
+    variants = {}
+    for vname, v in (("standard", 1), ("reduced", 2), ("selective", 3), ("incompat", 4),
+                     ("gauss27", 6)):
+        row = {}
+        for pname, prec in (("fp32_ms", ufc_hip.PREC_MIXED), ("fp64_ms", ufc_hip.PREC_FP64)):
+            ts = []
+            for k in range(4):
+                ctx.assemble_k(pr.young_dev, pr.nu, precision=prec, variant=v)
+                if k:
+                    ts.append(ctx.timing().assemble_ms)
+            row[pname] = float(np.mean(ts))
+        variants[vname] = row
+        log(f"assembly variant {vname}: FP32 {row['fp32_ms']:.2f} ms, FP64 {row['fp64_ms']:.2f} ms")
+    asm["variants"] = variants
+    asm["variants_note"] = ("incompat includes its condensation-factor kernel (one wave per "
+                            "element, 24x9 doubles per element in HBM)")
     ex = measured_assembly_flop(pr.n_elems)
     if ex is not None:
         asm["executed_flop"] = ex

@@ -422,7 +422,9 @@ int ufc_rt_restart_load(ufc_ctx* ctx, const char* path, ufc_rt_restart_data* dat
 /* A static C3D8 step read from an Abaqus-style input deck: replaces the
  * keyword reading of AP_Input_Domain_ParseKeyword (L6_AP/Input/Parser/
  * AP_Inp_Mgr.f90:99-125, keyword dispatch AP_Inp_KWBrg.f90:249) for the
- * keywords of this path.  Read: *NODE, *ELEMENT (TYPE=C3D8), *NSET, *ELSET
+ * keywords of this path.  Read: *NODE, *ELEMENT (TYPE=C3D8, C3D8R, C3D8H,
+ * C3D8I, C3D8M, C3D8S: the PH_ELEM_C3D8_VARIANT_* of PH_Elem_C3D8.f90:110-116,
+ * one per model, returned in variant), *NSET, *ELSET
  * (GENERATE), *SURFACE (TYPE=ELEMENT, S1..S6), *MATERIAL + *ELASTIC,
  * *SOLID SECTION, *BOUNDARY (dof ranges, ENCASTRE, PINNED, X/Y/ZSYMM),
  * *STEP + *STATIC, *CLOAD, *DLOAD (P1..P6), *DSLOAD, one *PART with one
@@ -451,7 +453,7 @@ typedef struct ufc_ap_model {
   const int32_t* cload_eq;   /* (n_cload)                                    */
   const double* cload_val;   /* (n_cload)                                    */
   int32_t n_face;            /* pressure-loaded element faces                */
-  int32_t reserved;
+  int32_t variant;           /* UFC_VARIANT_* of the element type            */
   const int32_t* face_elem;  /* (n_face)                                     */
   const int32_t* face_id;    /* (n_face) 0..5                                */
   const double* face_p;      /* (n_face) pressure                            */

@@ -23,7 +23,10 @@ def say(*a):
 
 def _oracle_solution(orc, d, penalty=1e12):
     rp, ci = orc.csr_pattern(d.coords.shape[0], d.conn)
-    val = orc.assemble(d.conn, d.coords, d.young, d.nu, rp, ci)
+    if d.variant == 1:
+        val = orc.assemble(d.conn, d.coords, d.young, d.nu, rp, ci)
+    else:
+        val = orc.assemble_variant(d.conn, d.coords, d.young, d.nu, d.variant, rp, ci)
     F = np.zeros(d.n_dof)
     for p in np.unique(d.face_p):
         sel = d.face_p == p
@@ -98,3 +101,24 @@ def test_fortran_deck_driver_vs_oracle(orc, tmp_path, prec):
     fields = r.stdout.split()
     assert int(fields[fields.index("faces") + 1]) == d.face_elem.size
     assert int(fields[fields.index("cloads") + 1]) == d.cload_eq.size
+
+
+@pytest.mark.parametrize("etype", ["C3D8R", "C3D8I", "C3D8H"])
+def test_fortran_deck_driver_element_variants(orc, tmp_path, etype):
+    """*ELEMENT, TYPE=C3D8R / C3D8I / C3D8H reaches the assembly as the
+    variant of the deck (ufc_ap_model.variant -> ufc_rt_asm_algo.variant)."""
+    from ufc_hip import inp
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran drivers not built (no Fortran compiler)")
+    path = str(tmp_path / "block.inp")
+    deck_cases.block_deck(path, replace={"type=C3D8": f"type={etype}"})
+    d = inp.read_inp(path)
+    uo = _oracle_solution(orc, d)
+    out = tmp_path / "u.bin"
+    say("fortran deck driver", etype)
+    r = subprocess.run([EXE, path, "fp64", str(out), "1e-12"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    u = np.fromfile(out, dtype=np.float64)
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8

@@ -95,11 +95,23 @@ def test_boundary_dofs_beyond_three_are_ignored(tmp_path):
     assert np.array_equal(d.bc_val, [bc[e] for e in sorted(bc)])
 
 
+@pytest.mark.parametrize("etype,variant", [("C3D8", 1), ("C3D8R", 2), ("C3D8H", 3),
+                                           ("C3D8I", 4), ("C3D8M", 5), ("C3D8S", 6)])
+def test_element_type_sets_the_variant(tmp_path, etype, variant):
+    path = str(tmp_path / "v.inp")
+    want = deck_cases.block_deck(path, replace={"type=C3D8": f"type={etype}"})
+    d = _read(path)
+    assert d.variant == variant
+    _assert_matches(d, want)
+
+
 BAD = [
     ("two steps", {"*End Step": "*End Step\r\n*Step\r\n*Static\r\n*End Step"}, 11, "*STEP"),
     ("two Poisson ratios", {" 3.0e9, 0.3": " 3.0e9, 0.25"}, 11, "Poisson"),
     ("nlgeom", {"nlgeom=NO": "nlgeom=YES"}, 11, "NLGEOM"),
     ("other element", {"type=C3D8": "type=C3D20R"}, 11, "C3D20R"),
+    ("mixed variants", {"*Nset, nset=Bottom": "*Element, type=C3D8R\r\n9001, 101, 102, 106, 105, "
+                        "117, 118, 122, 121\r\n*Nset, nset=Bottom"}, 11, "several C3D8 variants"),
     ("face label", {"Top, S2": "Top, S7"}, 11, "S7"),
     ("traction", {"Blk-1.TopS, P, 1.0e5": "Blk-1.TopS, TRVEC, 1.0, 0, 0, 1"}, 11, "TRVEC"),
     ("moment", {"Blk-1.Tip, 3, -500.": "Blk-1.Tip, 5, -500."}, 11, "moment"),

@@ -340,8 +340,8 @@ __device__ __forceinline__ void load_x(const double* __restrict__ coords,
 // J0^-T dP/dxi detJ0/detJ (Taylor, Beresford & Wilson 1976) in place of the
 // reference's three scalar modes (PH_Elem_C3D8.f90:240-259, which fail the
 // patch test); Kaa = L L^T, C = Kua L^-T.  One wave per element: lanes 0-23
-// form the rows of Kua, lanes 24-32 the rows of Kaa, lane 0 factors Kaa and
-// lanes 0-23 solve their row of C.  A Kaa that is not positive definite
+// form the rows of Kua, lanes 24-32 the rows of Kaa, every lane factors Kaa
+// and lanes 0-23 solve their row of C.  A Kaa that is not positive definite
 // (degenerate element) gives C = 0, i.e. Kuu, as the reference's
 // det(Kaa) ~ 0 branch (:457-460).
 __global__ __launch_bounds__(64) void k_incompat_c(int32_t n_elems,
@@ -351,8 +351,6 @@ __global__ __launch_bounds__(64) void k_incompat_c(int32_t n_elems,
                                                    double* __restrict__ cout) {
   __shared__ double kua[24][9];
   __shared__ double kaa[9][9];
-  __shared__ double lf[9][9];
-  __shared__ int ok;
   const int64_t e = blockIdx.x;
   const int r = threadIdx.x;
   double x[8][3];
@@ -408,33 +406,40 @@ __global__ __launch_bounds__(64) void k_incompat_c(int32_t n_elems,
     for (int k = 0; k < 9; ++k) kaa[r - 24][k] = acc[k];
   }
   __syncthreads();
-  if (r == 0) {
-    int good = c_ok ? 1 : 0;
-    for (int j = 0; j < 9 && good; ++j) {
-      double sj = kaa[j][j];
-      for (int k = 0; k < j; ++k) sj -= lf[j][k] * lf[j][k];
-      if (!(sj > 0.0)) {
-        good = 0;
-        break;
-      }
-      lf[j][j] = sqrt(sj);
-      for (int i2 = j + 1; i2 < 9; ++i2) {
-        double t = kaa[i2][j];
-        for (int k = 0; k < j; ++k) t -= lf[i2][k] * lf[j][k];
-        lf[i2][j] = t / lf[j][j];
-      }
+  // every lane factors Kaa = L L^T in registers (the same arithmetic in each
+  // lane, no LDS round trips in the dependent chain), lanes 0-23 then solve
+  // L w = Kua row r
+  double l[45];  // lower triangle, row-major: l[i (i + 1) / 2 + j]
+  bool good = c_ok;
+#pragma unroll
+  for (int i2 = 0; i2 < 9; ++i2)
+#pragma unroll
+    for (int j = 0; j <= i2; ++j) l[i2 * (i2 + 1) / 2 + j] = kaa[i2][j];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    double d = l[j * (j + 1) / 2 + j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= l[j * (j + 1) / 2 + k] * l[j * (j + 1) / 2 + k];
+    good = good && d > 0.0;
+    d = good ? sqrt(d) : 1.0;
+    l[j * (j + 1) / 2 + j] = d;
+    const double inv = 1.0 / d;
+#pragma unroll
+    for (int i2 = j + 1; i2 < 9; ++i2) {
+      double t = l[i2 * (i2 + 1) / 2 + j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= l[i2 * (i2 + 1) / 2 + k] * l[j * (j + 1) / 2 + k];
+      l[i2 * (i2 + 1) / 2 + j] = t * inv;
     }
-    ok = good;
   }
-  __syncthreads();
   if (r < 24) {
     double w[9];
 #pragma unroll
     for (int i2 = 0; i2 < 9; ++i2) {
       double t = kua[r][i2];
 #pragma unroll
-      for (int k = 0; k < i2; ++k) t -= lf[i2][k] * w[k];
-      w[i2] = ok ? t / lf[i2][i2] : 0.0;
+      for (int k = 0; k < i2; ++k) t -= l[i2 * (i2 + 1) / 2 + k] * w[k];
+      w[i2] = good ? t / l[i2 * (i2 + 1) / 2 + i2] : 0.0;
     }
     double* out = cout + int64_t(kIncompatC) * e + 9 * r;
 #pragma unroll

@@ -140,6 +140,7 @@ struct Model {
   std::vector<int32_t> conn, bc_eq, cload_eq, face_elem, face_id;
   std::vector<double> young, bc_val, cload_val, face_p;
   double nu = 0.0;
+  int variant = 0;
 };
 
 class Reader {
@@ -241,10 +242,16 @@ class Reader {
         raise(UFC_STATUS_UNSUPPORTED, k.line, "more than one *INSTANCE");
       instance_part_[upper(param(k, "NAME"))] = p;
     } else if (n == "ELEMENT") {
-      const std::string t = upper(param(k, "TYPE"));
-      if (t != "C3D8")
+      // C3D8 family -> PH_ELEM_C3D8_VARIANT_* (PH_Elem_C3D8.f90:110-116)
+      static const std::map<std::string, int> kVariant = {
+          {"C3D8", 1}, {"C3D8R", 2}, {"C3D8H", 3}, {"C3D8I", 4}, {"C3D8M", 5}, {"C3D8S", 6}};
+      const auto it = kVariant.find(upper(param(k, "TYPE")));
+      if (it == kVariant.end())
         raise(UFC_STATUS_UNSUPPORTED, k.line,
-              "element type '" + param(k, "TYPE") + "' (this path reads C3D8)");
+              "element type '" + param(k, "TYPE") + "' (this path reads the C3D8 family)");
+      if (m_.variant != 0 && m_.variant != it->second)
+        raise(UFC_STATUS_UNSUPPORTED, k.line, "several C3D8 variants in one model");
+      m_.variant = it->second;
     } else if (n == "MATERIAL") {
       material_ = upper(param(k, "NAME"));
       m_.material[material_] = {0.0, -1.0};
@@ -401,7 +408,7 @@ class Reader {
   void finish() {
     const size_t ne = m_.elem_label.size();
     if (m_.node_label.empty() || ne == 0)
-      throw InpError{UFC_STATUS_INVALID, "the deck defines no C3D8 elements"};
+      throw InpError{UFC_STATUS_INVALID, "the deck defines no C3D8-family elements"};
     m_.conn.resize(m_.conn_label.size());
     for (size_t i = 0; i < m_.conn_label.size(); ++i) {
       auto it = m_.node_of.find(m_.conn_label[i]);
@@ -505,6 +512,7 @@ int ufc_ap_read_inp(const char* path, ufc_ap_model* model, ufc_status* st) {
     model->conn = m->conn.data();
     model->young = m->young.data();
     model->nu = m->nu;
+    model->variant = m->variant;
     model->n_bc = int32_t(m->bc_eq.size());
     model->bc_eq = m->bc_eq.data();
     model->bc_val = m->bc_val.data();

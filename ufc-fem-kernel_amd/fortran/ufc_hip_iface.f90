@@ -173,7 +173,7 @@ MODULE ufc_hip_iface
     TYPE(C_PTR) :: cload_eq = C_NULL_PTR   ! (n_cload)
     TYPE(C_PTR) :: cload_val = C_NULL_PTR
     INTEGER(C_INT32_T) :: n_face = 0
-    INTEGER(C_INT32_T) :: reserved = 0
+    INTEGER(C_INT32_T) :: variant = 0      ! UFC_VARIANT_* of the element type
     TYPE(C_PTR) :: face_elem = C_NULL_PTR  ! (n_face), 0-based
     TYPE(C_PTR) :: face_id = C_NULL_PTR    ! (n_face), 0..5
     TYPE(C_PTR) :: face_p = C_NULL_PTR     ! (n_face)

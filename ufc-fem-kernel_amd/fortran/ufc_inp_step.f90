@@ -73,6 +73,7 @@ PROGRAM ufc_inp_step
 
   aalgo%precision = prec
   aalgo%material = UFC_MAT_ELASTIC
+  aalgo%variant = model%variant            ! C3D8R, C3D8I, ... from *ELEMENT, TYPE=
   aargs%young = model%young
   aargs%nu = model%nu
   rc = ufc_rt_asm_assemble_k(ctx, aalgo, aargs, st)

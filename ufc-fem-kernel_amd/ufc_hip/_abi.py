@@ -121,7 +121,7 @@ class ApModel(C.Structure):
                 ("nu", C.c_double), ("n_bc", C.c_int32), ("n_cload", C.c_int32),
                 ("bc_eq", C.c_void_p), ("bc_val", C.c_void_p),
                 ("cload_eq", C.c_void_p), ("cload_val", C.c_void_p),
-                ("n_face", C.c_int32), ("reserved", C.c_int32),
+                ("n_face", C.c_int32), ("variant", C.c_int32),
                 ("face_elem", C.c_void_p), ("face_id", C.c_void_p), ("face_p", C.c_void_p),
                 ("impl", C.c_void_p)]
 

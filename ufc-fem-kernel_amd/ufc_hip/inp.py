@@ -39,6 +39,7 @@ class Deck:
     face_elem: np.ndarray    # pressure-loaded faces (0-based element index)
     face_id: np.ndarray      # 0..5
     face_p: np.ndarray       # pressure along the outward normal
+    variant: int = 1         # UFC_VARIANT_* of the element type (C3D8R: 2, ...)
 
     @property
     def n_dof(self):
@@ -81,6 +82,7 @@ def read_inp(path: str) -> Deck:
             face_elem=_copy(m.face_elem, m.n_face, i32, np.int32),
             face_id=_copy(m.face_id, m.n_face, i32, np.int32),
             face_p=_copy(m.face_p, m.n_face, f64, np.float64),
+            variant=int(m.variant),
         )
     finally:
         L.ufc_ap_free(C.byref(m), C.byref(st))
@@ -90,7 +92,7 @@ def setup_context(ctx, deck: Deck, penalty=PENALTY):
     """Pattern, F (consistent face pressures, one library call per distinct
     pressure, then the concentrated loads) and the penalty boundary
     conditions of `deck` on `ctx`; assembly is separate
-    (ctx.assemble_k(deck.young, deck.nu, precision))."""
+    (ctx.assemble_k(deck.young, deck.nu, precision, variant=deck.variant))."""
     ctx.build_pattern(deck.coords, deck.conn)
     F = np.zeros(deck.n_dof)
     first = True
@@ -105,7 +107,7 @@ def setup_context(ctx, deck: Deck, penalty=PENALTY):
     ctx.apply_bc(deck.bc_eq, deck.bc_val, penalty)
 
 
-def write_inp(path: str, pr, part="PART-1", instance="PART-1-1") -> None:
+def write_inp(path: str, pr, part="PART-1", instance="PART-1-1", element="C3D8") -> None:
     """Writes problems.Problem `pr` as a deck: one part with one instance,
     nodes and elements labelled from 1, one elastic material per distinct
     Young's modulus, *BOUNDARY on the prescribed equations (with their
@@ -116,7 +118,7 @@ def write_inp(path: str, pr, part="PART-1", instance="PART-1-1") -> None:
     lines = [f"** {pr.name}: {pr.coords.shape[0]} nodes, {pr.conn.shape[0]} C3D8",
              f"*PART, NAME={part}", "*NODE"]
     lines += [f"{i + 1}, {x!r}, {y!r}, {z!r}" for i, (x, y, z) in enumerate(pr.coords.tolist())]
-    lines.append("*ELEMENT, TYPE=C3D8, ELSET=ALL")
+    lines.append(f"*ELEMENT, TYPE={element}, ELSET=ALL")
     lines += [f"{e + 1}, " + ", ".join(str(n + 1) for n in row)
               for e, row in enumerate(pr.conn.tolist())]
     young = np.asarray(pr.young, np.float64)
