@@ -354,6 +354,24 @@ def main():
         variants[vname] = row
         log(f"assembly variant {vname}: FP32 {row['fp32_ms']:.2f} ms, FP64 {row['fp64_ms']:.2f} ms")
     asm["variants"] = variants
+    # y = K^T x through the mirrored blocks (FP64), beside y = K x (device data)
+    ctx.assemble_k(pr.young_dev, pr.nu, precision=ufc_hip.PREC_FP64)
+    xt = torch.randn(pr.n_dof, dtype=torch.float64, device=dev)
+    yt = torch.empty_like(xt)
+    tr = {}
+    for name, fn in (("spmv_fp64_ms", lambda: ctx.spmv(xt, precision=ufc_hip.PREC_FP64, out=yt)),
+                     ("spmv_t_fp64_ms", lambda: ctx.spmv_t(xt, out=yt))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        tr[name] = (time.perf_counter() - t0) / 5 * 1e3
+    tr["note"] = ("wall time per C-ABI call on device vectors (includes the call's own "
+                  "synchronisation); K^T x reads each neighbour block transposed in place")
+    log(f"K x {tr['spmv_fp64_ms']:.2f} ms, K^T x {tr['spmv_t_fp64_ms']:.2f} ms (FP64)")
+    out["transpose"] = tr
     asm["variants_note"] = ("incompat includes its condensation-factor kernel (one wave per "
                             "element, 24x9 doubles per element in HBM)")
     ex = measured_assembly_flop(pr.n_elems)

@@ -268,6 +268,15 @@ typedef struct ufc_rt_load_args {
 } ufc_rt_load_args;
 int ufc_rt_asm_load(ufc_ctx* ctx, const ufc_rt_load_args* args, ufc_status* st);
 
+/* Replaces the FP64 values (nnz, CSR order, as ufc_rt_asm_get_values returns
+ * them, penalties of the current boundary conditions included): a caller's
+ * own operator on the pattern, e.g. a non-symmetric tangent
+ * (RT_CSRMatrix%values, NM_Assem_Sparse.f90:45-56).  Only FP64 solves, the
+ * FP64 SpMVs and ufc_nm_adjoint_solve then apply; MIXED / FP32 solves and
+ * ufc_rt_residual, whose operators come from the elements, return
+ * UFC_STATUS_INVALID until the next ufc_rt_asm_assemble_k. */
+int ufc_rt_asm_set_values(ufc_ctx* ctx, int32_t precision, const double* values, ufc_status* st);
+
 /* Sets / reads the load vector F (n_dof); penalty terms of apply_bc are
  * included when read. */
 int ufc_rt_set_rhs(ufc_ctx* ctx, const double* f, ufc_status* st);
@@ -303,7 +312,12 @@ typedef struct ufc_nm_solver_algo {
                        /* dropped by replace_rtol (<= 0: 0.1), and the inner */
                        /* PCG keeps its search direction across refinement   */
                        /* steps; >= 1: no replacement, each step restarts    */
+  int32_t krylov;      /* ufc_nm_adjoint_solve: UFC_KRYLOV_PCG (K symmetric) */
+                       /* or UFC_KRYLOV_GMRES (any K, FP64)                  */
+  int32_t restart;     /* GMRES restart length (<= 0: 30, the reference's)   */
 } ufc_nm_solver_algo;
+#define UFC_KRYLOV_PCG 0
+#define UFC_KRYLOV_GMRES 1
 
 /* NM_Solver_State (NM_Solv_Def.f90:102-120). */
 typedef struct ufc_nm_solver_state {
@@ -326,12 +340,22 @@ typedef struct ufc_nm_solver_state {
 int ufc_nm_pcg(ufc_ctx* ctx, const ufc_nm_solver_algo* algo, ufc_nm_solver_state* state,
                ufc_status* st);
 
-/* K^T lambda = dJdu with the same Jacobi-PCG (K symmetric, so K^T = K):
- * replaces NM_Adjoint_Solve (NM_Solv_GMRESTranspose.f90:229-291) ->
- * NM_CG_Solve_Transpose (:118-224, unpreconditioned, lambda_0 = 0).
- * state->x receives lambda; state->b is ignored. */
+/* K^T lambda = dJdu, replacing NM_Adjoint_Solve (NM_Solv_GMRESTranspose.f90:
+ * 229-291).  algo->krylov = UFC_KRYLOV_PCG: the same Jacobi-PCG as ufc_nm_pcg
+ * (K symmetric, so K^T = K; the reference's is_symmetric path,
+ * NM_CG_Solve_Transpose, :118-224, unpreconditioned).  UFC_KRYLOV_GMRES: the
+ * is_symmetric = .FALSE. path, GMRES(restart) on K^T with right Jacobi
+ * preconditioning over the FP64 values (algo->precision must be FP64; the
+ * reference's NM_GMRES_Solve_Transpose forms K^T and then runs a damped
+ * Richardson placeholder, :337-393); state->n_outer receives the number of
+ * restarts.  lambda_0 = 0; state->x receives lambda; state->b is ignored. */
 int ufc_nm_adjoint_solve(ufc_ctx* ctx, const ufc_nm_solver_algo* algo, const double* dJdu,
                          ufc_nm_solver_state* state, ufc_status* st);
+
+/* y = K^T x with the FP64 values (precision must be UFC_PREC_FP64), read in
+ * place through the mirrored block of each neighbour: replaces
+ * NM_CSR_Transpose + SpMV (NM_Mtx_SpMVCSRTranspose.f90:46-145). */
+int ufc_nm_spmv_t(ufc_ctx* ctx, int32_t precision, const double* x, double* y, ufc_status* st);
 
 /* Device time in ms of the last assembly and the last solve on this context
  * (HIP events on the context's stream), and the PCG iteration counts. */

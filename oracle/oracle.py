@@ -74,6 +74,8 @@ def lib():
         L.orc_face_loads_traction.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
         L.orc_face_loads_pressure.argtypes = [C.c_int32, vp, vp, vp, vp, C.c_double, vp]
         L.orc_spmv.argtypes = [C.c_int32, vp, vp, vp, vp, vp, C.c_int]
+        L.orc_spmv_t.restype = None
+        L.orc_spmv_t.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
         L.orc_pcg.restype = C.c_int
         L.orc_pcg.argtypes = [C.c_int32, vp, vp, vp, vp, vp, C.c_int32, C.c_double, vp, vp,
                               C.c_int]
@@ -283,6 +285,13 @@ def face_loads_pressure(elem, face, conn, coords, p, F):
 def spmv(rp, ci, val, x, nthreads=1):
     y = np.empty(rp.size - 1)
     lib().orc_spmv(rp.size - 1, _p(rp), _p(ci), _p(val), _p(_f64(x)), _p(y), nthreads)
+    return y
+
+
+def spmv_t(rp, ci, val, x):
+    """y = A^T x (NM_CSR_Transpose + SpMV, NM_Mtx_SpMVCSRTranspose.f90:46-145)."""
+    y = np.empty(rp.size - 1)
+    lib().orc_spmv_t(rp.size - 1, _p(rp), _p(ci), _p(_f64(val)), _p(_f64(x)), _p(y))
     return y
 
 

@@ -825,6 +825,15 @@ void orc_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const d
   }
 }
 
+/* y = A^T x for a square CSR A: the product NM_CSR_Transpose + SpMV forms
+ * (NM_Mtx_SpMVCSRTranspose.f90:46-145), as a scatter over the rows of A */
+void orc_spmv_t(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                const double* x, double* y) {
+  for (int32_t i = 0; i < n; ++i) y[i] = 0.0;
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p) y[col_idx[p]] += val[p] * x[i];
+}
+
 static double dot(int32_t n, const double* a, const double* b, int nthreads) {
   double s = 0.0;
 #ifdef _OPENMP

@@ -72,6 +72,8 @@ void orc_face_loads_pressure(int32_t n_faces, const int32_t* elem, const int32_t
 /* ---- solvers --------------------------------------------------------------- */
 void orc_spmv(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
               const double* x, double* y, int nthreads);
+void orc_spmv_t(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                const double* x, double* y);
 int  orc_pcg(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
              const double* b, double* x, int32_t max_iter, double rtol,
              int32_t* niter, double* rnorm, int nthreads);

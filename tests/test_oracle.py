@@ -438,3 +438,20 @@ def test_j2_newton_converges_quadratically(orc):
     assert max(its) <= 6
     peeq = state.reshape(-1, 13)[:, 6]
     assert peeq.max() > 0  # yielded by 0.3% strain (yield strain 0.125%)
+
+
+def test_spmv_transpose_against_dense(orc):
+    """orc_spmv_t against the dense A^T x on a random non-symmetric CSR, and
+    equal to orc_spmv on a symmetric stiffness matrix."""
+    rng = np.random.default_rng(4)
+    n = 40
+    dense = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.2)
+    rows, cols = np.nonzero(dense)
+    rp, ci, val = orc.csr_from_triplets(n, rows, cols, dense[rows, cols])
+    x = rng.standard_normal(n)
+    assert np.allclose(orc.spmv_t(rp, ci, val, x), dense.T @ x, rtol=0, atol=1e-13)
+    coords, conn = orc.mesh_box(3, 2, 2)
+    rp, ci = orc.csr_pattern(coords.shape[0], conn)
+    K = orc.assemble(conn, coords, np.full(conn.shape[0], 1e9), 0.3, rp, ci)
+    x = rng.standard_normal(rp.size - 1)
+    assert np.abs(orc.spmv_t(rp, ci, K, x) - orc.spmv(rp, ci, K, x)).max() < 1e-12 * 1e9

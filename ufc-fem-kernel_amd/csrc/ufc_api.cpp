@@ -641,6 +641,7 @@ int ufc_rt_asm_assemble_k(ufc_ctx* c, const ufc_rt_asm_algo* algo, const ufc_rt_
       c->have32 = false;
     }
     c->last_material = algo->material;
+    c->user_values = false;
     return UFC_STATUS_OK;
   });
 }
@@ -657,6 +658,26 @@ int ufc_rt_asm_get_values(ufc_ctx* c, int32_t precision, void* values, ufc_statu
       UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
       copy_out(c, static_cast<float*>(values), c->val32.get(), size_t(c->nnz));
     }
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_set_values(ufc_ctx* c, int32_t precision, const double* values,
+                          ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_set_values", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(values != nullptr, UFC_STATUS_INVALID, "values is null");
+    UFC_REQUIRE(precision == UFC_PREC_FP64, UFC_STATUS_UNSUPPORTED, "FP64 values only");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    c->val64.resize(size_t(c->nnz));
+    c->diag64.resize(size_t(c->n_dof));
+    UFC_HIP_CHECK(hipMemcpyAsync(c->val64.get(), values, c->val64.bytes(), hipMemcpyDefault,
+                                 c->stream));
+    ufc::launch_diag(c, false);
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->have64 = true;
+    c->have32 = false;
+    c->user_values = true;
     return UFC_STATUS_OK;
   });
 }
@@ -792,6 +813,8 @@ int ufc_rt_residual(ufc_ctx* c, const double* u, double* r, double* norms, ufc_s
     UFC_REQUIRE(u != nullptr, UFC_STATUS_INVALID, "u is null");
     UFC_REQUIRE(c->have_j2 || c->have_young, UFC_STATUS_INVALID,
                 "no material: assemble or update J2 first");
+    UFC_REQUIRE(!c->user_values, UFC_STATUS_INVALID,
+                "values set by ufc_rt_asm_set_values: the element operator is not known");
     UFC_HIP_CHECK(hipSetDevice(c->device));
     ufc::ensure_solver_buffers(c, true, false);
     ufc::DBuf<double> tu;
@@ -833,6 +856,23 @@ int ufc_nm_spmv(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_s
       copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
     }
     ufc::reset_spmv_events(c);
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_nm_spmv_t(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_status* st) {
+  return guarded(st, "ufc_nm_spmv_t", [&](std::string&) {
+    check_mesh(c);
+    UFC_REQUIRE(x && y, UFC_STATUS_INVALID, "null argument");
+    UFC_REQUIRE(precision == UFC_PREC_FP64, UFC_STATUS_UNSUPPORTED,
+                "K^T x is formed from the FP64 values");
+    UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    ufc::ensure_solver_buffers(c, true, false);
+    ufc::DBuf<double> tx;
+    const double* dx = dev_in(c, x, size_t(c->n_dof), tx);
+    ufc::spmv_t64(c, dx, c->w_t.get());
+    copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
     return UFC_STATUS_OK;
   });
 }
@@ -1004,12 +1044,48 @@ int ufc_nm_pcg(ufc_ctx* c, const ufc_nm_solver_algo* algo, ufc_nm_solver_state* 
   });
 }
 
+static int gmres_adjoint(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* dJdu,
+                         ufc_nm_solver_state* s, std::string& msg) {
+  UFC_REQUIRE(s->x != nullptr, UFC_STATUS_INVALID, "null argument");
+  UFC_REQUIRE(algo->precision == UFC_PREC_FP64, UFC_STATUS_UNSUPPORTED,
+              "GMRES adjoint runs on the FP64 values");
+  UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
+  UFC_REQUIRE(algo->max_iter >= 0, UFC_STATUS_INVALID, "max_iter < 0");
+  UFC_HIP_CHECK(hipSetDevice(c->device));
+  const size_t n = size_t(c->n_dof);
+  ufc::ensure_solver_buffers(c, true, false);
+  UFC_HIP_CHECK(hipMemcpyAsync(c->w_b.get(), dJdu, n * 8, hipMemcpyDefault, c->stream));
+  UFC_HIP_CHECK(hipEventRecord(c->ev2, c->stream));
+  int niter = 0, nrest = 0;
+  double rnorm = 0.0;
+  const int restart = algo->restart > 0 ? algo->restart : 30;
+  const int code = ufc::gmres_t64(c, c->w_b.get(), c->w_x.get(), restart, algo->max_iter,
+                                  algo->rtol, &niter, &nrest, &rnorm, msg);
+  UFC_HIP_CHECK(hipEventRecord(c->ev3, c->stream));
+  UFC_HIP_CHECK(hipMemcpyAsync(s->x, c->w_x.get(), n * 8, hipMemcpyDefault, c->stream));
+  UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  UFC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev2, c->ev3));
+  c->solve_ms = ms;
+  c->spmv_ms_avg = 0.0;
+  c->spmv_calls = 0;
+  s->niter = niter;
+  s->n_outer = nrest;
+  s->rnorm = rnorm;
+  s->converged = code == UFC_STATUS_OK;
+  if (code == UFC_STATUS_OK) msg = "converged";
+  return code;
+}
+
 int ufc_nm_adjoint_solve(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* dJdu,
                          ufc_nm_solver_state* s, ufc_status* st) {
   return guarded(st, "ufc_nm_adjoint_solve", [&](std::string& msg) {
     check_mesh(c);
     UFC_REQUIRE(dJdu != nullptr, UFC_STATUS_INVALID, "dJdu is null");
-    UFC_REQUIRE(s != nullptr, UFC_STATUS_INVALID, "state is null");
+    UFC_REQUIRE(s != nullptr && algo != nullptr, UFC_STATUS_INVALID, "null argument");
+    UFC_REQUIRE(algo->krylov == UFC_KRYLOV_PCG || algo->krylov == UFC_KRYLOV_GMRES,
+                UFC_STATUS_INVALID, "unknown krylov method");
+    if (algo->krylov == UFC_KRYLOV_GMRES) return gmres_adjoint(c, algo, dJdu, s, msg);
     // lambda_0 = 0 as in NM_CG_Solve_Transpose (NM_Solv_GMRESTranspose.f90:163-166)
     const int32_t keep = s->x0_zero;
     s->x0_zero = 1;

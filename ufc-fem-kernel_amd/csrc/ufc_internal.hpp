@@ -181,6 +181,9 @@ struct ufc_ctx {
   bool have64 = false, have32 = false;
   int last_material = UFC_MAT_ELASTIC;
   int variant = 1;                 // element variant of the last elastic assembly
+  bool user_values = false;        // FP64 values from ufc_rt_asm_set_values
+  ufc::DBuf<int32_t> nb_mirror;    // per block (n, k): slot of n in node nb_idx's list
+  bool have_mirror = false;
   ufc::DBuf<double> inc_c;         // (24, 9, n_elems) incompatible-mode factor
 
   // material of the last elastic assembly
@@ -277,5 +280,10 @@ void build_col_idx(ufc_ctx* c);
 double spmv_event_avg_ms(ufc_ctx* c, int* calls);
 void reset_spmv_events(ufc_ctx* c);
 void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32);
+// transpose operator and GMRES on K^T (ufc_gmres.hip)
+void ensure_mirror(ufc_ctx* c);
+void spmv_t64(ufc_ctx* c, const double* x, double* y);
+int gmres_t64(ufc_ctx* c, const double* g, double* lam, int restart, int max_iter, double rtol,
+              int* niter, int* n_restart, double* rnorm, std::string& msg);
 
 }  // namespace ufc

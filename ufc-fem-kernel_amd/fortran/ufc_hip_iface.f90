@@ -26,6 +26,7 @@ MODULE ufc_hip_iface
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_VARIANT_STANDARD = 1, UFC_VARIANT_REDUCED = 2, &
     UFC_VARIANT_SELECTIVE = 3, UFC_VARIANT_INCOMPAT = 4, UFC_VARIANT_27PT = 6
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_LOAD_TRACTION = 0, UFC_LOAD_PRESSURE = 1
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_KRYLOV_PCG = 0, UFC_KRYLOV_GMRES = 1
 
   TYPE, BIND(C), PUBLIC :: ufc_status
     INTEGER(C_INT32_T) :: status_code = 0
@@ -93,6 +94,8 @@ MODULE ufc_hip_iface
     INTEGER(C_INT32_T) :: max_outer = 20
     INTEGER(C_INT32_T) :: check_every = 0
     REAL(C_DOUBLE) :: replace_rtol = 0.0_C_DOUBLE
+    INTEGER(C_INT32_T) :: krylov = UFC_KRYLOV_PCG   ! adjoint: PCG (K symmetric) or GMRES
+    INTEGER(C_INT32_T) :: restart = 30              ! GMRES restart length
   END TYPE ufc_nm_solver_algo
 
   TYPE, BIND(C), PUBLIC :: ufc_nm_solver_state
@@ -187,7 +190,7 @@ MODULE ufc_hip_iface
             ufc_rt_get_rhs, ufc_rt_residual, ufc_nm_spmv, ufc_nm_pcg, ufc_nm_adjoint_solve, &
             ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_ph_j2_get_state, &
             ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_ap_read_inp, ufc_ap_free, &
-            ufc_rt_restart_save, ufc_rt_restart_load, &
+            ufc_rt_restart_save, ufc_rt_restart_load, ufc_nm_spmv_t, ufc_rt_asm_set_values, &
             ufc_status_message
 
   INTERFACE
@@ -269,6 +272,23 @@ MODULE ufc_hip_iface
       TYPE(C_PTR), VALUE :: ctx
       INTEGER(C_INT32_T), VALUE :: precision
       TYPE(C_PTR), VALUE :: x, y
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_nm_spmv_t(ctx, precision, x, y, st) BIND(C, NAME='ufc_nm_spmv_t')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: precision
+      TYPE(C_PTR), VALUE :: x, y
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_set_values(ctx, precision, values, st) &
+        BIND(C, NAME='ufc_rt_asm_set_values')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: precision
+      TYPE(C_PTR), VALUE :: values
       TYPE(ufc_status), INTENT(INOUT) :: st
     END FUNCTION
 

@@ -26,6 +26,9 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_MIXED = 2
 
+KRYLOV_PCG = 0
+KRYLOV_GMRES = 1
+
 MAT_ELASTIC = 0
 MAT_J2 = 1
 
@@ -99,7 +102,8 @@ class LoadArgs(C.Structure):
 class SolverAlgo(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_iter", C.c_int32), ("rtol", C.c_double),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int32),
-                ("check_every", C.c_int32), ("replace_rtol", C.c_double)]
+                ("check_every", C.c_int32), ("replace_rtol", C.c_double),
+                ("krylov", C.c_int32), ("restart", C.c_int32)]
 
 
 class SolverState(C.Structure):
@@ -157,6 +161,8 @@ _SIGS = {
     "ufc_rt_residual": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_nm_spmv": [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_nm_pcg": [C.c_void_p, P(SolverAlgo), P(SolverState), P(Status)],
+    "ufc_nm_spmv_t": [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
+    "ufc_rt_asm_set_values": [C.c_void_p, C.c_int32, C.c_void_p, P(Status)],
     "ufc_nm_adjoint_solve": [C.c_void_p, P(SolverAlgo), C.c_void_p, P(SolverState), P(Status)],
     "ufc_mesh_box": [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                      C.c_double, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],

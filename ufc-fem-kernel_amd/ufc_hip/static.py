@@ -288,7 +288,7 @@ class Context:
         return t
 
     def _solve(self, fname, extra, precision, rtol, max_iter, inner_rtol, max_outer, x, b,
-               x0, check_every, replace_rtol):
+               x0, check_every, replace_rtol, krylov=A.KRYLOV_PCG, restart=30):
         arg = _Arg()
         out = np.empty(self.n_dof) if x is None else x
         xp = arg.out(out, np.float64)
@@ -301,7 +301,7 @@ class Context:
                 out[...] = np.asarray(x0)
             x0_zero = 0
         algo = A.SolverAlgo(precision, max_iter, rtol, inner_rtol, max_outer, check_every,
-                            replace_rtol)
+                            replace_rtol, krylov, restart)
         s = A.SolverState(xp, arg.ptr(b, np.float64) if b is not None else None, 0, 0, 0.0, 0,
                           x0_zero)
         args = [C.byref(algo)] + ([arg.ptr(extra, np.float64)] if extra is not None else []) + \
@@ -322,9 +322,26 @@ class Context:
                            max_outer, x, b, x0, check_every, replace_rtol)
 
     def adjoint_solve(self, dJdu, precision=A.PREC_FP64, rtol=1e-10, max_iter=100000,
-                      inner_rtol=1e-5, max_outer=20, x=None, check_every=0, replace_rtol=0.0):
+                      inner_rtol=1e-5, max_outer=20, x=None, check_every=0, replace_rtol=0.0,
+                      krylov=A.KRYLOV_PCG, restart=30):
+        """K^T lambda = dJdu: Jacobi-PCG (K symmetric) or, krylov=KRYLOV_GMRES,
+        GMRES(restart) on K^T over the FP64 values (info.n_outer = restarts)."""
         return self._solve("ufc_nm_adjoint_solve", dJdu, precision, rtol, max_iter, inner_rtol,
-                           max_outer, x, None, None, check_every, replace_rtol)
+                           max_outer, x, None, None, check_every, replace_rtol, krylov, restart)
+
+    def spmv_t(self, x, out=None):
+        """y = K^T x with the FP64 values."""
+        arg = _Arg()
+        out = np.empty(self.n_dof) if out is None else out
+        self._call("ufc_nm_spmv_t", A.PREC_FP64, arg.ptr(x, np.float64),
+                   arg.out(out, np.float64))
+        return out
+
+    def set_values(self, values):
+        """Replace the FP64 values (CSR order, penalties included, as
+        get_values(PREC_FP64) returns them)."""
+        arg = _Arg()
+        self._call("ufc_rt_asm_set_values", A.PREC_FP64, arg.ptr(values, np.float64))
 
     def info(self):
         i = A.AsmInfo()
