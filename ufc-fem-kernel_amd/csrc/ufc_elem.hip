@@ -351,50 +351,57 @@ __global__ __launch_bounds__(64) void k_incompat_c(int32_t n_elems,
                                                    double* __restrict__ cout) {
   __shared__ double kua[24][9];
   __shared__ double kaa[9][9];
+  __shared__ double pg[8][11][3];  // per Gauss point: 8 node + 3 mode gradients
+  __shared__ double pdj[8];        // detJ, 0 for a skipped point
   const int64_t e = blockIdx.x;
   const int r = threadIdx.x;
-  double x[8][3];
-  load_x(coords, conn, e, x);
   const double E = young[e];
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   const double mu = E / (2.0 * (1.0 + nu));
-  double g0[8][3], dj0, J0i[3][3];
-  const bool c_ok = c3d8_dndx_at(x, 0.0, 0.0, 0.0, g0, dj0, J0i);
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (r < 33 && c_ok) {
-#pragma unroll
-    for (int gp = 0; gp < 8; ++gp) {
+  // lanes 0-7: one Gauss point each (gradients of the nodes and of the modes)
+  bool c_ok = true;
+  {
+    double x[8][3];
+    load_x(coords, conn, e, x);
+    double g0[8][3], dj0, J0i[3][3];
+    c_ok = c3d8_dndx_at(x, 0.0, 0.0, 0.0, g0, dj0, J0i);
+    if (r < 8) {
+      const int gp = r;
       const double xc[3] = {(gp & 4) ? kGaussPt : -kGaussPt, (gp & 2) ? kGaussPt : -kGaussPt,
                             (gp & 1) ? kGaussPt : -kGaussPt};
       double g[8][3], dj;
-      if (!c3d8_dndx(x, gp, g, dj)) continue;
-      double ge[3][3];
+      const bool ok = c3d8_dndx(x, gp, g, dj) && c_ok;
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) pg[gp][a][j] = ok ? g[a][j] : 0.0;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) ge[k][j] = J0i[k][j] * (-2.0 * xc[k]) * dj0 / dj;
-      // this lane's row: pseudo-node gradient gr, component i
-      double gr[3];
-      int i;
-      if (r < 24) {
-        UFC_SELECT_ROW(g, r / 3, gr);
-        i = r % 3;
-      } else {
-        const int k = (r - 24) / 3;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) gr[q] = k == 0 ? ge[0][q] : (k == 1 ? ge[1][q] : ge[2][q]);
-        i = (r - 24) % 3;
-      }
+        for (int j = 0; j < 3; ++j)
+          pg[gp][8 + k][j] = ok ? J0i[k][j] * (-2.0 * xc[k]) * dj0 / dj : 0.0;
+      pdj[gp] = ok ? dj : 0.0;
+    }
+  }
+  __syncthreads();
+  // lanes 0-23: row (node r/3, component r%3) of Kua; lanes 24-32: row
+  // (mode (r-24)/3, component (r-24)%3) of Kaa; columns (mode q, component j)
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < 33) {
+    const int pn = r < 24 ? r / 3 : 8 + (r - 24) / 3;
+    const int i = r < 24 ? r % 3 : (r - 24) % 3;
+    for (int gp = 0; gp < 8; ++gp) {
+      const double dj = pdj[gp];
+      const double gr0 = pg[gp][pn][0], gr1 = pg[gp][pn][1], gr2 = pg[gp][pn][2];
+      const double gri = i == 0 ? gr0 : (i == 1 ? gr1 : gr2);
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const double dd = mu * (gr[0] * ge[q][0] + gr[1] * ge[q][1] + gr[2] * ge[q][2]);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const double gri = i == 0 ? gr[0] : (i == 1 ? gr[1] : gr[2]);
-          const double grj = gr[j];
-          const double gqi = i == 0 ? ge[q][0] : (i == 1 ? ge[q][1] : ge[q][2]);
-          acc[3 * q + j] += dj * (lam * gri * ge[q][j] + mu * grj * gqi + (i == j ? dd : 0.0));
-        }
+        const double gq0 = pg[gp][8 + q][0], gq1 = pg[gp][8 + q][1], gq2 = pg[gp][8 + q][2];
+        const double gqi = i == 0 ? gq0 : (i == 1 ? gq1 : gq2);
+        const double dd = mu * (gr0 * gq0 + gr1 * gq1 + gr2 * gq2);
+        acc[3 * q + 0] += dj * (lam * gri * gq0 + mu * gr0 * gqi + (i == 0 ? dd : 0.0));
+        acc[3 * q + 1] += dj * (lam * gri * gq1 + mu * gr1 * gqi + (i == 1 ? dd : 0.0));
+        acc[3 * q + 2] += dj * (lam * gri * gq2 + mu * gr2 * gqi + (i == 2 ? dd : 0.0));
       }
     }
   }
