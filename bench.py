@@ -204,8 +204,8 @@ def main():
                          "precision plus a 2 GiB device copy for calibration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "sharding of this path is described in DESIGN.md and not measured.",
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the library drives one "
+              "GPU per context and a multi-GPU split of the mesh is not implemented or measured.",
               file=sys.stderr)
         sys.exit(2)
 

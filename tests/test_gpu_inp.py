@@ -96,7 +96,7 @@ def test_fortran_deck_driver_vs_oracle(orc, tmp_path, prec):
     u = np.fromfile(out, dtype=np.float64)
     assert u.size == d.n_dof
     # ||b|| is dominated by the penalty terms of the prescribed tip displacement,
-    # so the driver's default 1e-10 leaves ~3e-8 here: run it to 1e-12
+    # so ufc_inp_step's default 1e-10 leaves ~3e-8 here: run it to 1e-12
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
     fields = r.stdout.split()
     assert int(fields[fields.index("faces") + 1]) == d.face_elem.size

@@ -1,4 +1,4 @@
-"""J2 plasticity (config-5 form, small mesh): the GPU Newton loop over the
+"""J2 plasticity (jittered cube, prescribed top displacement, small mesh): the GPU Newton loop over the
 C-ABI against the oracle's Newton loop, increment by increment; FP64 and
 FP32-in-FP64 inner solves converge to the same state."""
 import numpy as np
