@@ -40,6 +40,9 @@ def test_ke_batch_vs_oracle(ctx, orc, variant):
 
 def _problem(kind="jitter5"):
     from ufc_hip import problems as P
+    if kind == "irregular":  # notched, renumbered: node degrees 1..8, ragged rows
+        from test_gpu_parity import _irregular
+        return _irregular()
     pr = P.cube_pressure(5, name=kind, jitter=0.1)
     pr.young = np.random.default_rng(7).uniform(0.5e9, 2e9, pr.n_elems)
     return pr
@@ -55,12 +58,13 @@ def _oracle_system(orc, pr, variant, penalty=1e12):
     return rp, ci, k_only, val, F
 
 
+@pytest.mark.parametrize("mesh", ["jitter5", "irregular"])
 @pytest.mark.parametrize("variant", VARIANTS, ids=[NAMES[v] for v in VARIANTS])
-def test_assembly_vs_oracle(ctx, orc, variant):
+def test_assembly_vs_oracle(ctx, orc, variant, mesh):
     from ufc_hip import problems as P
-    pr = _problem()
+    pr = _problem(mesh)
     rp, ci, k_only, val, F = _oracle_system(orc, pr, variant)
-    say("assembly", NAMES[variant], "dof", pr.n_dof)
+    say("assembly", NAMES[variant], mesh, "dof", pr.n_dof)
     P.setup_context(ctx, pr, 0)
     ctx.assemble_k(pr.young, pr.nu, precision=0, variant=variant)
     v64 = ctx.get_values(0)
@@ -74,15 +78,16 @@ def test_assembly_vs_oracle(ctx, orc, variant):
     assert np.abs(rowsum).max() <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("mesh", ["jitter5", "irregular"])
 @pytest.mark.parametrize("prec", [0, 2], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("variant", VARIANTS, ids=[NAMES[v] for v in VARIANTS])
-def test_solve_vs_oracle(ctx, orc, variant, prec):
+def test_solve_vs_oracle(ctx, orc, variant, prec, mesh):
     from ufc_hip import problems as P
-    pr = _problem()
+    pr = _problem(mesh)
     rp, ci, _, val, F = _oracle_system(orc, pr, variant)
     uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
     assert st == 0
-    say("solve", NAMES[variant], "precision", prec)
+    say("solve", NAMES[variant], mesh, "precision", prec)
     P.setup_context(ctx, pr, prec)
     ctx.assemble_k(pr.young, pr.nu, precision=prec, variant=variant)
     u, info = ctx.pcg(precision=prec, rtol=1e-12, max_iter=200000, replace_rtol=0.1)
