@@ -13,9 +13,11 @@ OUT   := $(HERE)_ref
 SRCS := $(REF)/L1_IF/Error/IF_Err_Def.f90 \
         $(REF)/L1_IF/Error/IF_Err_Brg.f90 \
         $(REF)/L1_IF/Precision/IF_Prec_Core.f90 \
-        $(REF)/L2_NM/Matrix/NM_Mtx_Sparse.f90
+        $(REF)/L2_NM/Matrix/NM_Mtx_Sparse.f90 \
+        $(REF)/L2_NM/Solver/NM_Mtx_SpMVCSRTranspose.f90
 
-OBJS := $(OUT)/IF_Err_Def.o $(OUT)/IF_Err_Brg.o $(OUT)/IF_Prec_Core.o $(OUT)/NM_Mtx_Sparse.o
+OBJS := $(OUT)/IF_Err_Def.o $(OUT)/IF_Err_Brg.o $(OUT)/IF_Prec_Core.o $(OUT)/NM_Mtx_Sparse.o \
+        $(OUT)/NM_Mtx_SpMVCSRTranspose.o
 
 all: $(OUT)/ref_golden
 
@@ -26,6 +28,9 @@ $(OUT)/IF_Err_Brg.o: $(REF)/L1_IF/Error/IF_Err_Brg.f90 $(OUT)/IF_Err_Def.o
 $(OUT)/IF_Prec_Core.o: $(REF)/L1_IF/Precision/IF_Prec_Core.f90 $(OUT)/IF_Err_Brg.o
 	$(FC) $(FFLAGS) -J $(OUT) -c $< -o $@
 $(OUT)/NM_Mtx_Sparse.o: $(REF)/L2_NM/Matrix/NM_Mtx_Sparse.f90 $(OUT)/IF_Prec_Core.o
+	$(FC) $(FFLAGS) -J $(OUT) -c $< -o $@
+
+$(OUT)/NM_Mtx_SpMVCSRTranspose.o: $(REF)/L2_NM/Solver/NM_Mtx_SpMVCSRTranspose.f90 $(OUT)/IF_Prec_Core.o
 	$(FC) $(FFLAGS) -J $(OUT) -c $< -o $@
 
 $(OUT)/ref_golden: $(HERE)ref_golden.f90 $(OBJS)

@@ -12,19 +12,24 @@
 !   3. assembles with NM_CSR_AssembleFromElements (NM_Mtx_Sparse.f90:1044-1091),
 !   4. applies NM_CSR_MatVec_Optimized (NM_Mtx_Sparse.f90:341-365) to
 !      x(i) = 1 + MOD(5*i, 13) (integers: the product is exact),
-! and writes n, nnz, ia, ja, a and y as text to the file named on the command
-! line.  Usage: ref_golden nx ny nz out.txt
+!   5. transposes the (non-symmetric) matrix with NM_CSR_Transpose
+!      (NM_Mtx_SpMVCSRTranspose.f90:46-145) and applies the transposed rows to
+!      the same x: yt = A^T x,
+! and writes n, nnz, ia, ja, a, y and yt as text to the file named on the
+! command line.  Usage: ref_golden nx ny nz out.txt
 PROGRAM ref_golden
   USE IF_Prec_Core, ONLY: wp, i4
   USE IF_Err_Brg, ONLY: ErrorStatusType
   USE NM_Mtx_Sparse, ONLY: NM_CSR_Type, NM_CSR_AssembleFromElements, NM_CSR_MatVec_Optimized
+  USE NM_Mtx_SpMVCSRTranspose, ONLY: NM_SparseMatrix_CSR, NM_CSR_Transpose
   IMPLICIT NONE
   INTEGER(i4) :: nx, ny, nz, nn, ne, ndof, i, j, k, e, a, d, n0, sx, sy, u
   INTEGER(i4) :: nodes(8)
   CHARACTER(len=256) :: arg, outfile
-  REAL(wp), ALLOCATABLE :: elem_K(:,:,:), x(:), y(:)
+  REAL(wp), ALLOCATABLE :: elem_K(:,:,:), x(:), y(:), yt(:)
   INTEGER(i4), ALLOCATABLE :: elem_dofs(:,:)
   TYPE(NM_CSR_Type) :: Kc
+  TYPE(NM_SparseMatrix_CSR) :: As, At
   TYPE(ErrorStatusType) :: st
 
   CALL GET_COMMAND_ARGUMENT(1, arg); READ(arg, *) nx
@@ -68,6 +73,22 @@ PROGRAM ref_golden
   END DO
   CALL NM_CSR_MatVec_Optimized(Kc, x, y, st)
 
+  As%nrows = Kc%n
+  As%ncols = Kc%n
+  As%nnz = Kc%nnz
+  ALLOCATE(As%row_ptr(Kc%n + 1), As%col_ind(Kc%nnz), As%values(Kc%nnz))
+  As%row_ptr = Kc%ia(1:Kc%n + 1)
+  As%col_ind = Kc%ja(1:Kc%nnz)
+  As%values = Kc%a(1:Kc%nnz)
+  CALL NM_CSR_Transpose(As, At, st)
+  ALLOCATE(yt(ndof))
+  DO i = 1, At%nrows
+    yt(i) = 0.0_wp
+    DO k = At%row_ptr(i), At%row_ptr(i + 1) - 1
+      yt(i) = yt(i) + At%values(k) * x(At%col_ind(k))
+    END DO
+  END DO
+
   OPEN(NEWUNIT=u, FILE=TRIM(outfile), STATUS='REPLACE', ACTION='WRITE')
   WRITE(u, '(I0,1X,I0)') Kc%n, Kc%nnz
   DO i = 1, Kc%n + 1
@@ -78,6 +99,9 @@ PROGRAM ref_golden
   END DO
   DO i = 1, Kc%n
     WRITE(u, '(F0.1)') y(i)
+  END DO
+  DO i = 1, Kc%n
+    WRITE(u, '(F0.1)') yt(i)
   END DO
   CLOSE(u)
 END PROGRAM ref_golden

@@ -10,6 +10,7 @@ container, and stores what it printed:
   a      : values for the integer element matrices Ke(i,j) = 1 + (7e+3i+j) mod 11
            (1-based e, i, j), exact in FP64
   y      : NM_CSR_MatVec_Optimized(A, x) with x(i) = 1 + (5i) mod 13 (1-based i)
+  yt     : A^T x with A^T from NM_CSR_Transpose (NM_Mtx_SpMVCSRTranspose.f90:46-145)
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -40,8 +41,9 @@ def run(nx, ny, nz):
                 ja[k] = int(s[0])
                 a[k] = float(s[1])
             y = np.array([float(f.readline()) for _ in range(n)], np.float64)
+            yt = np.array([float(f.readline()) for _ in range(n)], np.float64)
     np.savez_compressed(os.path.join(HERE, f"ref_csr_{nx}x{ny}x{nz}.npz"),
-                        ia=ia, ja=ja, a=a, y=y, mesh=np.array([nx, ny, nz], np.int32))
+                        ia=ia, ja=ja, a=a, y=y, yt=yt, mesh=np.array([nx, ny, nz], np.int32))
     return n, nnz
 
 

@@ -110,3 +110,24 @@ def test_set_values_round_trip_and_guards(ctx):
     assert info.status == STATUS_INVALID and not info.converged and info.niter == 3
     ctx.assemble_k(pr.young, pr.nu, precision=0)
     ctx.residual(u1)  # the element operator is back
+
+
+@pytest.mark.parametrize("mesh", ["1x1x1", "2x1x1", "3x2x2", "20x4x4"])
+def test_spmv_and_spmv_t_match_reference_golden(ctx, golden_dir, mesh):
+    """The golden matrix of the reference's own assembler (non-symmetric
+    integer element matrices) set on the GPU pattern: K x equals the
+    reference's NM_CSR_MatVec_Optimized and K^T x its NM_CSR_Transpose
+    product, bit for bit (integer arithmetic, exact in FP64)."""
+    import os
+    import ufc_hip
+    g = np.load(os.path.join(golden_dir, f"ref_csr_{mesh}.npz"))
+    nx, ny, nz = (int(v) for v in g["mesh"])
+    coords, conn = ufc_hip.mesh_box(nx, ny, nz, index_base=1)
+    say("golden K^T x", mesh)
+    ctx.build_pattern(coords, conn, index_base=1)
+    rp, ci = ctx.get_pattern(index_base=1)
+    assert np.array_equal(rp, g["ia"]) and np.array_equal(ci, g["ja"])
+    ctx.set_values(g["a"])
+    x = 1.0 + (5 * np.arange(1, rp.size)) % 13
+    assert np.array_equal(ctx.spmv(x, precision=0), g["y"])
+    assert np.array_equal(ctx.spmv_t(x), g["yt"])
