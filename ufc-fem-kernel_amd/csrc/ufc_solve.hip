@@ -179,6 +179,84 @@ __device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __res
   for (int d = 0; d < 3; ++d) a[d] = sum_group32(a[d]);
 }
 
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// FP32 node-block view, difference form, for the two consecutive nodes n0
+// (lanes 0-31) and n0 + 1 (lanes 32-63) of one wave; n0 is wave-uniform.
+// Three things keep the texture path to five vector loads per wave and node
+// pair (three value runs, the neighbour ids, one x gather):
+//   - nb_ptr[n0 .. n0+2] are scalar loads (the wave's two nodes are adjacent);
+//   - x_n is not gathered a second time: the lane whose neighbour is the node
+//     itself (every node is its own neighbour) has it, and readlane
+//     broadcasts it to its group;
+//   - after the DPP sums inside each 16-lane row, v_permlane16_swap pairs
+//     rows 0/1 and 2/3, so every lane of a group holds the same two row sums
+//     the xor-16 shuffle gave (same operands, same order, same result).
+// Row sums of the group's node in a[0..2]; the caller writes lanes k < 3.
+template <bool OFF16>
+__device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, int lane,
+                                                 const int32_t* __restrict__ nb_ptr,
+                                                 const int32_t* __restrict__ nb_idx,
+                                                 const int16_t* __restrict__ nb_off,
+                                                 const float* __restrict__ val,
+                                                 const float* __restrict__ x, float (&a)[3]) {
+  const bool hi = lane >= GROUP;
+  const int k = lane & (GROUP - 1);
+  const bool has1 = n0 + 1 < n_nodes;
+  const int32_t p0 = nb_ptr[n0], p1 = nb_ptr[n0 + 1];
+  const int32_t p2 = has1 ? nb_ptr[n0 + 2] : p1;
+  const int64_t n = n0 + (hi ? 1 : 0);
+  const int32_t q0 = hi ? p1 : p0;
+  const int deg = (hi ? p2 : p1) - q0;
+  float v[9], xm[3] = {0.f, 0.f, 0.f};
+  bool self = false;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) v[j] = 0.f;
+  if (k < deg) {
+    // issue order: neighbour id, then the 9 values (they depend only on
+    // nb_ptr), then -- behind a scheduling barrier, so the compiler cannot
+    // sink the value loads below the wait for the id -- the x gather
+    const int64_t m = OFF16 ? n + int64_t(nb_off[q0 + k]) : int64_t(nb_idx[q0 + k]);
+    const int64_t base = 9 * int64_t(q0) + 3 * k;
+    const int d3 = 3 * deg;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[3 * d + c] = val[base + d * d3 + c];
+    __builtin_amdgcn_sched_barrier(0);
+    self = m == n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xm[c] = x[3 * m + c];
+  }
+  const uint64_t smask = __ballot(self);
+  const uint32_t m_lo = uint32_t(smask), m_hi = uint32_t(smask >> 32);
+  const int l0 = m_lo ? __builtin_ctz(m_lo) : 0;
+  const int l1 = m_hi ? __builtin_ctz(m_hi) + GROUP : GROUP;
+  const float s = self ? 0.f : 1.f;
+  float xd[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    // both broadcasts in every lane, then a select (a divergent readlane
+    // would become two exec-masked branches)
+    const float xn0 = lane_value(xm[c], l0);
+    const float xn1 = lane_value(xm[c], l1);
+    xd[c] = xm[c] - s * (hi ? xn1 : xn0);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float t = v[3 * d] * xd[0] + v[3 * d + 1] * xd[1] + v[3 * d + 2] * xd[2];
+    t += dpp_f32<0xB1>(t);
+    t += dpp_f32<0x4E>(t);
+    t += dpp_f32<0x141>(t);
+    t += dpp_f32<0x140>(t);
+    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(t), __float_as_uint(t),
+                                                     false, false);
+    a[d] = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+  }
+}
+
 // y = K x; with sc != null an early return once the solve is done.  OFF16:
 // the FP32 view's neighbours come as 16-bit offsets m - n (2 bytes instead of
 // 4 per neighbour, measured 0.456 -> 0.440 ms at 128^3)
@@ -190,21 +268,40 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
     const PcgScalars* __restrict__ sc) {
   if (sc && sc->done) return;
   // A workgroup takes 2 x GROUPS_PER_BLOCK consecutive nodes per pass (group
-  // g: nodes g and g + GROUPS_PER_BLOCK), so the y it writes is whole 128-byte
-  // lines and no line is shared with a workgroup on another XCD (measured
-  // 0.443 -> 0.433 ms for the FP32 SpMV at 128^3)
+  // g: nodes g and g + GROUPS_PER_BLOCK; wave w: nodes 2w, 2w + 1 and
+  // 2w + GROUPS_PER_BLOCK, 2w + GROUPS_PER_BLOCK + 1), so the y it writes is
+  // whole 128-byte lines and no line is shared with a workgroup on another XCD
+  // (measured 0.443 -> 0.433 ms for the FP32 SpMV at 128^3)
   constexpr int kNodesPerPass = 2 * GROUPS_PER_BLOCK;
-  const int k = threadIdx.x & (GROUP - 1);
-  const int gi = threadIdx.x / GROUP;
   const int64_t stride = int64_t(gridDim.x) * kNodesPerPass;
-  for (int64_t b = int64_t(blockIdx.x) * kNodesPerPass; b < n_nodes; b += stride) {
+  if constexpr (BLOCK_INDEX) {
+    static_assert(sizeof(T) == 4, "the node-block view is the FP32 operator");
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int k = lane & (GROUP - 1);
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / WAVE);
+    for (int64_t b = int64_t(blockIdx.x) * kNodesPerPass; b < n_nodes; b += stride) {
 #pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
-      const int64_t n = b + gi + h * GROUPS_PER_BLOCK;
-      if (n >= n_nodes) break;
-      T a[3];
-      node_rows<T, BLOCK_INDEX, OFF16>(n, k, nb_ptr, nb_idx, nb_off, col_idx, val, x, a);
-      if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+      for (int h = 0; h < 2; ++h) {
+        const int64_t n0 = b + 2 * w + h * GROUPS_PER_BLOCK;
+        if (n0 >= n_nodes) break;
+        float a[3];
+        node_pair_rows32<OFF16>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, val, x, a);
+        const int64_t n = n0 + (lane >= GROUP ? 1 : 0);
+        if (k < 3 && n < n_nodes) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+      }
+    }
+  } else {
+    const int k = threadIdx.x & (GROUP - 1);
+    const int gi = threadIdx.x / GROUP;
+    for (int64_t b = int64_t(blockIdx.x) * kNodesPerPass; b < n_nodes; b += stride) {
+#pragma unroll 1
+      for (int h = 0; h < 2; ++h) {
+        const int64_t n = b + gi + h * GROUPS_PER_BLOCK;
+        if (n >= n_nodes) break;
+        T a[3];
+        node_rows<T, BLOCK_INDEX, OFF16>(n, k, nb_ptr, nb_idx, nb_off, col_idx, val, x, a);
+        if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+      }
     }
   }
 }
