@@ -509,7 +509,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     c->h_bc_rhs.clear();
     c->coords.resize(size_t(nn) * 3);
     c->conn.resize(size_t(ne) * 8);
-    c->nb_ptr.resize(size_t(nn) + 1);
+    c->nb_ptr.resize(size_t(nn) + 1 + ufc::kNbPtrPad);
     c->nb_idx.resize(size_t(npairs));
     c->row_ptr.resize(size_t(nn) * 3 + 1);
     c->diag_pos.resize(size_t(nn) * 3);
@@ -520,7 +520,14 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     };
     up(c->coords.get(), c->h_coords.data(), c->coords.bytes());
     up(c->conn.get(), c->h_conn.data(), c->conn.bytes());
-    up(c->nb_ptr.get(), c->h_nb_ptr.data(), c->nb_ptr.bytes());
+    {
+      // padded with copies of the last entry: the FP32 SpMV reads the row
+      // pointers of a node pair as one scalar load past the last node
+      std::vector<int32_t> padded(c->h_nb_ptr);
+      padded.resize(size_t(nn) + 1 + ufc::kNbPtrPad, c->h_nb_ptr[size_t(nn)]);
+      up(c->nb_ptr.get(), padded.data(), c->nb_ptr.bytes());
+      UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
     up(c->nb_idx.get(), nb_idx.data(), c->nb_idx.bytes());
     std::vector<int16_t> off16;
     if (c->have_off16) {

@@ -81,7 +81,8 @@ constexpr int kSolveBlocks = 2048;  // vector kernels: 8 workgroups per CU on 25
 constexpr int kSolveThreads = 256;  // vector kernels: 4 wave64 per workgroup
 constexpr int kSpmvThreads = 512;   // SpMV: 8 wave64 = 16 node groups per workgroup
 constexpr int kSpmvBlocks64 = 32768;  // FP64 CSR SpMV grid (measured best on gfx950)
-constexpr int kSpmvBlocks32 = 16384;  // FP32 SpMV grid (measured best on gfx950)
+constexpr int kSpmvBlocks32 = 32768;  // FP32 SpMV grid (measured best on gfx950: 0.415 ms at 16384, 0.405 at 32768, 0.415 at 65536 for 128^3)
+constexpr int kNbPtrPad = 3;  // device nb_ptr entries past n_nodes, equal to nb_ptr[n_nodes]
 constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vector kernels)
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
@@ -157,7 +158,7 @@ struct ufc_ctx {
   int64_t nnz = 0;
   int32_t max_row_nnz = 0;
   std::vector<int32_t> h_nb_ptr;  // (n_nodes + 1)
-  ufc::DBuf<int32_t> nb_ptr;      // (n_nodes + 1)
+  ufc::DBuf<int32_t> nb_ptr;      // (n_nodes + 1 + kNbPtrPad), tail = nb_ptr[n_nodes]
   ufc::DBuf<int32_t> nb_idx;      // (nnz / 9)
   // (nnz / 9) neighbour as m - n, when every |m - n| fits in 16 bits: the FP32
   // SpMV then reads 2 instead of 4 bytes per neighbour
@@ -219,6 +220,7 @@ struct ufc_ctx {
   ufc::DBuf<double> w_t;
   // reductions
   ufc::DBuf<double> partials;            // kSolveBlocks * 4
+  ufc::DBuf<double> spmv_partials;       // kSpmvBlocks32: p.Ap per FP32 SpMV workgroup
   ufc::DBuf<ufc::PcgScalars> scalars;    // 1
   ufc::PcgScalars* h_scalars = nullptr;  // pinned mirror
 

@@ -7,9 +7,10 @@
 // with ||r|| / ||b|| < rtol (||b|| < 1e-30 -> 1), z = r / diag,
 // beta = (r.z)_new / (r.z)_old, p = z + beta p.
 //
-// One iteration is six launches; every scalar stays on the device:
+// One iteration is six launches (five for the FP32 view, whose SpMV forms the
+// p.Ap partials itself); every scalar stays on the device:
 //   spmv      : Ap = K p
-//   dot_pap   : per-workgroup partials of p.Ap
+//   dot_pap   : per-workgroup partials of p.Ap (FP64 CSR only)
 //   fin_alpha : one workgroup adds the partials in index order, alpha
 //   update    : x += alpha p, r -= alpha Ap, partials of r.r and r.(r/diag)
 //   fin_beta  : ||r||/||b||, convergence / breakdown / max_iter, beta
@@ -115,15 +116,29 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* partials
 
 constexpr int kFinThreads = 1024;
 
-// one workgroup of kFinThreads: ordered sum of nblocks partials per value,
-// valid in thread 0
+// the most partials a finalize reads per thread (the FP32 SpMV's grid)
+constexpr int kFinPerThread = kSpmvBlocks32 / kFinThreads;
+static_assert(kSpmvBlocks32 % kFinThreads == 0 && kMaxPartials <= kSpmvBlocks32, "partials");
+
+// one workgroup of kFinThreads: ordered sum of nblocks <= kSpmvBlocks32
+// partials per value, valid in thread 0.  Thread t adds partials t,
+// t + kFinThreads, ... in that order; all its loads are issued before the
+// first add (a loop of dependent adds would wait for each load in turn).
 template <int NV>
 __device__ __forceinline__ void sum_partials(double (&out)[NV], const double* partials,
                                              int nblocks) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
+    double v[kFinPerThread];
+#pragma unroll
+    for (int j = 0; j < kFinPerThread; ++j) {
+      const int b = int(threadIdx.x) + j * kFinThreads;
+      v[j] = b < nblocks ? partials[k * kMaxPartials + b] : 0.0;
+    }
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kFinThreads) s += partials[k * kMaxPartials + b];
+#pragma unroll
+    for (int j = 0; j < kFinPerThread; ++j)
+      if (int(threadIdx.x) + j * kFinThreads < nblocks) s += v[j];
     out[k] = s;
   }
   block_sum<NV, kFinThreads>(out);
@@ -187,26 +202,32 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 // (lanes 0-31) and n0 + 1 (lanes 32-63) of one wave; n0 is wave-uniform.
 // Three things keep the texture path to five vector loads per wave and node
 // pair (three value runs, the neighbour ids, one x gather):
-//   - nb_ptr[n0 .. n0+2] are scalar loads (the wave's two nodes are adjacent);
+//   - nb_ptr[n0 .. n0+2] are scalar loads (the wave's two nodes are adjacent;
+//     the device row pointers are padded past the last node);
 //   - x_n is not gathered a second time: the lane whose neighbour is the node
 //     itself (every node is its own neighbour) has it, and readlane
 //     broadcasts it to its group;
 //   - after the DPP sums inside each 16-lane row, v_permlane16_swap pairs
 //     rows 0/1 and 2/3, so every lane of a group holds the same two row sums
 //     the xor-16 shuffle gave (same operands, same order, same result).
-// Row sums of the group's node in a[0..2]; the caller writes lanes k < 3.
+// Row sums of the group's node in a[0..2] and its own x_n[k] in xk (lanes
+// k < 3); the caller writes lanes k < 3.
 template <bool OFF16>
 __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, int lane,
                                                  const int32_t* __restrict__ nb_ptr,
                                                  const int32_t* __restrict__ nb_idx,
                                                  const int16_t* __restrict__ nb_off,
                                                  const float* __restrict__ val,
-                                                 const float* __restrict__ x, float (&a)[3]) {
+                                                 const float* __restrict__ x, float (&a)[3],
+                                                 float& xk) {
   const bool hi = lane >= GROUP;
   const int k = lane & (GROUP - 1);
-  const bool has1 = n0 + 1 < n_nodes;
-  const int32_t p0 = nb_ptr[n0], p1 = nb_ptr[n0 + 1];
-  const int32_t p2 = has1 ? nb_ptr[n0 + 2] : p1;
+  // The row pointers through the constant address space: wave-uniform
+  // addresses there are always scalar loads.  n0 + 2 <= n_nodes + 1 is inside
+  // the padded row pointers; past the last node the padding repeats
+  // nb_ptr[n_nodes], so the hi group of a lone last node has degree 0.
+  const auto* kptr = (const __attribute__((address_space(4))) int32_t*)nb_ptr;
+  const int32_t p0 = kptr[n0], p1 = kptr[n0 + 1], p2 = kptr[n0 + 2];
   const int64_t n = n0 + (hi ? 1 : 0);
   const int32_t q0 = hi ? p1 : p0;
   const int deg = (hi ? p2 : p1) - q0;
@@ -235,15 +256,17 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
   const int l0 = m_lo ? __builtin_ctz(m_lo) : 0;
   const int l1 = m_hi ? __builtin_ctz(m_hi) + GROUP : GROUP;
   const float s = self ? 0.f : 1.f;
-  float xd[3];
+  float xd[3], xn[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     // both broadcasts in every lane, then a select (a divergent readlane
     // would become two exec-masked branches)
     const float xn0 = lane_value(xm[c], l0);
     const float xn1 = lane_value(xm[c], l1);
-    xd[c] = xm[c] - s * (hi ? xn1 : xn0);
+    xn[c] = hi ? xn1 : xn0;
+    xd[c] = xm[c] - s * xn[c];
   }
+  xk = (k == 0) ? xn[0] : ((k == 1) ? xn[1] : xn[2]);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     float t = v[3 * d] * xd[0] + v[3 * d + 1] * xd[1] + v[3 * d + 2] * xd[2];
@@ -257,15 +280,38 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
   }
 }
 
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const uint64_t b = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(b), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(b >> 32), lane);
+  return __longlong_as_double(int64_t((uint64_t(hi) << 32) | lo));
+}
+
+// (v[0] + v[1] + v[2]) + (v[32] + v[33] + v[34]): the sum over a wave of a
+// value held only by lanes 0-2 of each 32-lane group (wave-uniform result)
+__device__ __forceinline__ double group3_sum(double v) {
+  return ((lane_value(v, 0) + lane_value(v, 1)) + lane_value(v, 2)) +
+         ((lane_value(v, GROUP) + lane_value(v, GROUP + 1)) + lane_value(v, GROUP + 2));
+}
+
 // y = K x; with sc != null an early return once the solve is done.  OFF16:
 // the FP32 view's neighbours come as 16-bit offsets m - n (2 bytes instead of
-// 4 per neighbour, measured 0.456 -> 0.440 ms at 128^3)
+// 4 per neighbour, measured 0.456 -> 0.440 ms at 128^3).
+// pap != null (FP32 view inside PCG): also the workgroup partials of p.Ap,
+// pap[blockIdx.x] = sum over the workgroup's nodes of double(x_i) double(y_i)
+// (x = p, y = Ap as stored).  Lanes k < 3 have x_n[k] and y_n[k] in registers,
+// so the dot costs no memory traffic.  The workgroup's waves combine without
+// a barrier: each wave sums its lanes, stores the sum in an LDS slot and
+// increments an LDS counter; the wave that arrives last adds the slots in
+// wave order.  (A barrier there holds every wave until the slowest has
+// finished; that cost 0.04 ms per launch at 128^3.)  The order of every sum
+// is fixed, so p.Ap is bit-reproducible.
 template <typename T, bool BLOCK_INDEX, bool OFF16>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
     int32_t n_nodes, const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
     const int16_t* __restrict__ nb_off, const int32_t* __restrict__ col_idx,
     const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
-    const PcgScalars* __restrict__ sc) {
+    const PcgScalars* __restrict__ sc, double* __restrict__ pap) {
   if (sc && sc->done) return;
   // A workgroup takes 2 x GROUPS_PER_BLOCK consecutive nodes per pass (group
   // g: nodes g and g + GROUPS_PER_BLOCK; wave w: nodes 2w, 2w + 1 and
@@ -276,18 +322,45 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
   const int64_t stride = int64_t(gridDim.x) * kNodesPerPass;
   if constexpr (BLOCK_INDEX) {
     static_assert(sizeof(T) == 4, "the node-block view is the FP32 operator");
+    constexpr int kWaves = kSpmvThreads / WAVE;
+    __shared__ double slot[kWaves];
+    __shared__ int arrived;
     const int lane = threadIdx.x & (WAVE - 1);
     const int k = lane & (GROUP - 1);
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / WAVE);
+    if (pap) {
+      if (threadIdx.x == 0) arrived = 0;
+      __syncthreads();
+    }
+    double dot = 0.0;
     for (int64_t b = int64_t(blockIdx.x) * kNodesPerPass; b < n_nodes; b += stride) {
 #pragma unroll 1
       for (int h = 0; h < 2; ++h) {
         const int64_t n0 = b + 2 * w + h * GROUPS_PER_BLOCK;
         if (n0 >= n_nodes) break;
-        float a[3];
-        node_pair_rows32<OFF16>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, val, x, a);
+        float a[3], xk;
+        node_pair_rows32<OFF16>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, val, x, a, xk);
         const int64_t n = n0 + (lane >= GROUP ? 1 : 0);
-        if (k < 3 && n < n_nodes) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+        if (k < 3 && n < n_nodes) {
+          const float yk = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+          y[3 * n + k] = yk;
+          dot += double(xk) * double(yk);
+        }
+      }
+    }
+    if (pap) {
+      dot = group3_sum(dot);
+      if (lane == 0) {
+        slot[w] = dot;
+        const int before = __hip_atomic_fetch_add(&arrived, 1, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (before == kWaves - 1) {
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < kWaves; ++i)
+            t += __hip_atomic_load(&slot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          pap[blockIdx.x] = t;
+        }
       }
     }
   } else {
@@ -602,20 +675,23 @@ int spmv_blocks(const ufc_ctx* c) {
   return int(need < cap ? need : cap);
 }
 
+// pap: workgroup partials of x.y (FP32 view only), or null
 template <typename T, bool BLOCK_INDEX>
-void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg) {
+void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
+                 double* pap = nullptr) {
   const PcgScalars* sc = in_pcg ? c->scalars.get() : nullptr;
   if constexpr (BLOCK_INDEX) {
     if (c->have_off16) {
       hipLaunchKernelGGL((k_spmv<T, true, true>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads),
                          0, c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(),
-                         c->nb_off16.get(), c->col_idx.get(), val, x, y, sc);
+                         c->nb_off16.get(), c->col_idx.get(), val, x, y, sc, pap);
       return;
     }
   }
   hipLaunchKernelGGL((k_spmv<T, BLOCK_INDEX, false>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads),
                      0, c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(),
-                     c->nb_off16.get(), c->col_idx.get(), val, x, y, sc);
+                     c->nb_off16.get(), c->col_idx.get(), val, x, y, sc,
+                     BLOCK_INDEX ? pap : nullptr);
 }
 
 // resume: continue a CG after a residual replacement -- r := b (the new true
@@ -642,12 +718,20 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
   for (;;) {
     for (int k = 0; k < every && launched < max_iter; ++k, ++launched) {
       spmv_event_begin(c);
-      launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
-      spmv_event_end(c);
-      hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                         p, ap, part, sc);
-      hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part,
-                         kSolveBlocks, sc);
+      if constexpr (BLOCK_INDEX) {
+        // the FP32 SpMV forms the p.Ap partials itself
+        launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
+        spmv_event_end(c);
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream,
+                           c->spmv_partials.get(), spmv_blocks<T>(c), sc);
+      } else {
+        launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
+        spmv_event_end(c);
+        hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
+                           p, ap, part, sc);
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part,
+                           kSolveBlocks, sc);
+      }
       hipLaunchKernelGGL((k_update<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                          x, r, p, ap, diag, part, sc);
       hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc);
@@ -687,6 +771,7 @@ void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32) {
     c->s_p.resize(n);
     c->s_ap.resize(n);
     c->s_x.resize(n);
+    c->spmv_partials.resize(size_t(kSpmvBlocks32));
   }
   c->partials.resize(size_t(kMaxPartials) * 4);
   if (!c->scalars.get()) {
