@@ -43,8 +43,16 @@ def log(*a):
 
 def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4):
     """Algorithmic bytes of one FP32 node-block SpMV: values + node index view
-    (2-byte offsets or 4-byte ids per neighbour) + node pointers + x + y."""
-    return nnz * 4 + (nnz // 9) * index_bytes + (n_nodes + 1) * 4 + 2 * n_dof * 4
+    (one stencil-pattern byte per node, or 2-byte offsets / 4-byte ids per
+    neighbour) + node pointers + x + y.  The pattern table (at most 256 x 64
+    bytes, cache resident) is not counted."""
+    index = n_nodes if index_bytes == 1 else (nnz // 9) * index_bytes
+    return nnz * 4 + index + (n_nodes + 1) * 4 + 2 * n_dof * 4
+
+
+# kernel name of the FP32 SpMV for each neighbour-index width
+SPMV32_KERNEL = {1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
+                 4: "k_spmv<float, true, 0>"}
 
 
 def spmv64_bytes(n_dof, nnz):
@@ -310,7 +318,7 @@ def main():
     # roofline of the dominant kernel (FP32 SpMV + p.Ap inside the mixed solve)
     spmv_ms = mixed[-1]["spmv_ms"]
     b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ctx.nb_index_bytes)
-    k32 = "k_spmv<float, true, true>" if ctx.nb_index_bytes == 2 else "k_spmv<float, true, false>"
+    k32 = SPMV32_KERNEL[ctx.nb_index_bytes]
     ach = b32 / (spmv_ms * 1e6)
     copy_gbs = copy_bandwidth(torch, dev)
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

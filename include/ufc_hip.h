@@ -102,8 +102,12 @@ typedef struct ufc_rt_asm_info {
   int32_t n_nodes;
   int32_t n_elems;
   int32_t max_row_nnz;
-  int32_t nb_index_bytes; /* neighbour index width of the FP32 SpMV view: 2  */
-                          /* when every |node - neighbour| < 2^15, else 4    */
+  int32_t nb_index_bytes; /* neighbour index of the FP32 SpMV view: 1 (one  */
+                          /* stencil-pattern byte per node) when every       */
+                          /* |node - neighbour| < 2^15 and at most 256       */
+                          /* distinct neighbour-offset lists occur, else 2   */
+                          /* (16-bit offset per neighbour) when the offsets  */
+                          /* fit, else 4 (node id per neighbour)             */
 } ufc_rt_asm_info;
 int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
 

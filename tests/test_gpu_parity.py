@@ -348,6 +348,55 @@ def test_fp32_view_wide_numbering_vs_oracle(ctx, orc):
     u, info = ctx.pcg(precision=2, rtol=1e-10)
     assert info.converged
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
-    # the structured numbering of the same mesh takes the 16-bit path
+    # the structured numbering takes the stencil-pattern view (27 patterns)
     P.setup_context(ctx, P.cube_pressure(8, name="cube8"), 2)
-    assert ctx.nb_index_bytes == 2
+    assert ctx.nb_index_bytes == 1
+
+
+def _renumbered(pr, perm):
+    coords = np.empty_like(pr.coords)
+    coords[perm] = pr.coords
+    pr.conn = perm[pr.conn].astype(np.int32)
+    pr.coords = coords
+    pr.bc_eq = (3 * perm[pr.bc_eq // 3] + pr.bc_eq % 3).astype(np.int32)
+    return pr
+
+
+def test_fp32_views_vs_oracle(ctx, orc):
+    """Neighbour indexes of the FP32 view on one 16^3 cube: the structured
+    numbering (one stencil-pattern byte per node, 27 patterns), a numbering
+    shuffled inside windows of 64 ids and a random numbering (both more than
+    256 patterns, offsets below 2^15: 16-bit offsets; the 4-byte ids are
+    covered on a 32^3 cube above).  Each SpMV, and the mixed solve, matches
+    the oracle; the shuffled numberings give the structured solution back
+    after un-permuting."""
+    from ufc_hip import problems as P
+    n = 16
+    base = P.cube_pressure(n, name="cube16")
+    nn = base.coords.shape[0]
+    rng = np.random.default_rng(5)
+    window = np.concatenate([w + rng.permutation(min(64, nn - w)) for w in range(0, nn, 64)])
+    cases = [("pattern", np.arange(nn), 1), ("window", window, 2),
+             ("random", rng.permutation(nn), 2)]
+    ref = None
+    for name, perm, width in cases:
+        pr = _renumbered(P.cube_pressure(n, name="cube16"), perm)
+        rp, ci, val, F, _ = _oracle_system(orc, pr)
+        say("fp32 view", name)
+        P.setup_context(ctx, pr, 2)
+        assert ctx.nb_index_bytes == width, name
+        ctx.assemble_k(pr.young, pr.nu, precision=1)
+        x = np.random.default_rng(2).normal(size=pr.n_dof)
+        yo = orc.spmv(rp, ci, val, x)
+        y32 = ctx.spmv(x, precision=1)
+        assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max(), name
+        ctx.assemble_k(pr.young, pr.nu, precision=2)
+        u, info = ctx.pcg(precision=2, rtol=1e-10)
+        assert info.converged, name
+        uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
+        assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8, name
+        u_nat = u.reshape(-1, 3)[perm].ravel()
+        if ref is None:
+            ref = u_nat
+        else:
+            assert np.linalg.norm(u_nat - ref) / np.linalg.norm(ref) < 1e-8, name

@@ -3,6 +3,7 @@
 // the solver orchestration (FP64 / FP32 PCG, FP32-in-FP64 iterative
 // refinement, adjoint).  Compiled as HIP host code.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -148,6 +149,73 @@ void apply_fp64_operator(ufc_ctx* c, const double* u, double* y) {
     ufc::launch_apply_elem64(c, 0, u, y);
 }
 
+// Width of the FP32 view's neighbour index: 1 (one stencil-pattern byte per
+// node), 2 (16-bit offset per neighbour) or 4 (node id per neighbour).
+int32_t nb_index_bytes(const ufc_ctx* c) { return c->n_pat > 0 ? 1 : (c->have_off16 ? 2 : 4); }
+
+// Stencil-pattern dictionary of the 16-bit view (off16, CSR pair order):
+// every node's (degree, offsets) list gets an id; with at most kMaxPatterns
+// distinct lists the ids and the list table go to the device, otherwise
+// n_pat = 0 and the SpMV keeps the per-neighbour offsets.  Lists are matched
+// by a 64-bit hash and then compared element by element, so a hash collision
+// only disables the view.
+void build_stencil_patterns(ufc_ctx* c, const std::vector<int16_t>& off16) {
+  c->n_pat = 0;
+  c->nb_pat.release();
+  c->pat_off.release();
+  if (!c->have_off16) return;
+  const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
+  auto list = [&](int64_t n, int* deg) {
+    const int32_t q0 = c->h_nb_ptr[size_t(n)];
+    *deg = c->h_nb_ptr[size_t(n) + 1] - q0;
+    return off16.data() + q0;
+  };
+  std::vector<uint64_t> hash(static_cast<size_t>(nn));
+  parallel_for(nn, [&](int64_t n) {
+    int deg;
+    const int16_t* o = list(n, &deg);
+    uint64_t h = 1469598103934665603ull ^ uint64_t(deg);  // FNV-1a
+    for (int k = 0; k < deg; ++k) h = (h ^ uint64_t(uint16_t(o[k]))) * 1099511628211ull;
+    hash[size_t(n)] = h;
+  });
+  std::map<uint64_t, int> ids;
+  std::vector<int64_t> rep;  // first node of each pattern
+  const size_t padded = (size_t(nn) + 3) / 4 * 4 + 4;
+  std::vector<uint8_t> pat(padded, 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    auto it = ids.find(hash[size_t(n)]);
+    if (it == ids.end()) {
+      if (int(rep.size()) == ufc::kMaxPatterns) return;
+      it = ids.emplace(hash[size_t(n)], int(rep.size())).first;
+      rep.push_back(n);
+    }
+    pat[size_t(n)] = uint8_t(it->second);
+  }
+  std::atomic<int> mismatch{0};
+  parallel_for(nn, [&](int64_t n) {
+    int d0, d1;
+    const int16_t* a = list(n, &d0);
+    const int16_t* b = list(rep[pat[size_t(n)]], &d1);
+    if (d0 != d1 || !std::equal(a, a + d0, b)) mismatch.store(1, std::memory_order_relaxed);
+  });
+  if (mismatch.load()) return;
+  const int np = int(rep.size());
+  std::vector<int16_t> table(size_t(np) * ufc::kMaxNodeNbrs, 0);
+  for (int p = 0; p < np; ++p) {
+    int deg;
+    const int16_t* o = list(rep[size_t(p)], &deg);
+    std::copy(o, o + deg, table.begin() + size_t(p) * ufc::kMaxNodeNbrs);
+  }
+  c->nb_pat.resize(padded);
+  c->pat_off.resize(table.size());
+  UFC_HIP_CHECK(hipMemcpyAsync(c->nb_pat.get(), pat.data(), c->nb_pat.bytes(),
+                               hipMemcpyHostToDevice, c->stream));
+  UFC_HIP_CHECK(hipMemcpyAsync(c->pat_off.get(), table.data(), c->pat_off.bytes(),
+                               hipMemcpyHostToDevice, c->stream));
+  UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->n_pat = np;
+}
+
 }  // namespace
 
 extern "C" {
@@ -206,7 +274,7 @@ int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
     info->n_nodes = c->n_nodes;
     info->n_elems = c->n_elems;
     info->max_row_nnz = c->max_row_nnz;
-    info->nb_index_bytes = c->have_off16 ? 2 : 4;
+    info->nb_index_bytes = nb_index_bytes(c);
     return UFC_STATUS_OK;
   });
 }
@@ -541,6 +609,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     } else {
       c->nb_off16.release();
     }
+    build_stencil_patterns(c, off16);
     up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
     up(c->diag_pos.get(), diag_pos.data(), c->diag_pos.bytes());
     up(c->slot_elem.get(), s_elem.data(), c->slot_elem.bytes());
@@ -554,7 +623,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       info->n_nodes = nn;
       info->n_elems = ne;
       info->max_row_nnz = c->max_row_nnz;
-      info->nb_index_bytes = c->have_off16 ? 2 : 4;
+      info->nb_index_bytes = nb_index_bytes(c);
     }
     msg = "pattern built: nnz " + std::to_string(nnz);
     return UFC_STATUS_OK;

@@ -86,6 +86,7 @@ constexpr int kNbPtrPad = 3;  // device nb_ptr entries past n_nodes, equal to nb
 constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vector kernels)
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
+constexpr int kMaxPatterns = 256;   // stencil patterns of the FP32 view (one-byte ids)
 
 // C3D8 variants, PH_ELEM_C3D8_VARIANT_* (PH_Elem_C3D8.f90:111-116)
 constexpr int kVarFull = 1, kVarReduced = 2, kVarSelective = 3, kVarIncompat = 4,
@@ -164,6 +165,14 @@ struct ufc_ctx {
   // SpMV then reads 2 instead of 4 bytes per neighbour
   ufc::DBuf<int16_t> nb_off16;
   bool have_off16 = false;
+  // Stencil patterns of the 16-bit view: when at most kMaxPatterns distinct
+  // neighbour-offset lists (degree and offsets) occur -- any structured hex
+  // box has at most 27 -- node n's list is pat_off[nb_pat[n]] and the FP32
+  // SpMV reads one byte per node instead of 2 per neighbour.  n_pat = 0: no
+  // pattern view.
+  ufc::DBuf<uint8_t> nb_pat;      // (n_nodes rounded up to 4, + 4), pattern id
+  ufc::DBuf<int16_t> pat_off;     // (n_pat, kMaxNodeNbrs) offsets m - n, zero padded
+  int32_t n_pat = 0;
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)
   ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
   ufc::DBuf<int32_t> diag_pos;    // (n_dof) position of K(i,i) in values

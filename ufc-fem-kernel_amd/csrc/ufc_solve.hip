@@ -49,6 +49,10 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int GROUP = 32;  // lanes per node
+// neighbour index of the FP32 node-block view
+constexpr int kViewId32 = 0;     // 4-byte neighbour ids
+constexpr int kViewOff16 = 1;    // 16-bit offsets m - n per neighbour
+constexpr int kViewPattern = 2;  // one stencil-pattern byte per node
 constexpr int WAVES_PER_BLOCK = kSolveThreads / WAVE;
 constexpr int GROUPS_PER_BLOCK = kSpmvThreads / GROUP;
 
@@ -145,49 +149,24 @@ __device__ __forceinline__ void sum_partials(double (&out)[NV], const double* pa
 }
 
 // ---------------------------------------------------------------- SpMV body
-// Row sums of node n in a[0..2] (valid in every lane of the 32-lane group).
-template <typename T, bool BLOCK_INDEX, bool OFF16>
-__device__ __forceinline__ void node_rows(int64_t n, int k, const int32_t* __restrict__ nb_ptr,
-                                          const int32_t* __restrict__ nb_idx,
-                                          const int16_t* __restrict__ nb_off,
-                                          const int32_t* __restrict__ col_idx,
-                                          const T* __restrict__ val, const T* __restrict__ x,
-                                          T (&a)[3]) {
+// Plain CSR rows of node n in a[0..2] (valid in every lane of the 32-lane
+// group): every value's column comes from col_idx.
+template <typename T>
+__device__ __forceinline__ void node_rows_csr(int64_t n, int k, const int32_t* __restrict__ nb_ptr,
+                                              const int32_t* __restrict__ col_idx,
+                                              const T* __restrict__ val, const T* __restrict__ x,
+                                              T (&a)[3]) {
   const int32_t q0 = nb_ptr[n];
   const int deg = nb_ptr[n + 1] - q0;
   a[0] = a[1] = a[2] = T(0);
   if (k < deg) {
     const int64_t base = 9 * int64_t(q0) + 3 * k;
     const int d3 = 3 * deg;
-    if constexpr (BLOCK_INDEX) {
-      // Issue order: the neighbour id, then the 9 values (they depend only on
-      // nb_ptr), then -- behind a scheduling barrier, so the compiler cannot
-      // sink the value loads below the wait for the id -- the x gathers that
-      // need the id.  The value loads then overlap both the id and the x
-      // latency (measured 0.535 -> 0.457 ms with the DPP reduction, 128^3).
-      // Difference form: the self block holds the penalty diagonal and
-      // multiplies x_n; every other block multiplies x_m - x_n (s = 0 leaves
-      // x_m exact, branch-free).
-      const int64_t m = OFF16 ? n + int64_t(nb_off[q0 + k]) : int64_t(nb_idx[q0 + k]);
-      T v[9];
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) v[3 * d + c] = val[base + d * d3 + c];
-      __builtin_amdgcn_sched_barrier(0);
-      const T s = (m != n) ? T(1) : T(0);
-      const T x0 = x[3 * m] - s * x[3 * n];
-      const T x1 = x[3 * m + 1] - s * x[3 * n + 1];
-      const T x2 = x[3 * m + 2] - s * x[3 * n + 2];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) a[d] = v[3 * d] * x0 + v[3 * d + 1] * x1 + v[3 * d + 2] * x2;
-    } else {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const T* v = val + base + d * d3;
-        const int32_t* c = col_idx + base + d * d3;
-        a[d] = v[0] * x[c[0]] + v[1] * x[c[1]] + v[2] * x[c[2]];
-      }
+    for (int d = 0; d < 3; ++d) {
+      const T* v = val + base + d * d3;
+      const int32_t* c = col_idx + base + d * d3;
+      a[d] = v[0] * x[c[0]] + v[1] * x[c[1]] + v[2] * x[c[2]];
     }
   }
 #pragma unroll
@@ -212,11 +191,13 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 //     the xor-16 shuffle gave (same operands, same order, same result).
 // Row sums of the group's node in a[0..2] and its own x_n[k] in xk (lanes
 // k < 3); the caller writes lanes k < 3.
-template <bool OFF16>
+template <int VIEW>
 __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, int lane,
                                                  const int32_t* __restrict__ nb_ptr,
                                                  const int32_t* __restrict__ nb_idx,
                                                  const int16_t* __restrict__ nb_off,
+                                                 const uint8_t* __restrict__ nb_pat,
+                                                 const int16_t* __restrict__ pat_off,
                                                  const float* __restrict__ val,
                                                  const float* __restrict__ x, float (&a)[3],
                                                  float& xk) {
@@ -226,8 +207,18 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
   // addresses there are always scalar loads.  n0 + 2 <= n_nodes + 1 is inside
   // the padded row pointers; past the last node the padding repeats
   // nb_ptr[n_nodes], so the hi group of a lone last node has degree 0.
+  // Stencil view: the pattern ids of n0 and n0 + 1 are bytes of one scalar
+  // dword (n0 is even; the id array is padded past the last node).  The empty
+  // asm consumes it next to the row pointers, so both scalar loads are issued
+  // together instead of the id load sinking into the branch below, behind a
+  // wait of its own.
+  uint32_t word = 0;
+  if constexpr (VIEW == kViewPattern)
+    word = ((const __attribute__((address_space(4))) uint32_t*)nb_pat)[n0 >> 2];
   const auto* kptr = (const __attribute__((address_space(4))) int32_t*)nb_ptr;
   const int32_t p0 = kptr[n0], p1 = kptr[n0 + 1], p2 = kptr[n0 + 2];
+  if constexpr (VIEW == kViewPattern) __asm__ volatile("" : : "s"(word));
+  const uint32_t pid = (word >> (8 * int(n0 & 3) + (hi ? 8 : 0))) & 0xffu;
   const int64_t n = n0 + (hi ? 1 : 0);
   const int32_t q0 = hi ? p1 : p0;
   const int deg = (hi ? p2 : p1) - q0;
@@ -239,7 +230,13 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
     // issue order: neighbour id, then the 9 values (they depend only on
     // nb_ptr), then -- behind a scheduling barrier, so the compiler cannot
     // sink the value loads below the wait for the id -- the x gather
-    const int64_t m = OFF16 ? n + int64_t(nb_off[q0 + k]) : int64_t(nb_idx[q0 + k]);
+    int64_t m;
+    if constexpr (VIEW == kViewPattern)
+      m = n + int64_t(pat_off[pid * kMaxNodeNbrs + k]);
+    else if constexpr (VIEW == kViewOff16)
+      m = n + int64_t(nb_off[q0 + k]);
+    else
+      m = int64_t(nb_idx[q0 + k]);
     const int64_t base = 9 * int64_t(q0) + 3 * k;
     const int d3 = 3 * deg;
 #pragma unroll
@@ -294,9 +291,11 @@ __device__ __forceinline__ double group3_sum(double v) {
          ((lane_value(v, GROUP) + lane_value(v, GROUP + 1)) + lane_value(v, GROUP + 2));
 }
 
-// y = K x; with sc != null an early return once the solve is done.  OFF16:
-// the FP32 view's neighbours come as 16-bit offsets m - n (2 bytes instead of
-// 4 per neighbour, measured 0.456 -> 0.440 ms at 128^3).
+// y = K x; with sc != null an early return once the solve is done.  VIEW, the
+// FP32 view's neighbour index: kViewId32 (4-byte node ids), kViewOff16
+// (16-bit offsets m - n, measured 0.456 -> 0.440 ms at 128^3 against 4-byte
+// ids) or kViewPattern (one stencil-pattern byte per node; the offsets come
+// from the small pattern table, which stays in cache).
 // pap != null (FP32 view inside PCG): also the workgroup partials of p.Ap,
 // pap[blockIdx.x] = sum over the workgroup's nodes of double(x_i) double(y_i)
 // (x = p, y = Ap as stored).  Lanes k < 3 have x_n[k] and y_n[k] in registers,
@@ -306,10 +305,11 @@ __device__ __forceinline__ double group3_sum(double v) {
 // wave order.  (A barrier there holds every wave until the slowest has
 // finished; that cost 0.04 ms per launch at 128^3.)  The order of every sum
 // is fixed, so p.Ap is bit-reproducible.
-template <typename T, bool BLOCK_INDEX, bool OFF16>
+template <typename T, bool BLOCK_INDEX, int VIEW>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
     int32_t n_nodes, const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
-    const int16_t* __restrict__ nb_off, const int32_t* __restrict__ col_idx,
+    const int16_t* __restrict__ nb_off, const uint8_t* __restrict__ nb_pat,
+    const int16_t* __restrict__ pat_off, const int32_t* __restrict__ col_idx,
     const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
     const PcgScalars* __restrict__ sc, double* __restrict__ pap) {
   if (sc && sc->done) return;
@@ -339,7 +339,8 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
         const int64_t n0 = b + 2 * w + h * GROUPS_PER_BLOCK;
         if (n0 >= n_nodes) break;
         float a[3], xk;
-        node_pair_rows32<OFF16>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, val, x, a, xk);
+        node_pair_rows32<VIEW>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, nb_pat, pat_off, val,
+                               x, a, xk);
         const int64_t n = n0 + (lane >= GROUP ? 1 : 0);
         if (k < 3 && n < n_nodes) {
           const float yk = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
         const int64_t n = b + gi + h * GROUPS_PER_BLOCK;
         if (n >= n_nodes) break;
         T a[3];
-        node_rows<T, BLOCK_INDEX, OFF16>(n, k, nb_ptr, nb_idx, nb_off, col_idx, val, x, a);
+        node_rows_csr<T>(n, k, nb_ptr, col_idx, val, x, a);
         if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
       }
     }
@@ -680,18 +681,25 @@ template <typename T, bool BLOCK_INDEX>
 void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
                  double* pap = nullptr) {
   const PcgScalars* sc = in_pcg ? c->scalars.get() : nullptr;
+  const dim3 grid(spmv_blocks<T>(c)), block(kSpmvThreads);
   if constexpr (BLOCK_INDEX) {
-    if (c->have_off16) {
-      hipLaunchKernelGGL((k_spmv<T, true, true>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads),
-                         0, c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(),
-                         c->nb_off16.get(), c->col_idx.get(), val, x, y, sc, pap);
-      return;
-    }
+    if (c->n_pat > 0)
+      hipLaunchKernelGGL((k_spmv<T, true, kViewPattern>), grid, block, 0, c->stream, c->n_nodes,
+                         c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
+                         c->pat_off.get(), c->col_idx.get(), val, x, y, sc, pap);
+    else if (c->have_off16)
+      hipLaunchKernelGGL((k_spmv<T, true, kViewOff16>), grid, block, 0, c->stream, c->n_nodes,
+                         c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
+                         c->pat_off.get(), c->col_idx.get(), val, x, y, sc, pap);
+    else
+      hipLaunchKernelGGL((k_spmv<T, true, kViewId32>), grid, block, 0, c->stream, c->n_nodes,
+                         c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
+                         c->pat_off.get(), c->col_idx.get(), val, x, y, sc, pap);
+  } else {
+    hipLaunchKernelGGL((k_spmv<T, false, kViewId32>), grid, block, 0, c->stream, c->n_nodes,
+                       c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
+                       c->pat_off.get(), c->col_idx.get(), val, x, y, sc, nullptr);
   }
-  hipLaunchKernelGGL((k_spmv<T, BLOCK_INDEX, false>), dim3(spmv_blocks<T>(c)), dim3(kSpmvThreads),
-                     0, c->stream, c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(),
-                     c->nb_off16.get(), c->col_idx.get(), val, x, y, sc,
-                     BLOCK_INDEX ? pap : nullptr);
 }
 
 // resume: continue a CG after a residual replacement -- r := b (the new true
