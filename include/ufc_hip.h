@@ -372,9 +372,12 @@ typedef struct ufc_timing {
                        /* solves of the last solve                           */
 } ufc_timing;
 int ufc_ctx_timing(ufc_ctx* ctx, ufc_timing* t, ufc_status* st);
-/* per_kernel = 1 brackets every SpMV launch with a HIP event pair so that
- * spmv_ms_avg is the mean kernel duration (no counterpart in the reference,
- * whose timers are CPU_TIME around whole phases, RT_Asm_Global.f90:354,399). */
+/* per_kernel = N > 0 brackets every N-th SpMV launch with a HIP event pair
+ * so that spmv_ms_avg is the mean kernel duration over the sampled launches
+ * (an event record costs a few microseconds of queue time, so N > 1 keeps a
+ * timed solve close to an untimed one); 0 switches it off.  No counterpart
+ * in the reference, whose timers are CPU_TIME around whole phases
+ * (RT_Asm_Global.f90:354,399). */
 int ufc_ctx_enable_timing(ufc_ctx* ctx, int32_t per_kernel, ufc_status* st);
 
 /* ====================================================================== */

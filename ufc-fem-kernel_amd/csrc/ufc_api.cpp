@@ -282,7 +282,8 @@ int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
 int ufc_ctx_enable_timing(ufc_ctx* c, int32_t per_kernel, ufc_status* st) {
   return guarded(st, "ufc_ctx_enable_timing", [&](std::string&) {
     check_ctx(c);
-    c->per_kernel = per_kernel != 0;
+    UFC_REQUIRE(per_kernel >= 0, UFC_STATUS_INVALID, "per_kernel < 0");
+    c->per_kernel = per_kernel;
     return UFC_STATUS_OK;
   });
 }

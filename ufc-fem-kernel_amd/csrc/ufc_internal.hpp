@@ -235,10 +235,13 @@ struct ufc_ctx {
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-  bool per_kernel = false;
+  int32_t per_kernel = 0;           // bracket every per_kernel-th SpMV launch (0: none)
   std::vector<hipEvent_t> spmv_ev;  // pairs
   size_t spmv_ev_used = 0;
   std::vector<char> spmv_active;    // per pair: 1 = the launch did work
+  std::vector<int64_t> spmv_pair_launch;  // per pair: index of the bracketed launch
+  int64_t spmv_launches = 0;        // SpMV launches since the last reset
+  bool spmv_sampled = false;        // the current launch is bracketed
   double assemble_ms = 0.0, solve_ms = 0.0, spmv_ms_avg = 0.0;
   int32_t spmv_calls = 0;
   int32_t n_replace = 0;  // FP64 residual replacements of the last MIXED solve

@@ -642,10 +642,14 @@ __global__ __launch_bounds__(kSolveThreads) void k_expand_cols(int32_t n_nodes,
 
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
-// SpMV launches are bracketed by event pairs when per-kernel timing is on;
-// `active` marks the pairs whose launch did work (not a post-convergence no-op).
+// SpMV launches are bracketed by event pairs when per-kernel timing is on:
+// every per_kernel-th launch (an event record costs ~5.6 us of queue time, so
+// sampling keeps the timed solve close to an untimed one).  Each pair keeps
+// the index of the launch it brackets, so pairs of launches that returned at
+// entry (after convergence) can be left out of the average.
 void spmv_event_begin(ufc_ctx* c) {
-  if (!c->per_kernel) return;
+  c->spmv_sampled = c->per_kernel > 0 && c->spmv_launches % c->per_kernel == 0;
+  if (!c->spmv_sampled) return;
   if (c->spmv_ev_used + 2 > c->spmv_ev.size()) {
     for (int k = 0; k < 256; ++k) {
       hipEvent_t e;
@@ -656,10 +660,13 @@ void spmv_event_begin(ufc_ctx* c) {
   UFC_HIP_CHECK(hipEventRecord(c->spmv_ev[c->spmv_ev_used], c->stream));
 }
 void spmv_event_end(ufc_ctx* c) {
-  if (!c->per_kernel) return;
-  UFC_HIP_CHECK(hipEventRecord(c->spmv_ev[c->spmv_ev_used + 1], c->stream));
-  c->spmv_ev_used += 2;
-  c->spmv_active.push_back(1);
+  if (c->spmv_sampled) {
+    UFC_HIP_CHECK(hipEventRecord(c->spmv_ev[c->spmv_ev_used + 1], c->stream));
+    c->spmv_ev_used += 2;
+    c->spmv_active.push_back(1);
+    c->spmv_pair_launch.push_back(c->spmv_launches);
+  }
+  ++c->spmv_launches;
 }
 
 void poll(ufc_ctx* c) {
@@ -721,7 +728,7 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
                        p, r, diag, sc);
   UFC_HIP_CHECK(hipGetLastError());
   const int every = check_every > 0 ? check_every : 32;
-  const size_t first_event = c->spmv_active.size();
+  const int64_t first_launch = c->spmv_launches;
   int launched = 0;
   for (;;) {
     for (int k = 0; k < every && launched < max_iter; ++k, ++launched) {
@@ -753,8 +760,8 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
   *niter = c->h_scalars->iter;
   *rnorm = c->h_scalars->rnorm;
   // SpMV launches after the converging iteration returned at entry
-  for (size_t i = first_event + size_t(*niter); i < c->spmv_active.size(); ++i)
-    c->spmv_active[i] = 0;
+  for (size_t i = 0; i < c->spmv_active.size(); ++i)
+    if (c->spmv_pair_launch[i] >= first_launch + *niter) c->spmv_active[i] = 0;
   const int done = c->h_scalars->done;
   if (done == 1) return UFC_STATUS_OK;
   if (done == 2) return UFC_STATUS_ERROR;
@@ -890,6 +897,8 @@ double spmv_event_avg_ms(ufc_ctx* c, int* calls) {
 void reset_spmv_events(ufc_ctx* c) {
   c->spmv_ev_used = 0;
   c->spmv_active.clear();
+  c->spmv_pair_launch.clear();
+  c->spmv_launches = 0;
 }
 
 void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u) {

@@ -280,6 +280,8 @@ class Context:
         return out
 
     def enable_timing(self, per_kernel=True):
+        """HIP event pairs around every per_kernel-th SpMV launch (True: every
+        launch, 0/False: none); timing().spmv_ms_avg is their mean."""
         self._call("ufc_ctx_enable_timing", int(per_kernel))
 
     def timing(self):
