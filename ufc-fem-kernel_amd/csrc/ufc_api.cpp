@@ -200,11 +200,19 @@ void build_stencil_patterns(ufc_ctx* c, const std::vector<int16_t>& off16) {
   });
   if (mismatch.load()) return;
   const int np = int(rep.size());
+  // table row: the offsets of the neighbours other than the node itself in
+  // ascending order, then 0 (the node itself) at position deg - 1 -- the lane
+  // order of the off-diagonal values (val32m)
   std::vector<int16_t> table(size_t(np) * ufc::kMaxNodeNbrs, 0);
   for (int p = 0; p < np; ++p) {
     int deg;
     const int16_t* o = list(rep[size_t(p)], &deg);
-    std::copy(o, o + deg, table.begin() + size_t(p) * ufc::kMaxNodeNbrs);
+    int16_t* row = table.data() + size_t(p) * ufc::kMaxNodeNbrs;
+    int j = 0;
+    for (int k = 0; k < deg; ++k)
+      if (o[k] != 0) row[j++] = o[k];
+    if (j != deg - 1) return;  // every list holds the node itself once
+    row[deg - 1] = 0;
   }
   c->nb_pat.resize(padded);
   c->pat_off.resize(table.size());
@@ -807,6 +815,7 @@ int ufc_rt_asm_apply_bc(ufc_ctx* c, const ufc_rt_bc_args* a, ufc_status* st) {
     if (c->have32) {
       ufc::launch_penalty(c, true, +1.0);
       ufc::launch_diag32_penalty(c);
+      ufc::launch_fp32_offdiag(c);
     }
     upload_rhs(c);
     return UFC_STATUS_OK;

@@ -171,7 +171,9 @@ struct ufc_ctx {
   // SpMV reads one byte per node instead of 2 per neighbour.  n_pat = 0: no
   // pattern view.
   ufc::DBuf<uint8_t> nb_pat;      // (n_nodes rounded up to 4, + 4), pattern id
-  ufc::DBuf<int16_t> pat_off;     // (n_pat, kMaxNodeNbrs) offsets m - n, zero padded
+  // (n_pat, kMaxNodeNbrs): offsets m - n of the neighbours other than the
+  // node itself (ascending), then 0 at position deg - 1, zero padded
+  ufc::DBuf<int16_t> pat_off;
   int32_t n_pat = 0;
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)
   ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
@@ -188,6 +190,15 @@ struct ufc_ctx {
   ufc::DBuf<double> diag64;
   ufc::DBuf<float> diag32;   // K(i,i) + penalty (Jacobi)
   ufc::DBuf<float> kdiag32;  // K(i,i) of the FP32 assembly, no penalty
+  // Off-diagonal copy of the FP32 difference-form values for the stencil-
+  // pattern SpMV (n_pat > 0): node n's three rows without the self block,
+  // 9 (deg - 1) values at 9 (nb_ptr[n] - n), row d holding the neighbours
+  // other than n in ascending order; the self block, which in difference form
+  // holds only the penalty diagonal, becomes pen32 plus one bit per node
+  // (pen_flag) marking nodes with a nonzero penalty.
+  ufc::DBuf<float> val32m;     // (9 (nnz / 9 - n_nodes))
+  ufc::DBuf<float> pen32;      // (n_dof)
+  ufc::DBuf<uint32_t> pen_flag;  // (n_nodes / 32 + 1)
   bool have64 = false, have32 = false;
   int last_material = UFC_MAT_ELASTIC;
   int variant = 1;                 // element variant of the last elastic assembly
@@ -264,6 +275,8 @@ void launch_penalty(ufc_ctx* c, bool fp32, double sign);
 // by the penalty diagonal, diag32 = kdiag32 + penalty
 void launch_fp32_difference_form(ufc_ctx* c);
 void launch_diag32_penalty(ufc_ctx* c);
+// val32m, pen32 and pen_flag from val32 (stencil-pattern view only)
+void launch_fp32_offdiag(ufc_ctx* c);
 // y = K u in FP64 without penalties: element forces then node gather.
 // mode 0: elastic with c->young; mode 1: J2 tangent D_gp; mode 2: J2 trial stress (u unused)
 void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y);

@@ -196,8 +196,6 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
                                                  const int32_t* __restrict__ nb_ptr,
                                                  const int32_t* __restrict__ nb_idx,
                                                  const int16_t* __restrict__ nb_off,
-                                                 const uint8_t* __restrict__ nb_pat,
-                                                 const int16_t* __restrict__ pat_off,
                                                  const float* __restrict__ val,
                                                  const float* __restrict__ x, float (&a)[3],
                                                  float& xk) {
@@ -207,18 +205,9 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
   // addresses there are always scalar loads.  n0 + 2 <= n_nodes + 1 is inside
   // the padded row pointers; past the last node the padding repeats
   // nb_ptr[n_nodes], so the hi group of a lone last node has degree 0.
-  // Stencil view: the pattern ids of n0 and n0 + 1 are bytes of one scalar
-  // dword (n0 is even; the id array is padded past the last node).  The empty
-  // asm consumes it next to the row pointers, so both scalar loads are issued
-  // together instead of the id load sinking into the branch below, behind a
-  // wait of its own.
-  uint32_t word = 0;
-  if constexpr (VIEW == kViewPattern)
-    word = ((const __attribute__((address_space(4))) uint32_t*)nb_pat)[n0 >> 2];
+  static_assert(VIEW == kViewOff16 || VIEW == kViewId32, "CSR-layout views");
   const auto* kptr = (const __attribute__((address_space(4))) int32_t*)nb_ptr;
   const int32_t p0 = kptr[n0], p1 = kptr[n0 + 1], p2 = kptr[n0 + 2];
-  if constexpr (VIEW == kViewPattern) __asm__ volatile("" : : "s"(word));
-  const uint32_t pid = (word >> (8 * int(n0 & 3) + (hi ? 8 : 0))) & 0xffu;
   const int64_t n = n0 + (hi ? 1 : 0);
   const int32_t q0 = hi ? p1 : p0;
   const int deg = (hi ? p2 : p1) - q0;
@@ -230,13 +219,7 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
     // issue order: neighbour id, then the 9 values (they depend only on
     // nb_ptr), then -- behind a scheduling barrier, so the compiler cannot
     // sink the value loads below the wait for the id -- the x gather
-    int64_t m;
-    if constexpr (VIEW == kViewPattern)
-      m = n + int64_t(pat_off[pid * kMaxNodeNbrs + k]);
-    else if constexpr (VIEW == kViewOff16)
-      m = n + int64_t(nb_off[q0 + k]);
-    else
-      m = int64_t(nb_idx[q0 + k]);
+    const int64_t m = VIEW == kViewOff16 ? n + int64_t(nb_off[q0 + k]) : int64_t(nb_idx[q0 + k]);
     const int64_t base = 9 * int64_t(q0) + 3 * k;
     const int d3 = 3 * deg;
 #pragma unroll
@@ -277,6 +260,93 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
   }
 }
 
+// Stencil-pattern view over the off-diagonal values (val32m): the same pair
+// of nodes n0, n0 + 1 as node_pair_rows32, with
+//   - the pattern ids of both nodes as bytes of one scalar dword (n0 is even;
+//     the id array is padded past the last node), issued with the row
+//     pointers (the empty asm consumes the id there, so the compiler cannot
+//     sink its load into the branch below, behind a wait of its own);
+//   - lane k < deg - 1 on neighbour k other than the node itself (offset from
+//     the pattern table, which stays in cache) with its 9 values, lane deg - 1
+//     on the node itself (offset 0: x_n, no values), so readlane takes x_n
+//     from a lane the row pointers name and no self test is needed;
+//   - the penalty diagonal P_n x_n added after the row sums, from pen32, in
+//     the pairs whose pen_flag bit is set (the clamped face only).
+__device__ __forceinline__ void node_pair_rows32_offdiag(
+    int64_t n0, int32_t n_nodes, int lane, const int32_t* __restrict__ nb_ptr,
+    const uint8_t* __restrict__ nb_pat, const int16_t* __restrict__ pat_off,
+    const float* __restrict__ valm, const float* __restrict__ pen,
+    const uint32_t* __restrict__ pen_flag, const float* __restrict__ x, float (&a)[3],
+    float& xk) {
+  const bool hi = lane >= GROUP;
+  const int k = lane & (GROUP - 1);
+  const uint32_t word = ((const __attribute__((address_space(4))) uint32_t*)nb_pat)[n0 >> 2];
+  const uint32_t flags =
+      ((const __attribute__((address_space(4))) uint32_t*)pen_flag)[n0 >> 5];
+  const auto* kptr = (const __attribute__((address_space(4))) int32_t*)nb_ptr;
+  const int32_t p0 = kptr[n0], p1 = kptr[n0 + 1], p2 = kptr[n0 + 2];
+  __asm__ volatile("" : : "s"(word));
+  const uint32_t pid = (word >> (8 * int(n0 & 3) + (hi ? 8 : 0))) & 0xffu;
+  // degrees of both groups (wave-uniform); past the last node deg_hi = 0
+  const int deg_lo = p1 - p0, deg_hi = p2 - p1;
+  const int64_t n = n0 + (hi ? 1 : 0);
+  const int deg = hi ? deg_hi : deg_lo;
+  const int nv = deg - 1;
+  // node n's off-diagonal segment: every node before it dropped one block
+  const int64_t seg = 9 * (int64_t(hi ? p1 : p0) - n);
+  float v[9], xm[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < 9; ++j) v[j] = 0.f;
+  if (k < deg) {
+    // issue order: table offset, values, then (behind a scheduling barrier,
+    // so the value loads are not sunk below the wait for the offset) x
+    // (the self lane re-reads the last block, in bounds and without a
+    // branch of its own, and multiplies it by zero below)
+    const int64_t m = n + int64_t(pat_off[pid * kMaxNodeNbrs + k]);
+    const int64_t base = seg + 3 * (k < nv ? k : nv - 1);
+    const int d3 = 3 * nv;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[3 * d + c] = valm[base + d * d3 + c];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xm[c] = x[3 * m + c];
+  }
+  const int l0 = deg_lo - 1, l1 = GROUP + deg_hi - 1;
+  const bool off = k < nv;
+  float xd[3], xn[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float xn0 = lane_value(xm[c], l0);
+    const float xn1 = lane_value(xm[c], l1);
+    xn[c] = hi ? xn1 : xn0;
+    xd[c] = off ? xm[c] - xn[c] : 0.f;
+  }
+  xk = (k == 0) ? xn[0] : ((k == 1) ? xn[1] : xn[2]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float t = v[3 * d] * xd[0] + v[3 * d + 1] * xd[1] + v[3 * d + 2] * xd[2];
+    t += dpp_f32<0xB1>(t);
+    t += dpp_f32<0x4E>(t);
+    t += dpp_f32<0x141>(t);
+    t += dpp_f32<0x140>(t);
+    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(t), __float_as_uint(t),
+                                                     false, false);
+    a[d] = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+  }
+  // penalty rows: bits n0 & 31 and (n0 & 31) + 1 of the same word (n0 even)
+  const uint32_t pbits = (flags >> (n0 & 31)) & 3u;
+  if (pbits && k < 3 && n < n_nodes) {
+    const float pk = pen[3 * n + k];
+    const float ak = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
+    const float r = ak + pk * xk;
+    if (k == 0) a[0] = r;
+    else if (k == 1) a[1] = r;
+    else a[2] = r;
+  }
+}
+
 __device__ __forceinline__ double lane_value(double v, int lane) {
   const uint64_t b = uint64_t(__double_as_longlong(v));
   const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(b), lane);
@@ -295,7 +365,9 @@ __device__ __forceinline__ double group3_sum(double v) {
 // FP32 view's neighbour index: kViewId32 (4-byte node ids), kViewOff16
 // (16-bit offsets m - n, measured 0.456 -> 0.440 ms at 128^3 against 4-byte
 // ids) or kViewPattern (one stencil-pattern byte per node; the offsets come
-// from the small pattern table, which stays in cache).
+// from the small pattern table, which stays in cache; the values are the
+// off-diagonal copy val32m, 9 values per node fewer than the CSR, plus the
+// penalty diagonal on the clamped nodes).
 // pap != null (FP32 view inside PCG): also the workgroup partials of p.Ap,
 // pap[blockIdx.x] = sum over the workgroup's nodes of double(x_i) double(y_i)
 // (x = p, y = Ap as stored).  Lanes k < 3 have x_n[k] and y_n[k] in registers,
@@ -310,7 +382,8 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
     int32_t n_nodes, const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ nb_idx,
     const int16_t* __restrict__ nb_off, const uint8_t* __restrict__ nb_pat,
     const int16_t* __restrict__ pat_off, const int32_t* __restrict__ col_idx,
-    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+    const T* __restrict__ val, const float* __restrict__ pen,
+    const uint32_t* __restrict__ pen_flag, const T* __restrict__ x, T* __restrict__ y,
     const PcgScalars* __restrict__ sc, double* __restrict__ pap) {
   if (sc && sc->done) return;
   // A workgroup takes 2 x GROUPS_PER_BLOCK consecutive nodes per pass (group
@@ -339,8 +412,11 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
         const int64_t n0 = b + 2 * w + h * GROUPS_PER_BLOCK;
         if (n0 >= n_nodes) break;
         float a[3], xk;
-        node_pair_rows32<VIEW>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, nb_pat, pat_off, val,
-                               x, a, xk);
+        if constexpr (VIEW == kViewPattern)
+          node_pair_rows32_offdiag(n0, n_nodes, lane, nb_ptr, nb_pat, pat_off, val, pen,
+                                   pen_flag, x, a, xk);
+        else
+          node_pair_rows32<VIEW>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, val, x, a, xk);
         const int64_t n = n0 + (lane >= GROUP ? 1 : 0);
         if (k < 3 && n < n_nodes) {
           const float yk = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
@@ -690,22 +766,27 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
   const PcgScalars* sc = in_pcg ? c->scalars.get() : nullptr;
   const dim3 grid(spmv_blocks<T>(c)), block(kSpmvThreads);
   if constexpr (BLOCK_INDEX) {
+    // the stencil-pattern view reads the off-diagonal copy of val (val32m)
     if (c->n_pat > 0)
       hipLaunchKernelGGL((k_spmv<T, true, kViewPattern>), grid, block, 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
-                         c->pat_off.get(), c->col_idx.get(), val, x, y, sc, pap);
+                         c->pat_off.get(), c->col_idx.get(), c->val32m.get(), c->pen32.get(),
+                         c->pen_flag.get(), x, y, sc, pap);
     else if (c->have_off16)
       hipLaunchKernelGGL((k_spmv<T, true, kViewOff16>), grid, block, 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
-                         c->pat_off.get(), c->col_idx.get(), val, x, y, sc, pap);
+                         c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
+                         pap);
     else
       hipLaunchKernelGGL((k_spmv<T, true, kViewId32>), grid, block, 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
-                         c->pat_off.get(), c->col_idx.get(), val, x, y, sc, pap);
+                         c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
+                         pap);
   } else {
     hipLaunchKernelGGL((k_spmv<T, false, kViewId32>), grid, block, 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
-                       c->pat_off.get(), c->col_idx.get(), val, x, y, sc, nullptr);
+                       c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
+                       nullptr);
   }
 }
 
