@@ -62,7 +62,8 @@ struct DBuf {
 // Layout of the scalars of one PCG run, kept on the device so that the
 // iteration never waits for the host.
 struct PcgScalars {
-  double rz;      // r.z of the current iterate
+  double rz;      // r.z of the current iterate (after convergence: of the one before)
+  double rzs[2];  // r.z of the iterate of iteration it in slot it & 1
   double pap;     // p.Ap
   double alpha;
   double beta;

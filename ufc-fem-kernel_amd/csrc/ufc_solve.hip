@@ -7,14 +7,14 @@
 // with ||r|| / ||b|| < rtol (||b|| < 1e-30 -> 1), z = r / diag,
 // beta = (r.z)_new / (r.z)_old, p = z + beta p.
 //
-// One iteration is six launches (five for the FP32 view, whose SpMV forms the
-// p.Ap partials itself); every scalar stays on the device:
+// One iteration is five launches (four for the FP32 view, whose SpMV forms
+// the p.Ap partials itself); every scalar stays on the device:
 //   spmv      : Ap = K p
 //   dot_pap   : per-workgroup partials of p.Ap (FP64 CSR only)
 //   fin_alpha : one workgroup adds the partials in index order, alpha
 //   update    : x += alpha p, r -= alpha Ap, partials of r.r and r.(r/diag)
-//   fin_beta  : ||r||/||b||, convergence / breakdown / max_iter, beta
-//   p_update  : p = r/diag + beta p
+//   p_update  : every workgroup adds the r.r and r.z partials: ||r||/||b||,
+//               convergence / breakdown / max_iter, beta; p = r/diag + beta p
 // The ordered two-level sums make every dot product bit-reproducible.  Once
 // `done` is set every later kernel returns at entry, so the host polls only
 // every few iterations while the stopping iteration stays exact.
@@ -128,24 +128,31 @@ static_assert(kSpmvBlocks32 % kFinThreads == 0 && kMaxPartials <= kSpmvBlocks32,
 // partials per value, valid in thread 0.  Thread t adds partials t,
 // t + kFinThreads, ... in that order; all its loads are issued before the
 // first add (a loop of dependent adds would wait for each load in turn).
-template <int NV>
-__device__ __forceinline__ void sum_partials(double (&out)[NV], const double* partials,
-                                             int nblocks) {
+// The same for a workgroup of BS threads reading at most PER partials each.
+template <int NV, int BS, int PER>
+__device__ __forceinline__ void sum_partials_bs(double (&out)[NV], const double* partials,
+                                                int nblocks) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double v[kFinPerThread];
+    double v[PER];
 #pragma unroll
-    for (int j = 0; j < kFinPerThread; ++j) {
-      const int b = int(threadIdx.x) + j * kFinThreads;
+    for (int j = 0; j < PER; ++j) {
+      const int b = int(threadIdx.x) + j * BS;
       v[j] = b < nblocks ? partials[k * kMaxPartials + b] : 0.0;
     }
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < kFinPerThread; ++j)
-      if (int(threadIdx.x) + j * kFinThreads < nblocks) s += v[j];
+    for (int j = 0; j < PER; ++j)
+      if (int(threadIdx.x) + j * BS < nblocks) s += v[j];
     out[k] = s;
   }
-  block_sum<NV, kFinThreads>(out);
+  block_sum<NV, BS>(out);
+}
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(double (&out)[NV], const double* partials,
+                                             int nblocks) {
+  sum_partials_bs<NV, kFinThreads, kFinPerThread>(out, partials, nblocks);
 }
 
 // ---------------------------------------------------------------- SpMV body
@@ -474,9 +481,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_dot_pap(int64_t n, const T* _
   write_partials<1, kSolveThreads>(v, partials);
 }
 
+// alpha of iteration it = (r.z of its iterate) / p.Ap
 __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restrict__ partials,
                                                           int nblocks,
-                                                          PcgScalars* __restrict__ sc) {
+                                                          PcgScalars* __restrict__ sc, int it) {
   if (sc->done) return;
   double s[1];
   sum_partials<1>(s, partials, nblocks);
@@ -486,7 +494,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restr
     if (!(pap > 0.0) || !isfinite(pap))
       sc->done = 2;
     else
-      sc->alpha = sc->rz / pap;
+      sc->alpha = sc->rzs[it & 1] / pap;
   }
 }
 
@@ -528,37 +536,55 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restri
   write_partials<2, kSolveThreads>(v, partials);
 }
 
-__global__ __launch_bounds__(kFinThreads) void k_fin_beta(const double* __restrict__ partials,
-                                                           PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  double s[2];
-  sum_partials<2>(s, partials, kSolveBlocks);
-  if (threadIdx.x == 0) {
-    sc->rr = s[0];
-    sc->iter += 1;
-    const double rn = sqrt(s[0]) / sc->bnorm;
-    sc->rnorm = rn;
-    if (!isfinite(rn)) {
-      sc->done = 2;
-    } else if (rn < sc->rtol) {
-      sc->done = 1;
-    } else if (sc->iter >= sc->max_iter) {
-      sc->done = 3;
-    } else {
-      sc->beta = s[1] / sc->rz;
-      sc->rz = s[1];
-    }
-  }
-}
-
-// p = r/diag + beta p
+// p = r/diag + beta p with the finalize of iteration it folded in: every
+// workgroup adds the update's r.r and r.z partials itself (2 x 2048 doubles
+// from L2, the same ordered sum in every workgroup), forms ||r||/||b||, the
+// stop decision and beta = (r.z)_new / (r.z)_it, and skips the update once
+// the solve stops.  Workgroup 0 publishes the results: iter, rnorm, rr, done
+// and, while running, the new r.z into slot (it + 1) & 1 (read by the next
+// iteration) and rz -- never the slot it & 1 the other workgroups read.  A
+// workgroup that sees done already set returns at entry, which only happens
+// when the solve has stopped.  One launch and one kernel boundary less per
+// iteration than a separate one-workgroup finalize.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restrict__ p,
                                                           const T* __restrict__ r,
                                                           const T* __restrict__ diag,
-                                                          const PcgScalars* __restrict__ sc) {
+                                                          const double* __restrict__ partials,
+                                                          PcgScalars* __restrict__ sc, int it) {
   if (sc->done) return;
-  const T beta = T(sc->beta);
+  double s[2];
+  sum_partials_bs<2, kSolveThreads, kSolveBlocks / kSolveThreads>(s, partials, kSolveBlocks);
+  __shared__ double sh_beta;
+  __shared__ int sh_stop;
+  if (threadIdx.x == 0) {
+    const double rn = sqrt(s[0]) / sc->bnorm;
+    int stop = 0;
+    if (!isfinite(rn))
+      stop = 2;
+    else if (rn < sc->rtol)
+      stop = 1;
+    else if (it + 1 >= sc->max_iter)
+      stop = 3;
+    const double beta = s[1] / sc->rzs[it & 1];
+    sh_beta = beta;
+    sh_stop = stop;
+    if (blockIdx.x == 0) {
+      sc->rr = s[0];
+      sc->iter = it + 1;
+      sc->rnorm = rn;
+      if (stop) {
+        sc->done = stop;
+      } else {
+        sc->beta = beta;
+        sc->rz = s[1];
+        sc->rzs[(it + 1) & 1] = s[1];
+      }
+    }
+  }
+  __syncthreads();
+  if (sh_stop) return;
+  const T beta = T(sh_beta);
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * kSolveThreads) {
     const T d = diag[i];
@@ -610,6 +636,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_setup(const double* __restr
   if (threadIdx.x == 0) {
     const double rz_old = sc->rz;
     sc->rz = s[0];
+    sc->rzs[0] = s[0];
     double bn = sqrt(s[1]);
     if (bn < 1.0e-30) bn = 1.0;
     sc->bnorm = bn;
@@ -819,20 +846,19 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
         launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
         spmv_event_end(c);
         hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream,
-                           c->spmv_partials.get(), spmv_blocks<T>(c), sc);
+                           c->spmv_partials.get(), spmv_blocks<T>(c), sc, launched);
       } else {
         launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
         spmv_event_end(c);
         hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                            p, ap, part, sc);
         hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part,
-                           kSolveBlocks, sc);
+                           kSolveBlocks, sc, launched);
       }
       hipLaunchKernelGGL((k_update<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                          x, r, p, ap, diag, part, sc);
-      hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc);
       hipLaunchKernelGGL(k_pupdate<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                         p, r, diag, sc);
+                         p, r, diag, part, sc, launched);
     }
     UFC_HIP_CHECK(hipGetLastError());
     poll(c);
