@@ -11,8 +11,9 @@
 // the p.Ap partials itself); every scalar stays on the device:
 //   spmv      : Ap = K p
 //   dot_pap   : per-workgroup partials of p.Ap (FP64 CSR only)
-//   fin_alpha : one workgroup adds the partials in index order, alpha
-//   update    : x += alpha p, r -= alpha Ap, partials of r.r and r.(r/diag)
+//   fin_pap   : eight workgroups add the partials in index order (8 pieces)
+//   update    : alpha from the pieces; x += alpha p, r -= alpha Ap, partials
+//               of r.r and r.(r/diag)
 //   p_update  : every workgroup adds the r.r and r.z partials: ||r||/||b||,
 //               convergence / breakdown / max_iter, beta; p = r/diag + beta p
 // The ordered two-level sums make every dot product bit-reproducible.  Once
@@ -481,21 +482,22 @@ __global__ __launch_bounds__(kSolveThreads) void k_dot_pap(int64_t n, const T* _
   write_partials<1, kSolveThreads>(v, partials);
 }
 
-// alpha of iteration it = (r.z of its iterate) / p.Ap
-__global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restrict__ partials,
-                                                          int nblocks,
-                                                          PcgScalars* __restrict__ sc, int it) {
+// p.Ap in kFinPapBlocks pieces: workgroup g adds partials
+// [g chunk, (g + 1) chunk) in index order into pap_piece[g]; the update kernel
+// adds the pieces (in order) and forms alpha.  Eight workgroups read the
+// up-to-32768 SpMV partials in parallel (one workgroup took 8 us at 128^3).
+constexpr int kFinPapBlocks = 8;
+__global__ __launch_bounds__(kFinThreads) void k_fin_pap(const double* __restrict__ partials,
+                                                        int nblocks,
+                                                        double* __restrict__ pap_piece,
+                                                        const PcgScalars* __restrict__ sc) {
   if (sc->done) return;
+  const int chunk = (nblocks + kFinPapBlocks - 1) / kFinPapBlocks;
+  const int lo = min(nblocks, int(blockIdx.x) * chunk);
+  const int hi = min(nblocks, lo + chunk);
   double s[1];
-  sum_partials<1>(s, partials, nblocks);
-  if (threadIdx.x == 0) {
-    const double pap = s[0];
-    sc->pap = pap;
-    if (!(pap > 0.0) || !isfinite(pap))
-      sc->done = 2;
-    else
-      sc->alpha = sc->rzs[it & 1] / pap;
-  }
+  sum_partials<1>(s, partials + lo, hi - lo);
+  if (threadIdx.x == 0) pap_piece[blockIdx.x] = s[0];
 }
 
 // x += alpha p, r -= alpha Ap; partials of r.r and r.z with z = r/diag.
@@ -508,15 +510,33 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_alpha(const double* __restr
 // into the FP64 u when its 0.1 reduction is reached, so the same noise is
 // ~1e-3 of a residual the segment only has to cut by 10.
 template <typename T, typename X>
+// alpha of iteration it = (r.z of its iterate) / p.Ap, with p.Ap the ordered
+// sum of the kFinPapBlocks pieces (the same in every workgroup); p.Ap <= 0 or
+// not finite is a breakdown (done = 2, set by workgroup 0).
 __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restrict__ x,
                                                          T* __restrict__ r,
                                                          const T* __restrict__ p,
                                                          const T* __restrict__ ap,
                                                          const T* __restrict__ diag,
                                                          double* __restrict__ partials,
-                                                         const PcgScalars* __restrict__ sc) {
+                                                         const double* __restrict__ pap_piece,
+                                                         PcgScalars* __restrict__ sc, int it) {
   if (sc->done) return;
-  const double alpha = sc->alpha;
+  double pap = 0.0;
+#pragma unroll
+  for (int g = 0; g < kFinPapBlocks; ++g) pap += pap_piece[g];
+  if (!(pap > 0.0) || !isfinite(pap)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pap = pap;
+      sc->done = 2;
+    }
+    return;
+  }
+  const double alpha = sc->rzs[it & 1] / pap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->pap = pap;
+    sc->alpha = alpha;
+  }
   const T a = T(alpha);
   double rr = 0.0, rz = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
@@ -826,6 +846,7 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
             int* niter, double* rnorm, bool resume = false) {
   const int64_t n = c->n_dof;
   double* part = c->partials.get();
+  double* pap_piece = c->pap_piece.get();
   PcgScalars* sc = c->scalars.get();
   hipLaunchKernelGGL(k_setup<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, b,
                      ax_or_null, r, p, diag, part, int(resume));
@@ -845,18 +866,18 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
         // the FP32 SpMV forms the p.Ap partials itself
         launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
         spmv_event_end(c);
-        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream,
-                           c->spmv_partials.get(), spmv_blocks<T>(c), sc, launched);
+        hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream,
+                           c->spmv_partials.get(), spmv_blocks<T>(c), pap_piece, sc);
       } else {
         launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
         spmv_event_end(c);
         hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                            p, ap, part, sc);
-        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, c->stream, part,
-                           kSolveBlocks, sc, launched);
+        hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream, part,
+                           kSolveBlocks, pap_piece, sc);
       }
       hipLaunchKernelGGL((k_update<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                         x, r, p, ap, diag, part, sc);
+                         x, r, p, ap, diag, part, pap_piece, sc, launched);
       hipLaunchKernelGGL(k_pupdate<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                          p, r, diag, part, sc, launched);
     }
@@ -896,6 +917,7 @@ void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32) {
     c->spmv_partials.resize(size_t(kSpmvBlocks32));
   }
   c->partials.resize(size_t(kMaxPartials) * 4);
+  c->pap_piece.resize(size_t(kFinPapBlocks));
   if (!c->scalars.get()) {
     c->scalars.resize(1);
     UFC_HIP_CHECK(hipMemsetAsync(c->scalars.get(), 0, sizeof(PcgScalars), c->stream));
