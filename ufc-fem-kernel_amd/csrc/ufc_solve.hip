@@ -464,6 +464,30 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
   }
 }
 
+// 16-byte accesses to V consecutive elements starting at a multiple of V
+// (every solver vector is a context buffer, 256-byte aligned); a 4-byte load
+// per lane streams ~40% slower than a 16-byte one on gfx950
+template <typename T, int V>
+__device__ __forceinline__ void load_vec(const T* __restrict__ a, int64_t i0, T (&v)[V]) {
+  static_assert((sizeof(T) * V) % 16 == 0, "whole 16-byte words");
+  constexpr int E = 16 / sizeof(T);
+#pragma unroll
+  for (int h = 0; h < V / E; ++h) {
+    const uint4 q = *reinterpret_cast<const uint4*>(a + i0 + h * E);
+    __builtin_memcpy(&v[h * E], &q, 16);
+  }
+}
+template <typename T, int V>
+__device__ __forceinline__ void store_vec(T* __restrict__ a, int64_t i0, const T (&v)[V]) {
+  constexpr int E = 16 / sizeof(T);
+#pragma unroll
+  for (int h = 0; h < V / E; ++h) {
+    uint4 q;
+    __builtin_memcpy(&q, &v[h * E], 16);
+    *reinterpret_cast<uint4*>(a + i0 + h * E) = q;
+  }
+}
+
 // partials of p.Ap.  A pass of its own: inside the SpMV the per-workgroup
 // reduction's barrier holds every group of a workgroup until the slowest has
 // finished, which cost 0.04 ms per FP32 SpMV at 128^3; re-reading p and Ap
@@ -475,9 +499,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_dot_pap(int64_t n, const T* _
                                                           const PcgScalars* __restrict__ sc) {
   if (sc->done) return;
   double s = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * kSolveThreads)
-    s += double(p[i]) * double(ap[i]);
+  constexpr int V = 16 / sizeof(T);
+  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
+  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
+  const int64_t nvec = n / V;
+  for (int64_t j = tid; j < nvec; j += nth) {
+    T pv[V], av[V];
+    load_vec(p, j * V, pv);
+    load_vec(ap, j * V, av);
+#pragma unroll
+    for (int e = 0; e < V; ++e) s += double(pv[e]) * double(av[e]);
+  }
+  for (int64_t i = nvec * V + tid; i < n; i += nth) s += double(p[i]) * double(ap[i]);
   double v[1] = {s};
   write_partials<1, kSolveThreads>(v, partials);
 }
@@ -509,10 +542,10 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_pap(const double* __restric
 // segment between residual replacements: the segment's correction is flushed
 // into the FP64 u when its 0.1 reduction is reached, so the same noise is
 // ~1e-3 of a residual the segment only has to cut by 10.
-template <typename T, typename X>
 // alpha of iteration it = (r.z of its iterate) / p.Ap, with p.Ap the ordered
 // sum of the kFinPapBlocks pieces (the same in every workgroup); p.Ap <= 0 or
 // not finite is a breakdown (done = 2, set by workgroup 0).
+template <typename T, typename X>
 __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restrict__ x,
                                                          T* __restrict__ r,
                                                          const T* __restrict__ p,
@@ -539,18 +572,41 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restri
   }
   const T a = T(alpha);
   double rr = 0.0, rz = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * kSolveThreads) {
+  auto step = [&](X& xi, T& ri, T pi, T api, T d) {
     if constexpr (sizeof(X) == 8)
-      x[i] = x[i] + alpha * double(p[i]);
+      xi = xi + alpha * double(pi);
     else
-      x[i] = x[i] + a * p[i];
-    const T ri = r[i] - a * ap[i];
-    r[i] = ri;
-    const T d = diag[i];
+      xi = xi + a * pi;
+    ri = ri - a * api;
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
     rr += double(ri) * double(ri);
     rz += double(ri) * double(zi);
+  };
+  // 16-byte chunks of V elements, then the n % V tail element by element
+  constexpr int V = 16 / sizeof(T);
+  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
+  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
+  const int64_t nvec = n / V;
+  for (int64_t j = tid; j < nvec; j += nth) {
+    const int64_t i0 = j * V;
+    X xv[V];
+    T rv[V], pv[V], av[V], dv[V];
+    load_vec(x, i0, xv);
+    load_vec(r, i0, rv);
+    load_vec(p, i0, pv);
+    load_vec(ap, i0, av);
+    load_vec(diag, i0, dv);
+#pragma unroll
+    for (int e = 0; e < V; ++e) step(xv[e], rv[e], pv[e], av[e], dv[e]);
+    store_vec(x, i0, xv);
+    store_vec(r, i0, rv);
+  }
+  for (int64_t i = nvec * V + tid; i < n; i += nth) {
+    X xi = x[i];
+    T ri = r[i];
+    step(xi, ri, p[i], ap[i], diag[i]);
+    x[i] = xi;
+    r[i] = ri;
   }
   double v[2] = {rr, rz};
   write_partials<2, kSolveThreads>(v, partials);
@@ -605,13 +661,25 @@ __global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restr
   __syncthreads();
   if (sh_stop) return;
   const T beta = T(sh_beta);
-  for (int64_t i = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * kSolveThreads) {
-    const T d = diag[i];
-    const T ri = r[i];
+  auto step = [&](T pi, T ri, T d) {
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
-    p[i] = zi + beta * p[i];
+    return zi + beta * pi;
+  };
+  constexpr int V = 16 / sizeof(T);
+  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
+  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
+  const int64_t nvec = n / V;
+  for (int64_t j = tid; j < nvec; j += nth) {
+    const int64_t i0 = j * V;
+    T pv[V], rv[V], dv[V];
+    load_vec(p, i0, pv);
+    load_vec(r, i0, rv);
+    load_vec(diag, i0, dv);
+#pragma unroll
+    for (int e = 0; e < V; ++e) pv[e] = step(pv[e], rv[e], dv[e]);
+    store_vec(p, i0, pv);
   }
+  for (int64_t i = nvec * V + tid; i < n; i += nth) p[i] = step(p[i], r[i], diag[i]);
 }
 
 // r = b - Ax (ax null: x0 = 0), p = z = r/diag (p kept when keep_p);
