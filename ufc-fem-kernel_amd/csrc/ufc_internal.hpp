@@ -75,7 +75,7 @@ struct PcgScalars {
   int32_t iter;   // completed iterations
   int32_t done;   // 0 running, 1 converged, 2 breakdown, 3 max_iter
   int32_t max_iter;
-  int32_t pad;
+  int32_t gate;   // 1: the current iteration's x/p pass does not run
 };
 
 constexpr int kSolveBlocks = 2048;  // vector kernels: 8 workgroups per CU on 256 CUs

@@ -12,10 +12,11 @@
 //   spmv      : Ap = K p
 //   dot_pap   : per-workgroup partials of p.Ap (FP64 CSR only)
 //   fin_pap   : eight workgroups add the partials in index order (8 pieces)
-//   update    : alpha from the pieces; x += alpha p, r -= alpha Ap, partials
-//               of r.r and r.(r/diag)
-//   p_update  : every workgroup adds the r.r and r.z partials: ||r||/||b||,
-//               convergence / breakdown / max_iter, beta; p = r/diag + beta p
+//   update    : alpha from the pieces; r -= alpha Ap, partials of r.r and
+//               r.(r/diag)
+//   xp_update : every workgroup adds the r.r and r.z partials: ||r||/||b||,
+//               convergence / breakdown / max_iter, beta; x += alpha p and
+//               p = r/diag + beta p
 // The ordered two-level sums make every dot product bit-reproducible.  Once
 // `done` is set every later kernel returns at entry, so the host polls only
 // every few iterations while the stopping iteration stays exact.
@@ -544,39 +545,44 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_pap(const double* __restric
 // ~1e-3 of a residual the segment only has to cut by 10.
 // alpha of iteration it = (r.z of its iterate) / p.Ap, with p.Ap the ordered
 // sum of the kFinPapBlocks pieces (the same in every workgroup); p.Ap <= 0 or
-// not finite is a breakdown (done = 2, set by workgroup 0).
-template <typename T, typename X>
-__global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restrict__ x,
-                                                         T* __restrict__ r,
-                                                         const T* __restrict__ p,
+// not finite is a breakdown (done = 2, set by workgroup 0).  r -= alpha Ap
+// and the partials of r.r and r.z with z = r/diag; the iterate is updated by
+// the next pass (k_xpupdate), which reads p anyway.  Workgroup 0 leaves
+// sc->gate = 1 when the iteration does not run (stopped earlier, or this
+// breakdown), so that pass skips it: a flag written in the kernel before it,
+// never by one of its own workgroups.
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restrict__ r,
                                                          const T* __restrict__ ap,
                                                          const T* __restrict__ diag,
                                                          double* __restrict__ partials,
                                                          const double* __restrict__ pap_piece,
                                                          PcgScalars* __restrict__ sc, int it) {
-  if (sc->done) return;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (sc->done) {
+    if (lead) sc->gate = 1;
+    return;
+  }
   double pap = 0.0;
 #pragma unroll
   for (int g = 0; g < kFinPapBlocks; ++g) pap += pap_piece[g];
   if (!(pap > 0.0) || !isfinite(pap)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lead) {
       sc->pap = pap;
       sc->done = 2;
+      sc->gate = 1;
     }
     return;
   }
   const double alpha = sc->rzs[it & 1] / pap;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (lead) {
     sc->pap = pap;
     sc->alpha = alpha;
+    sc->gate = 0;
   }
   const T a = T(alpha);
   double rr = 0.0, rz = 0.0;
-  auto step = [&](X& xi, T& ri, T pi, T api, T d) {
-    if constexpr (sizeof(X) == 8)
-      xi = xi + alpha * double(pi);
-    else
-      xi = xi + a * pi;
+  auto step = [&](T& ri, T api, T d) {
     ri = ri - a * api;
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
     rr += double(ri) * double(ri);
@@ -589,46 +595,50 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, X* __restri
   const int64_t nvec = n / V;
   for (int64_t j = tid; j < nvec; j += nth) {
     const int64_t i0 = j * V;
-    X xv[V];
-    T rv[V], pv[V], av[V], dv[V];
-    load_vec(x, i0, xv);
+    T rv[V], av[V], dv[V];
     load_vec(r, i0, rv);
-    load_vec(p, i0, pv);
     load_vec(ap, i0, av);
     load_vec(diag, i0, dv);
 #pragma unroll
-    for (int e = 0; e < V; ++e) step(xv[e], rv[e], pv[e], av[e], dv[e]);
-    store_vec(x, i0, xv);
+    for (int e = 0; e < V; ++e) step(rv[e], av[e], dv[e]);
     store_vec(r, i0, rv);
   }
   for (int64_t i = nvec * V + tid; i < n; i += nth) {
-    X xi = x[i];
     T ri = r[i];
-    step(xi, ri, p[i], ap[i], diag[i]);
-    x[i] = xi;
+    step(ri, ap[i], diag[i]);
     r[i] = ri;
   }
   double v[2] = {rr, rz};
   write_partials<2, kSolveThreads>(v, partials);
 }
 
-// p = r/diag + beta p with the finalize of iteration it folded in: every
-// workgroup adds the update's r.r and r.z partials itself (2 x 2048 doubles
-// from L2, the same ordered sum in every workgroup), forms ||r||/||b||, the
-// stop decision and beta = (r.z)_new / (r.z)_it, and skips the update once
-// the solve stops.  Workgroup 0 publishes the results: iter, rnorm, rr, done
-// and, while running, the new r.z into slot (it + 1) & 1 (read by the next
-// iteration) and rz -- never the slot it & 1 the other workgroups read.  A
-// workgroup that sees done already set returns at entry, which only happens
-// when the solve has stopped.  One launch and one kernel boundary less per
-// iteration than a separate one-workgroup finalize.
-template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restrict__ p,
-                                                          const T* __restrict__ r,
-                                                          const T* __restrict__ diag,
-                                                          const double* __restrict__ partials,
-                                                          PcgScalars* __restrict__ sc, int it) {
-  if (sc->done) return;
+// x += alpha p (alpha of iteration it), then, with the finalize of iteration
+// it folded in, p = r/diag + beta p: every workgroup adds the update's r.r
+// and r.z partials itself (2 x 2048 doubles from L2, the same ordered sum in
+// every workgroup) and forms ||r||/||b||, the stop decision and
+// beta = (r.z)_new / (r.z)_it.  The iterate is updated in every iteration
+// that ran, the stopping one included; p only while the solve goes on.
+// Workgroup 0 publishes iter, rnorm, rr, done and, while running, the new r.z
+// into slot (it + 1) & 1 (read by the next iteration) and rz -- never the
+// slot it & 1 the other workgroups read.  The entry test reads sc->gate, set
+// by the update kernel before this one, not sc->done, which workgroup 0 may
+// set while the others still have to update x.
+// The iterate X: FP64 for the FP64 solver and for a restarted FP32 inner
+// solve -- an FP32 iterate carries rounding noise of eps32 |x|, which K
+// amplifies to eps32 * lambda_max |x| in the true residual, and for the
+// smooth fields of an elastic solve that stalls a refinement step near
+// eps32 * cond(K) (~1e-3) instead of its 1e-5 tolerance.  FP32 for an inner
+// segment between residual replacements: the segment's correction is flushed
+// into the FP64 u when its 0.1 reduction is reached, so the same noise is
+// ~1e-3 of a residual the segment only has to cut by 10.
+template <typename T, typename X>
+__global__ __launch_bounds__(kSolveThreads) void k_xpupdate(int64_t n, X* __restrict__ x,
+                                                           T* __restrict__ p,
+                                                           const T* __restrict__ r,
+                                                           const T* __restrict__ diag,
+                                                           const double* __restrict__ partials,
+                                                           PcgScalars* __restrict__ sc, int it) {
+  if (sc->gate) return;
   double s[2];
   sum_partials_bs<2, kSolveThreads, kSolveBlocks / kSolveThreads>(s, partials, kSolveBlocks);
   __shared__ double sh_beta;
@@ -659,11 +669,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restr
     }
   }
   __syncthreads();
-  if (sh_stop) return;
+  const bool stop = sh_stop != 0;
+  const double alpha = sc->alpha;
+  const T a = T(alpha);
   const T beta = T(sh_beta);
-  auto step = [&](T pi, T ri, T d) {
+  auto step = [&](X& xi, T& pi, T ri, T d) {
+    if constexpr (sizeof(X) == 8)
+      xi = xi + alpha * double(pi);
+    else
+      xi = xi + a * pi;
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
-    return zi + beta * pi;
+    pi = zi + beta * pi;
   };
   constexpr int V = 16 / sizeof(T);
   const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
@@ -671,15 +687,24 @@ __global__ __launch_bounds__(kSolveThreads) void k_pupdate(int64_t n, T* __restr
   const int64_t nvec = n / V;
   for (int64_t j = tid; j < nvec; j += nth) {
     const int64_t i0 = j * V;
+    X xv[V];
     T pv[V], rv[V], dv[V];
+    load_vec(x, i0, xv);
     load_vec(p, i0, pv);
     load_vec(r, i0, rv);
     load_vec(diag, i0, dv);
 #pragma unroll
-    for (int e = 0; e < V; ++e) pv[e] = step(pv[e], rv[e], dv[e]);
-    store_vec(p, i0, pv);
+    for (int e = 0; e < V; ++e) step(xv[e], pv[e], rv[e], dv[e]);
+    store_vec(x, i0, xv);
+    if (!stop) store_vec(p, i0, pv);
   }
-  for (int64_t i = nvec * V + tid; i < n; i += nth) p[i] = step(p[i], r[i], diag[i]);
+  for (int64_t i = nvec * V + tid; i < n; i += nth) {
+    X xi = x[i];
+    T pi = p[i];
+    step(xi, pi, r[i], diag[i]);
+    x[i] = xi;
+    if (!stop) p[i] = pi;
+  }
 }
 
 // r = b - Ax (ax null: x0 = 0), p = z = r/diag (p kept when keep_p);
@@ -944,10 +969,10 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
         hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream, part,
                            kSolveBlocks, pap_piece, sc);
       }
-      hipLaunchKernelGGL((k_update<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                         x, r, p, ap, diag, part, pap_piece, sc, launched);
-      hipLaunchKernelGGL(k_pupdate<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                         p, r, diag, part, sc, launched);
+      hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, r,
+                         ap, diag, part, pap_piece, sc, launched);
+      hipLaunchKernelGGL((k_xpupdate<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
+                         n, x, p, r, diag, part, sc, launched);
     }
     UFC_HIP_CHECK(hipGetLastError());
     poll(c);
