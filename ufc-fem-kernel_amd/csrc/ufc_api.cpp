@@ -592,6 +592,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     c->diag_pos.resize(size_t(nn) * 3);
     c->slot_elem.resize(size_t(nn) * 8);
     c->slot_code.resize(size_t(nn) * 8);
+    c->slot_loc.resize(size_t(nn));
     auto up = [&](void* d, const void* h, size_t bytes) {
       UFC_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
     };
@@ -623,6 +624,13 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     up(c->diag_pos.get(), diag_pos.data(), c->diag_pos.bytes());
     up(c->slot_elem.get(), s_elem.data(), c->slot_elem.bytes());
     up(c->slot_code.get(), s_code.data(), c->slot_code.bytes());
+    std::vector<uint32_t> s_word(static_cast<size_t>(nn), 0);
+    parallel_for(nn, [&](int64_t n) {
+      uint32_t w = 0;
+      for (int s = 0; s < 8; ++s) w |= uint32_t(s_loc[8 * size_t(n) + s] & 7) << (4 * s);
+      s_word[size_t(n)] = w;
+    });
+    up(c->slot_loc.get(), s_word.data(), c->slot_loc.bytes());
     c->col_idx.release();
     UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
     upload_rhs(c);

@@ -741,26 +741,34 @@ __global__ __launch_bounds__(128) void k_elem_force(int32_t n_elems,
     for (int d = 0; d < 3; ++d) out[3 * a + d] = fe[a][d];
 }
 
-// y[3n+d] = sum over n's elements (ascending) of w[24 e + 3 a + d]
-__global__ void k_node_gather(int32_t n_nodes, const int32_t* __restrict__ slot_elem,
-                              const uint64_t* __restrict__ slot_code,
-                              const double* __restrict__ w, double* __restrict__ y) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n >= n_nodes) return;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    const int32_t e = slot_elem[8 * n + s];
-    if (e < 0) continue;
-    const int a = int((slot_code[8 * n + s] >> 40) & 7);
+// y[3n+d] = sum over n's elements of w[24 e + 3 a + d]: eight lanes per node,
+// lane s on slot s (its element id and n's local index a from the packed
+// slot_loc word), so the slot reads of a wave are one contiguous 256-byte run;
+// the eight contributions are added by a fixed xor tree ((s0+s1)+(s2+s3)) +
+// ((s4+s5)+(s6+s7)) and lanes s = 0..2 write component s.
+__global__ __launch_bounds__(256) void k_node_gather(int32_t n_nodes,
+                                                     const int32_t* __restrict__ slot_elem,
+                                                     const uint32_t* __restrict__ slot_loc,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ y) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = t >> 3;
+  const int s = int(t & 7);
+  const bool live = n < n_nodes;
+  const int64_t nn = live ? n : 0;
+  const int32_t e = slot_elem[8 * nn + s];
+  const int a = int((slot_loc[nn] >> (4 * s)) & 7);
+  double f[3] = {0.0, 0.0, 0.0};
+  if (live && e >= 0) {
     const double* p = w + 24 * int64_t(e) + 3 * a;
-    s0 += p[0];
-    s1 += p[1];
-    s2 += p[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] = p[d];
   }
-  y[3 * n + 0] = s0;
-  y[3 * n + 1] = s1;
-  y[3 * n + 2] = s2;
+#pragma unroll
+  for (int o = 1; o < 8; o <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] += __shfl_xor(f[d], o, 64);
+  if (live && s < 3) y[3 * n + s] = (s == 0) ? f[0] : ((s == 1) ? f[1] : f[2]);
 }
 
 // ---------------------------------------------------------------- variant apply
@@ -1195,8 +1203,9 @@ void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y) {
                        c->coords.get(), c->conn.get(), nullptr, 0.0, nullptr, c->j2_trial.get(),
                        nullptr, c->elem_w.get());
   UFC_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_node_gather, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
-                     c->n_nodes, c->slot_elem.get(), c->slot_code.get(), c->elem_w.get(), y);
+  hipLaunchKernelGGL(k_node_gather, dim3(blocks_for(int64_t(c->n_nodes) * 8, 256)), dim3(256), 0,
+                     c->stream, c->n_nodes, c->slot_elem.get(), c->slot_loc.get(),
+                     c->elem_w.get(), y);
   UFC_HIP_CHECK(hipGetLastError());
 }
 

@@ -184,6 +184,9 @@ struct ufc_ctx {
   // neighbour list of the element's local node b, bits 40..42 = n's local index
   ufc::DBuf<int32_t> slot_elem;   // (n_nodes * 8)
   ufc::DBuf<uint64_t> slot_code;  // (n_nodes * 8)
+  // n's local index in the element of slot s, bits 4s..4s+2 (slot_code bits
+  // 40..42 of all eight slots in one word, for the node gather)
+  ufc::DBuf<uint32_t> slot_loc;   // (n_nodes)
 
   // values
   ufc::DBuf<double> val64;

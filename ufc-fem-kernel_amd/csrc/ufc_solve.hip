@@ -44,6 +44,9 @@
 //         smooth x no longer loses ~eps32 * cond(K) to cancellation: the
 //         refinement residual after one inner solve falls to the inner
 //         tolerance instead of stalling near 1e-3.
+#include <algorithm>
+#include <cmath>
+
 #include "ufc_internal.hpp"
 
 namespace ufc {
