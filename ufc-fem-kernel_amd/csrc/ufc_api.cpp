@@ -242,6 +242,11 @@ int ufc_ctx_create(int32_t device, ufc_ctx** ctx, ufc_status* st) {
       UFC_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       UFC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_scalars), sizeof(ufc::PcgScalars),
                                   hipHostMallocDefault));
+      UFC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_progress), 2 * sizeof(int32_t),
+                                  hipHostMallocMapped | hipHostMallocCoherent));
+      c->h_progress[0] = c->h_progress[1] = 0;
+      UFC_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_progress),
+                                            c->h_progress, 0));
       UFC_HIP_CHECK(hipEventCreate(&c->ev0));
       UFC_HIP_CHECK(hipEventCreate(&c->ev1));
       UFC_HIP_CHECK(hipEventCreate(&c->ev2));
@@ -265,6 +270,7 @@ int ufc_ctx_destroy(ufc_ctx* c, ufc_status* st) {
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev2, c->ev3})
       if (e) (void)hipEventDestroy(e);
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    if (c->h_progress) (void)hipHostFree(c->h_progress);
     hipStream_t s = c->stream;
     delete c;  // frees device buffers
     if (s) (void)hipStreamDestroy(s);

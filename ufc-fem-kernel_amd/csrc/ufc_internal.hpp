@@ -248,6 +248,12 @@ struct ufc_ctx {
   ufc::DBuf<double> pap_piece;           // 8: p.Ap in eight ordered pieces
   ufc::DBuf<ufc::PcgScalars> scalars;    // 1
   ufc::PcgScalars* h_scalars = nullptr;  // pinned mirror
+  // Progress of the running PCG, written by the GPU into pinned, mapped,
+  // coherent host memory with system-scope stores: [0] iterations completed,
+  // [1] stop code (0 while running).  The host reads it without draining the
+  // stream.
+  int32_t* h_progress = nullptr;  // host address
+  int32_t* d_progress = nullptr;  // device address of the same memory
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;

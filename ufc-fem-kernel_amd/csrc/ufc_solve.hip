@@ -44,8 +44,8 @@
 //         smooth x no longer loses ~eps32 * cond(K) to cancellation: the
 //         refinement residual after one inner solve falls to the inner
 //         tolerance instead of stalling near 1e-3.
-#include <algorithm>
-#include <cmath>
+#include <chrono>
+#include <thread>
 
 #include "ufc_internal.hpp"
 
@@ -546,6 +546,11 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_pap(const double* __restric
 // segment between residual replacements: the segment's correction is flushed
 // into the FP64 u when its 0.1 reduction is reached, so the same noise is
 // ~1e-3 of a residual the segment only has to cut by 10.
+// the host-visible progress record (pinned host memory; null: none)
+__device__ __forceinline__ void report(int32_t* prog, int slot, int32_t v) {
+  if (prog) __hip_atomic_store(&prog[slot], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // alpha of iteration it = (r.z of its iterate) / p.Ap, with p.Ap the ordered
 // sum of the kFinPapBlocks pieces (the same in every workgroup); p.Ap <= 0 or
 // not finite is a breakdown (done = 2, set by workgroup 0).  r -= alpha Ap
@@ -560,7 +565,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
                                                          const T* __restrict__ diag,
                                                          double* __restrict__ partials,
                                                          const double* __restrict__ pap_piece,
-                                                         PcgScalars* __restrict__ sc, int it) {
+                                                         PcgScalars* __restrict__ sc, int it,
+                                                         int32_t* prog) {
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (sc->done) {
     if (lead) sc->gate = 1;
@@ -574,6 +580,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
       sc->pap = pap;
       sc->done = 2;
       sc->gate = 1;
+      report(prog, 1, 2);
     }
     return;
   }
@@ -640,7 +647,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_xpupdate(int64_t n, X* __rest
                                                            const T* __restrict__ r,
                                                            const T* __restrict__ diag,
                                                            const double* __restrict__ partials,
-                                                           PcgScalars* __restrict__ sc, int it) {
+                                                           PcgScalars* __restrict__ sc, int it,
+                                                           int32_t* prog) {
   if (sc->gate) return;
   double s[2];
   sum_partials_bs<2, kSolveThreads, kSolveBlocks / kSolveThreads>(s, partials, kSolveBlocks);
@@ -664,11 +672,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_xpupdate(int64_t n, X* __rest
       sc->rnorm = rn;
       if (stop) {
         sc->done = stop;
+        report(prog, 1, stop);
       } else {
         sc->beta = beta;
         sc->rz = s[1];
         sc->rzs[(it + 1) & 1] = s[1];
       }
+      report(prog, 0, it + 1);
     }
   }
   __syncthreads();
@@ -746,7 +756,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_setup(int64_t n, const double
 __global__ __launch_bounds__(kFinThreads) void k_fin_setup(const double* __restrict__ partials,
                                                             PcgScalars* __restrict__ sc,
                                                             double rtol, int32_t max_iter,
-                                                            int replace) {
+                                                            int replace, int32_t* prog) {
   double s[4];
   sum_partials<4>(s, partials, kSolveBlocks);
   if (threadIdx.x == 0) {
@@ -772,6 +782,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin_setup(const double* __restr
       sc->done = 1;
       sc->rnorm = rn0;
     }
+    if (sc->done) report(prog, 1, sc->done);
   }
 }
 
@@ -944,42 +955,79 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
   double* part = c->partials.get();
   double* pap_piece = c->pap_piece.get();
   PcgScalars* sc = c->scalars.get();
+  // Host loop: with check_every > 0, batches of check_every iterations, each
+  // followed by a poll that drains the stream.  By default the GPU reports
+  // every finished iteration (and a stop) into pinned host memory and the
+  // host keeps at most kAhead iterations queued beyond the last one reported:
+  // no poll bubbles (~40 us each) and at most kAhead iterations launched
+  // after convergence (each returns at entry but costs ~55 us of launches).
+  // A report that does not arrive within 50 ms falls back to a poll.  The
+  // stopping iteration is exact either way.
+  constexpr int kAhead = 3;
+  const bool flagged = check_every <= 0;
+  volatile int32_t* prog = c->h_progress;
+  int32_t* dprog = flagged ? c->d_progress : nullptr;
+  prog[0] = 0;
+  prog[1] = 0;
   hipLaunchKernelGGL(k_setup<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, b,
                      ax_or_null, r, p, diag, part, int(resume));
   hipLaunchKernelGGL(k_fin_setup, dim3(1), dim3(kFinThreads), 0, c->stream, part, sc, rtol,
-                     int32_t(max_iter), int(resume));
+                     int32_t(max_iter), int(resume), dprog);
   if (resume)
     hipLaunchKernelGGL(k_preplace<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
                        p, r, diag, sc);
   UFC_HIP_CHECK(hipGetLastError());
-  const int every = check_every > 0 ? check_every : 32;
   const int64_t first_launch = c->spmv_launches;
-  int launched = 0;
-  for (;;) {
-    for (int k = 0; k < every && launched < max_iter; ++k, ++launched) {
-      spmv_event_begin(c);
-      if constexpr (BLOCK_INDEX) {
-        // the FP32 SpMV forms the p.Ap partials itself
-        launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
-        spmv_event_end(c);
-        hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream,
-                           c->spmv_partials.get(), spmv_blocks<T>(c), pap_piece, sc);
-      } else {
-        launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
-        spmv_event_end(c);
-        hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                           p, ap, part, sc);
-        hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream, part,
-                           kSolveBlocks, pap_piece, sc);
+  auto launch_iteration = [&](int it) {
+    spmv_event_begin(c);
+    if constexpr (BLOCK_INDEX) {
+      // the FP32 SpMV forms the p.Ap partials itself
+      launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
+      spmv_event_end(c);
+      hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream,
+                         c->spmv_partials.get(), spmv_blocks<T>(c), pap_piece, sc);
+    } else {
+      launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
+      spmv_event_end(c);
+      hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
+                         p, ap, part, sc);
+      hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream, part,
+                         kSolveBlocks, pap_piece, sc);
+    }
+    hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, r,
+                       ap, diag, part, pap_piece, sc, it, dprog);
+    hipLaunchKernelGGL((k_xpupdate<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
+                       n, x, p, r, diag, part, sc, it, dprog);
+  };
+  if (flagged) {
+    for (int launched = 0; launched < max_iter; ++launched) {
+      if (prog[1] != 0) break;
+      if (launched - prog[0] >= kAhead) {
+        const auto t0 = std::chrono::steady_clock::now();
+        bool drained = false;
+        while (prog[1] == 0 && launched - prog[0] >= kAhead) {
+          std::this_thread::yield();
+          if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+            poll(c);  // no report seen: drain the stream and read the scalars
+            drained = true;
+            break;
+          }
+        }
+        if (prog[1] != 0 || (drained && c->h_scalars->done)) break;
       }
-      hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, r,
-                         ap, diag, part, pap_piece, sc, launched);
-      hipLaunchKernelGGL((k_xpupdate<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
-                         n, x, p, r, diag, part, sc, launched);
+      launch_iteration(launched);
     }
     UFC_HIP_CHECK(hipGetLastError());
     poll(c);
-    if (c->h_scalars->done || launched >= max_iter) break;
+  } else {
+    int launched = 0;
+    for (;;) {
+      for (int k = 0; k < check_every && launched < max_iter; ++k, ++launched)
+        launch_iteration(launched);
+      UFC_HIP_CHECK(hipGetLastError());
+      poll(c);
+      if (c->h_scalars->done || launched >= max_iter) break;
+    }
   }
   *niter = c->h_scalars->iter;
   *rnorm = c->h_scalars->rnorm;
