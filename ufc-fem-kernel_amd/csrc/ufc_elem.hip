@@ -590,51 +590,58 @@ __global__ void k_zero_self(int32_t n_nodes, const int32_t* __restrict__ nb_ptr,
     for (int c = 0; c < 3; ++c) val[base + d * d3 + c] = 0.f;
 }
 
-// Off-diagonal FP32 values: one wave per node copies its rows without the
-// self block (whose column position kself follows from diag_pos) into the
-// compact layout at 9 (nb_ptr[n] - n).
+// Off-diagonal FP32 values: a 32-lane group per node, lane k on neighbour k
+// other than the node itself (column position kself follows from diag_pos),
+// copies its three 3-value runs from the CSR rows into the compact layout at
+// 9 (nb_ptr[n] - n), where it is neighbour k - (k > kself) of 3 (deg - 1)
+// per row.  Both sides are contiguous across the lanes.
 __global__ __launch_bounds__(256) void k_offdiag_compact(int32_t n_nodes,
                                                          const int32_t* __restrict__ nb_ptr,
                                                          const int32_t* __restrict__ diag_pos,
                                                          const float* __restrict__ val,
                                                          float* __restrict__ valm) {
-  const int lane = threadIdx.x & 63;
-  const int64_t nw = int64_t(gridDim.x) * (blockDim.x / 64);
-  for (int64_t n = int64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64; n < n_nodes;
-       n += nw) {
+  const int k = threadIdx.x & 31;
+  const int64_t ng = int64_t(gridDim.x) * (blockDim.x / 32);
+  for (int64_t n = int64_t(blockIdx.x) * (blockDim.x / 32) + threadIdx.x / 32; n < n_nodes;
+       n += ng) {
     const int64_t q0 = nb_ptr[n];
     const int deg = int(nb_ptr[n + 1] - q0);
     const int kself = int((int64_t(diag_pos[3 * n]) - 9 * q0) / 3);
-    const int nv3 = 3 * (deg - 1);
-    for (int o = lane; o < 3 * nv3; o += 64) {
-      const int d = o / nv3;
-      const int rem = o - d * nv3;
-      const int kk = rem / 3;
-      const int cc = rem - 3 * kk;
-      const int k = kk < kself ? kk : kk + 1;
-      valm[9 * (q0 - n) + o] = val[9 * q0 + int64_t(d) * 3 * deg + 3 * k + cc];
-    }
+    if (k >= deg || k == kself) continue;
+    const int kk = k - (k > kself ? 1 : 0);
+    const int64_t src = 9 * q0 + 3 * k, dst = 9 * (q0 - n) + 3 * kk;
+    const int s3 = 3 * deg, d3 = 3 * (deg - 1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) valm[dst + d * d3 + c] = val[src + d * s3 + c];
   }
 }
 
 // pen32 = the self-block diagonal of the difference-form values (the
-// penalty), pen_flag bit j of word w = node 32 w + j has a nonzero one
-__global__ void k_offdiag_penalty(int32_t n_nodes, const int32_t* __restrict__ diag_pos,
-                                  const float* __restrict__ val, float* __restrict__ pen,
-                                  uint32_t* __restrict__ flag) {
-  const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (w * 32 >= n_nodes) return;
-  uint32_t bits = 0;
-  for (int j = 0; j < 32; ++j) {
-    const int64_t n = 32 * w + j;
-    if (n >= n_nodes) break;
+// penalty), pen_flag bit j of word w = node 32 w + j has a nonzero one: one
+// thread per node, the bits of a wave's 64 nodes formed by a ballot (waves
+// start at multiples of 64 nodes, so they fill two whole words)
+__global__ __launch_bounds__(256) void k_offdiag_penalty(int32_t n_nodes,
+                                                         const int32_t* __restrict__ diag_pos,
+                                                         const float* __restrict__ val,
+                                                         float* __restrict__ pen,
+                                                         uint32_t* __restrict__ flag) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool any = false;
+  if (n < n_nodes) {
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       const float p = val[diag_pos[3 * n + d]];
       pen[3 * n + d] = p;
-      if (p != 0.f) bits |= 1u << j;
+      any = any || p != 0.f;
     }
   }
-  flag[w] = bits;
+  const uint64_t bits = __ballot(any);
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (n - lane) / 32;  // first word of this wave
+  if (lane == 0 && n < n_nodes) flag[w] = uint32_t(bits);
+  if (lane == 32 && n < n_nodes) flag[w + 1] = uint32_t(bits >> 32);
 }
 
 // diag[eq[i]] += pen[i]
@@ -1154,9 +1161,10 @@ void launch_fp32_offdiag(ufc_ctx* c) {
   c->pen32.resize(size_t(c->n_dof));
   const int64_t nwords = int64_t(c->n_nodes) / 32 + 1;
   c->pen_flag.resize(size_t(nwords));
-  hipLaunchKernelGGL(k_offdiag_compact, dim3(kSolveBlocks), dim3(256), 0, c->stream, c->n_nodes,
+  const unsigned cgrid = unsigned(std::min<int64_t>(blocks_for(c->n_nodes, 8), 32768));
+  hipLaunchKernelGGL(k_offdiag_compact, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
                      c->nb_ptr.get(), c->diag_pos.get(), c->val32.get(), c->val32m.get());
-  hipLaunchKernelGGL(k_offdiag_penalty, dim3(blocks_for(nwords, 256)), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(k_offdiag_penalty, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
                      c->n_nodes, c->diag_pos.get(), c->val32.get(), c->pen32.get(),
                      c->pen_flag.get());
   UFC_HIP_CHECK(hipGetLastError());
