@@ -50,7 +50,7 @@ def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4):
     return nnz * 4 + index + (n_nodes + 1) * 4 + 2 * n_dof * 4
 
 
-SPMV_EVENT_EVERY = 16
+SPMV_EVENT_EVERY = 64
 
 # kernel name of the FP32 SpMV for each neighbour-index width
 SPMV32_KERNEL = {1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
@@ -238,9 +238,9 @@ def main():
     t2 = time.perf_counter()
     log(f"mesh {t1 - t0:.2f} s, pattern+slot map+loads+BC {t2 - t1:.2f} s, nnz {ctx.nnz}")
     x = torch.zeros(pr.n_dof, dtype=torch.float64, device=dev)
-    # HIP event pairs around every 16th SpMV launch of the timed steps (an
+    # HIP event pairs around every 64th SpMV launch of the timed steps (an
     # event record costs ~5.6 us of queue time; sampling keeps the timed
-    # steps within ~0.2% of untimed ones)
+    # steps within ~0.05% of untimed ones; ~40 samples per mixed step)
     ctx.enable_timing(SPMV_EVENT_EVERY)
     if args.pmc_sample:
         # calibration: a plain copy of 2 GiB (reads 2 GiB, writes 2 GiB)

@@ -1039,10 +1039,16 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
     int nrepl = 0;
     code = UFC_STATUS_INVALID;
     for (int k = 0; k <= max_eval; ++k) {
-      // r = b - K u in FP64, element by element (the refinement residual)
-      apply_fp64_operator(c, u, c->w_t.get());
-      ufc::residual64(c, b, c->w_t.get(), u, r);
-      rnorm = ufc::norm2_64(c, r) / bnorm;
+      // r = b - K u in FP64, element by element (the refinement residual);
+      // from u = 0 the first one is b itself
+      if (k == 0 && s->x0_zero) {
+        UFC_HIP_CHECK(hipMemcpyAsync(r, b, n * 8, hipMemcpyDeviceToDevice, c->stream));
+        rnorm = bnorm0 / bnorm;
+      } else {
+        apply_fp64_operator(c, u, c->w_t.get());
+        ufc::residual64(c, b, c->w_t.get(), u, r);
+        rnorm = ufc::norm2_64(c, r) / bnorm;
+      }
       if (!std::isfinite(rnorm)) {
         code = UFC_STATUS_ERROR;
         break;
