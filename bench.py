@@ -41,13 +41,18 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4):
-    """Algorithmic bytes of one FP32 node-block SpMV: values + node index view
-    (one stencil-pattern byte per node, or 2-byte offsets / 4-byte ids per
-    neighbour) + node pointers + x + y.  The pattern table (at most 256 x 64
-    bytes, cache resident) is not counted."""
-    index = n_nodes if index_bytes == 1 else (nnz // 9) * index_bytes
-    return nnz * 4 + index + (n_nodes + 1) * 4 + 2 * n_dof * 4
+def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4, n_penalty=0):
+    """Algorithmic bytes of one FP32 node-block SpMV.  Stencil-pattern view
+    (index_bytes 1): the off-diagonal values (nnz - 9 n_nodes, the self blocks
+    are not stored) + one pattern byte per node + the penalty diagonal of the
+    n_penalty clamped DOFs; 16-bit offset / 4-byte id views: all nnz values +
+    2 or 4 bytes per neighbour.  Both: node pointers + x + y.  The pattern
+    table (at most 256 x 64 bytes, cache resident) is not counted."""
+    if index_bytes == 1:
+        head = (nnz - 9 * n_nodes) * 4 + n_nodes + n_penalty * 4
+    else:
+        head = nnz * 4 + (nnz // 9) * index_bytes
+    return head + (n_nodes + 1) * 4 + 2 * n_dof * 4
 
 
 SPMV_EVENT_EVERY = 64
@@ -322,7 +327,8 @@ def main():
                          for k, v in sols.items() if k != "fp64"}
     # roofline of the dominant kernel (FP32 SpMV + p.Ap inside the mixed solve)
     spmv_ms = mixed[-1]["spmv_ms"]
-    b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ctx.nb_index_bytes)
+    b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ctx.nb_index_bytes,
+                       int(np.unique(pr.bc_eq).size))
     k32 = SPMV32_KERNEL[ctx.nb_index_bytes]
     ach = b32 / (spmv_ms * 1e6)
     copy_gbs = copy_bandwidth(torch, dev)
