@@ -30,12 +30,15 @@ def _problem_small(kind):
         return P.cube_pressure(6, name="jitter6", jitter=0.1)
     if kind == "irregular":
         return _irregular()
+    if kind == "notched":
+        return _irregular(shuffle=False)
     raise KeyError(kind)
 
 
-def _irregular():
+def _irregular(shuffle=True):
     """Notched, jittered 8x6x5 block with shuffled node and element numbering:
-    node degrees from 1 to 8 elements, ragged rows, no structure in the ids."""
+    node degrees from 1 to 8 elements, ragged rows, no structure in the ids.
+    shuffle=False keeps the box's node order (compacted) and element order."""
     from ufc_hip import problems as P
     base = P.cube_pressure(1, name="irr")  # for the Problem shape only
     import ufc_hip
@@ -48,17 +51,19 @@ def _irregular():
     conn = conn[e[keep].ravel()]
     used = np.unique(conn)
     rng = np.random.default_rng(5)
-    perm = rng.permutation(used.size)          # old id used[i] -> new id perm[i]
-    new_of = np.full(coords.shape[0], -1, np.int64)
+    perm = rng.permutation(used.size) if shuffle else np.arange(used.size)
+    new_of = np.full(coords.shape[0], -1, np.int64)   # old id used[i] -> new id perm[i]
     new_of[used] = perm
     coords_n = np.empty((used.size, 3))
     coords_n[perm] = coords[used]
-    conn_n = new_of[conn].astype(np.int32)[rng.permutation(conn.shape[0])]
+    eorder = rng.permutation(conn.shape[0]) if shuffle else np.arange(conn.shape[0])
+    conn_n = new_of[conn].astype(np.int32)[eorder]
     z0 = np.nonzero(coords_n[:, 2] == 0.0)[0]
     clamp = (3 * z0[:, None] + np.arange(3)).ravel().astype(np.int32)
     top = np.nonzero(np.all(coords_n[conn_n[:, 4:8], 2] == 1.0, axis=1))[0].astype(np.int32)
     young = 1e9 * (1.0 + rng.random(conn_n.shape[0]))
-    base.name, base.coords, base.conn, base.young = "irregular", coords_n, conn_n, young
+    base.name = "irregular" if shuffle else "notched"
+    base.coords, base.conn, base.young = coords_n, conn_n, young
     base.bc_eq, base.bc_val = clamp, np.zeros(clamp.size)
     base.load_elem, base.load_face = top, np.full(top.size, 1, np.int32)
     return base
@@ -183,18 +188,23 @@ def test_pcg_fp64_vs_oracle(ctx, orc, kind):
 
 
 @pytest.mark.parametrize("kind,replace_rtol", [("cube12", 1.0), ("cube12", 0.1),
-                                               ("irregular", 0.1)])
+                                               ("irregular", 0.1), ("notched", 0.1),
+                                               ("cantilever", 0.1)])
 def test_pcg_mixed_refinement_vs_oracle(ctx, orc, kind, replace_rtol):
     """FP32 inner PCG in FP64 refinement, both forms: restarted steps to
     inner_rtol (replace_rtol >= 1), and residual replacement every 0.1 with
-    the direction kept."""
+    the direction kept.  The notched block in box order runs the stencil-
+    pattern view with node degrees 8 to 27; the cantilever has its clamped
+    (penalty) nodes spread over the numbering (x = 0 face)."""
     from ufc_hip import problems as P
-    pr = P.cube_pressure(12, name="cube12") if kind == "cube12" else _irregular()
+    pr = P.cube_pressure(12, name="cube12") if kind == "cube12" else _problem_small(kind)
     rp, ci, val, F, _ = _oracle_system(orc, pr)
     uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
     P.setup_context(ctx, pr, 2)
+    if kind in ("cube12", "notched", "cantilever"):
+        assert ctx.nb_index_bytes == 1, kind
     ctx.assemble_k(pr.young, pr.nu, precision=2)
-    say("pcg mixed cube12 replace_rtol", replace_rtol)
+    say("pcg mixed", kind, "replace_rtol", replace_rtol)
     u, info = ctx.pcg(precision=2, rtol=1e-10, inner_rtol=1e-5, replace_rtol=replace_rtol)
     assert info.status == 0 and info.rnorm < 1e-10 and info.n_outer >= 2
     # true FP64 residual of the oracle's operator
@@ -400,3 +410,30 @@ def test_fp32_views_vs_oracle(ctx, orc):
             ref = u_nat
         else:
             assert np.linalg.norm(u_nat - ref) / np.linalg.norm(ref) < 1e-8, name
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_pcg_progress_record_matches_polling(ctx, precision):
+    """The default host loop (the GPU reports each iteration into pinned host
+    memory, three iterations queued) and fixed polling every 7 iterations run
+    the same kernels: bit-identical solutions and iteration counts.  Event
+    timing of every 5th SpMV samples about a fifth of the launches."""
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(10, name="cube10")
+    P.setup_context(ctx, pr, precision)
+    ctx.assemble_k(pr.young, pr.nu, precision=precision)
+    say("progress record vs polling, precision", precision)
+    ctx.enable_timing(5)
+    u0, i0 = ctx.pcg(precision=precision, rtol=1e-10, replace_rtol=0.1 if precision else 0.0)
+    ctx.enable_timing(0)
+    u1, i1 = ctx.pcg(precision=precision, rtol=1e-10, check_every=7,
+                     replace_rtol=0.1 if precision else 0.0)
+    assert i0.converged and i1.converged
+    assert i0.niter == i1.niter and i0.n_outer == i1.n_outer
+    assert np.array_equal(u0, u1)
+    assert 0.15 * i0.niter <= i0.spmv_calls <= 0.25 * i0.niter
+    if precision == 0:
+        # max_iter stops at exactly max_iter iterations in both loops
+        for ce in (0, 3):
+            _, im = ctx.pcg(precision=0, rtol=1e-14, max_iter=11, check_every=ce)
+            assert im.status == 1 and im.niter == 11
