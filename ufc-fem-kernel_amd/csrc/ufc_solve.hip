@@ -572,6 +572,19 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
     if (lead) sc->gate = 1;
     return;
   }
+  constexpr int V = 16 / sizeof(T);
+  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
+  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
+  const int64_t nvec = n / V;
+  // the thread's first chunk does not depend on alpha: its loads are in
+  // flight while the p.Ap pieces are read
+  T rv0[V], av0[V], dv0[V];
+  const bool first = tid < nvec;
+  if (first) {
+    load_vec(r, tid * V, rv0);
+    load_vec(ap, tid * V, av0);
+    load_vec(diag, tid * V, dv0);
+  }
   double pap = 0.0;
 #pragma unroll
   for (int g = 0; g < kFinPapBlocks; ++g) pap += pap_piece[g];
@@ -599,11 +612,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
     rz += double(ri) * double(zi);
   };
   // 16-byte chunks of V elements, then the n % V tail element by element
-  constexpr int V = 16 / sizeof(T);
-  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
-  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
-  const int64_t nvec = n / V;
-  for (int64_t j = tid; j < nvec; j += nth) {
+  if (first) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) step(rv0[e], av0[e], dv0[e]);
+    store_vec(r, tid * V, rv0);
+  }
+  for (int64_t j = tid + nth; j < nvec; j += nth) {
     const int64_t i0 = j * V;
     T rv[V], av[V], dv[V];
     load_vec(r, i0, rv);
@@ -650,6 +664,21 @@ __global__ __launch_bounds__(kSolveThreads) void k_xpupdate(int64_t n, X* __rest
                                                            PcgScalars* __restrict__ sc, int it,
                                                            int32_t* prog) {
   if (sc->gate) return;
+  constexpr int V = 16 / sizeof(T);
+  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
+  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
+  const int64_t nvec = n / V;
+  // the thread's first chunk does not depend on beta: its loads are in
+  // flight while the partials are added
+  X xv0[V];
+  T pv0[V], rv0[V], dv0[V];
+  const bool first = tid < nvec;
+  if (first) {
+    load_vec(x, tid * V, xv0);
+    load_vec(p, tid * V, pv0);
+    load_vec(r, tid * V, rv0);
+    load_vec(diag, tid * V, dv0);
+  }
   double s[2];
   sum_partials_bs<2, kSolveThreads, kSolveBlocks / kSolveThreads>(s, partials, kSolveBlocks);
   __shared__ double sh_beta;
@@ -694,11 +723,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_xpupdate(int64_t n, X* __rest
     const T zi = (d > T(1e-30) || d < T(-1e-30)) ? ri / d : ri;
     pi = zi + beta * pi;
   };
-  constexpr int V = 16 / sizeof(T);
-  const int64_t tid = int64_t(blockIdx.x) * kSolveThreads + threadIdx.x;
-  const int64_t nth = int64_t(gridDim.x) * kSolveThreads;
-  const int64_t nvec = n / V;
-  for (int64_t j = tid; j < nvec; j += nth) {
+  if (first) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) step(xv0[e], pv0[e], rv0[e], dv0[e]);
+    store_vec(x, tid * V, xv0);
+    if (!stop) store_vec(p, tid * V, pv0);
+  }
+  for (int64_t j = tid + nth; j < nvec; j += nth) {
     const int64_t i0 = j * V;
     X xv[V];
     T pv[V], rv[V], dv[V];
