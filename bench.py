@@ -47,7 +47,13 @@ def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4, n_penalty=0):
     are not stored) + one pattern byte per node + the penalty diagonal of the
     n_penalty clamped DOFs; 16-bit offset / 4-byte id views: all nnz values +
     2 or 4 bytes per neighbour.  Both: node pointers + x + y.  The pattern
-    table (at most 256 x 64 bytes, cache resident) is not counted."""
+    table (at most 256 x 64 bytes, cache resident) is not counted.
+    Symmetric block-diagonal view (index_bytes 0): each off-diagonal block
+    once, (nnz - 9 n_nodes) / 2 values (its second use, transposed for the
+    other node, is a cache re-read and not counted), the penalty diagonal,
+    one flag bit per node, x + y; no node pointers."""
+    if index_bytes == 0:
+        return (nnz - 9 * n_nodes) // 2 * 4 + n_penalty * 4 + n_nodes // 8 + 2 * n_dof * 4
     if index_bytes == 1:
         head = (nnz - 9 * n_nodes) * 4 + n_nodes + n_penalty * 4
     else:
@@ -58,8 +64,12 @@ def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4, n_penalty=0):
 SPMV_EVENT_EVERY = 64
 
 # kernel name of the FP32 SpMV for each neighbour-index width
-SPMV32_KERNEL = {1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
+SPMV32_KERNEL = {0: "k_spmv_sym", 1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
                  4: "k_spmv<float, true, 0>"}
+SPMV32_FORM = {0: "FP32 symmetric block-diagonal SpMV, difference form",
+               1: "FP32 stencil-pattern node-block SpMV, difference form",
+               2: "FP32 node-block SpMV (16-bit offsets), difference form",
+               4: "FP32 node-block SpMV (4-byte ids), difference form"}
 
 
 def spmv64_bytes(n_dof, nnz):
@@ -335,7 +345,7 @@ def main():
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(pr.n_dof, k32),
                        "copy_gbs": copy_gbs, "frac_of_copy": ach / copy_gbs,
-                       "kernel": k32 + " (FP32 node-block SpMV, difference form)",
+                       "kernel": k32 + " (" + SPMV32_FORM[ctx.nb_index_bytes] + ")",
                        "bytes_per_launch": b32, "ms_per_launch": spmv_ms,
                        "traffic_source": "profiles/spmv_pcg128_hbm_traffic.md (rocprofv3 "
                                          "FETCH_SIZE / WRITE_SIZE, copy-calibrated)"}

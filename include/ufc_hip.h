@@ -102,14 +102,29 @@ typedef struct ufc_rt_asm_info {
   int32_t n_nodes;
   int32_t n_elems;
   int32_t max_row_nnz;
-  int32_t nb_index_bytes; /* neighbour index of the FP32 SpMV view: 1 (one  */
-                          /* stencil-pattern byte per node) when every       */
-                          /* |node - neighbour| < 2^15 and at most 256       */
-                          /* distinct neighbour-offset lists occur, else 2   */
-                          /* (16-bit offset per neighbour) when the offsets  */
-                          /* fit, else 4 (node id per neighbour)             */
+  int32_t nb_index_bytes; /* neighbour index bytes per node or neighbour    */
+                          /* the FP32 SpMV view reads: 0 (symmetric block   */
+                          /* diagonals), 1 (stencil pattern), 2, 4          */
+  int32_t fp32_view;      /* UFC_FP32_VIEW_*: by default the fastest the    */
+                          /* mesh allows, in the order SYMDIAG, PATTERN,    */
+                          /* OFF16, ID32                                     */
+  int32_t reserved;
 } ufc_rt_asm_info;
 int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
+
+/* Index views of the FP32 operator (the FP32 SpMV inside FP32 and MIXED
+ * solves and ufc_nm_spmv in FP32).  All apply the same difference-form K:
+ *   SYMDIAG: when the offsets m - n > 0 of all neighbour pairs take at most
+ *            13 distinct values (a lexicographically numbered structured
+ *            box), each off-diagonal block is stored once, for its upper
+ *            node, and applied as K(n,m) and K(n,m)^T;
+ *   PATTERN: at most 256 distinct neighbour-offset lists, all offsets below
+ *            2^15: one pattern byte per node;
+ *   OFF16:   16-bit offset per neighbour;  ID32: node id per neighbour. */
+#define UFC_FP32_VIEW_ID32 0
+#define UFC_FP32_VIEW_OFF16 1
+#define UFC_FP32_VIEW_PATTERN 2
+#define UFC_FP32_VIEW_SYMDIAG 3
 
 /* ====================================================================== */
 /* L4_PH: element kernels                                                 */
@@ -280,6 +295,13 @@ int ufc_rt_asm_load(ufc_ctx* ctx, const ufc_rt_load_args* args, ufc_status* st);
  * ufc_rt_residual, whose operators come from the elements, return
  * UFC_STATUS_INVALID until the next ufc_rt_asm_assemble_k. */
 int ufc_rt_asm_set_values(ufc_ctx* ctx, int32_t precision, const double* values, ufc_status* st);
+
+/* Selects the FP32 view (UFC_FP32_VIEW_*, or -1 for the fastest the mesh
+ * allows, the default after ufc_rt_asm_build_pattern); assembled FP32
+ * values are re-laid out at once.  UFC_STATUS_UNSUPPORTED if the mesh does
+ * not allow the view.  No counterpart in the reference, whose SpMV has one
+ * storage format: for tests and measurements of the individual views. */
+int ufc_rt_asm_set_fp32_view(ufc_ctx* ctx, int32_t view, ufc_status* st);
 
 /* Sets / reads the load vector F (n_dof); penalty terms of apply_bc are
  * included when read. */

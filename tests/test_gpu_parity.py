@@ -35,6 +35,14 @@ def _problem_small(kind):
     raise KeyError(kind)
 
 
+def _n_diagonals(conn):
+    """Distinct offsets m - n > 0 over the node pairs that share an element:
+    at most 13 allow the symmetric block-diagonal FP32 view."""
+    c = np.asarray(conn, np.int64)
+    d = (c[:, None, :] - c[:, :, None]).ravel()
+    return np.unique(d[d > 0]).size
+
+
 def _irregular(shuffle=True):
     """Notched, jittered 8x6x5 block with shuffled node and element numbering:
     node degrees from 1 to 8 elements, ragged rows, no structure in the ids.
@@ -187,24 +195,37 @@ def test_pcg_fp64_vs_oracle(ctx, orc, kind):
     assert np.linalg.norm(u2 - uo2) / np.linalg.norm(uo2) < 1e-10
 
 
-@pytest.mark.parametrize("kind,replace_rtol", [("cube12", 1.0), ("cube12", 0.1),
-                                               ("irregular", 0.1), ("notched", 0.1),
-                                               ("cantilever", 0.1)])
-def test_pcg_mixed_refinement_vs_oracle(ctx, orc, kind, replace_rtol):
+@pytest.mark.parametrize("kind,replace_rtol,view", [("cube12", 1.0, None), ("cube12", 0.1, None),
+                                                    ("cube12", 0.1, "pattern"),
+                                                    ("irregular", 0.1, None),
+                                                    ("notched", 0.1, None),
+                                                    ("cantilever", 0.1, None),
+                                                    ("cantilever", 0.1, "pattern")])
+def test_pcg_mixed_refinement_vs_oracle(ctx, orc, kind, replace_rtol, view):
     """FP32 inner PCG in FP64 refinement, both forms: restarted steps to
     inner_rtol (replace_rtol >= 1), and residual replacement every 0.1 with
-    the direction kept.  The notched block in box order runs the stencil-
-    pattern view with node degrees 8 to 27; the cantilever has its clamped
-    (penalty) nodes spread over the numbering (x = 0 face)."""
+    the direction kept.  The structured cube and cantilever run the symmetric
+    block-diagonal view by default and the stencil-pattern view when asked;
+    the notched block in box order runs the stencil-pattern view with node
+    degrees 8 to 27; the cantilever has its clamped (penalty) nodes spread
+    over the numbering (x = 0 face)."""
+    from ufc_hip import _abi as A
     from ufc_hip import problems as P
     pr = P.cube_pressure(12, name="cube12") if kind == "cube12" else _problem_small(kind)
     rp, ci, val, F, _ = _oracle_system(orc, pr)
     uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=200000)
     P.setup_context(ctx, pr, 2)
-    if kind in ("cube12", "notched", "cantilever"):
-        assert ctx.nb_index_bytes == 1, kind
+    if kind in ("cube12", "cantilever"):
+        assert _n_diagonals(pr.conn) == 13
+        assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG and ctx.nb_index_bytes == 0, kind
+    if kind == "notched":
+        assert _n_diagonals(pr.conn) > 13
+        assert ctx.fp32_view == A.FP32_VIEW_PATTERN and ctx.nb_index_bytes == 1, kind
+    if view == "pattern":
+        assert ctx.set_fp32_view(A.FP32_VIEW_PATTERN) == A.FP32_VIEW_PATTERN
+        assert ctx.nb_index_bytes == 1
     ctx.assemble_k(pr.young, pr.nu, precision=2)
-    say("pcg mixed", kind, "replace_rtol", replace_rtol)
+    say("pcg mixed", kind, "replace_rtol", replace_rtol, "view", ctx.fp32_view)
     u, info = ctx.pcg(precision=2, rtol=1e-10, inner_rtol=1e-5, replace_rtol=replace_rtol)
     assert info.status == 0 and info.rnorm < 1e-10 and info.n_outer >= 2
     # true FP64 residual of the oracle's operator
@@ -358,8 +379,12 @@ def test_fp32_view_wide_numbering_vs_oracle(ctx, orc):
     u, info = ctx.pcg(precision=2, rtol=1e-10)
     assert info.converged
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
-    # the structured numbering takes the stencil-pattern view (27 patterns)
+    # the structured numbering takes the symmetric block-diagonal view (13
+    # diagonals); the stencil-pattern view (27 patterns) is there on request
+    from ufc_hip import _abi as A
     P.setup_context(ctx, P.cube_pressure(8, name="cube8"), 2)
+    assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG and ctx.nb_index_bytes == 0
+    assert ctx.set_fp32_view(A.FP32_VIEW_PATTERN) == A.FP32_VIEW_PATTERN
     assert ctx.nb_index_bytes == 1
 
 
@@ -374,26 +399,31 @@ def _renumbered(pr, perm):
 
 def test_fp32_views_vs_oracle(ctx, orc):
     """Neighbour indexes of the FP32 view on one 16^3 cube: the structured
-    numbering (one stencil-pattern byte per node, 27 patterns), a numbering
+    numbering (13 block diagonals, each off-diagonal block stored once; and,
+    on request, one stencil-pattern byte per node, 27 patterns), a numbering
     shuffled inside windows of 64 ids and a random numbering (both more than
     256 patterns, offsets below 2^15: 16-bit offsets; the 4-byte ids are
     covered on a 32^3 cube above).  Each SpMV, and the mixed solve, matches
     the oracle; the shuffled numberings give the structured solution back
     after un-permuting."""
+    from ufc_hip import _abi as A
     from ufc_hip import problems as P
     n = 16
     base = P.cube_pressure(n, name="cube16")
     nn = base.coords.shape[0]
     rng = np.random.default_rng(5)
     window = np.concatenate([w + rng.permutation(min(64, nn - w)) for w in range(0, nn, 64)])
-    cases = [("pattern", np.arange(nn), 1), ("window", window, 2),
-             ("random", rng.permutation(nn), 2)]
+    cases = [("symdiag", np.arange(nn), 0, None),
+             ("pattern", np.arange(nn), 1, A.FP32_VIEW_PATTERN),
+             ("window", window, 2, None), ("random", rng.permutation(nn), 2, None)]
     ref = None
-    for name, perm, width in cases:
+    for name, perm, width, force in cases:
         pr = _renumbered(P.cube_pressure(n, name="cube16"), perm)
         rp, ci, val, F, _ = _oracle_system(orc, pr)
         say("fp32 view", name)
         P.setup_context(ctx, pr, 2)
+        if force is not None:
+            ctx.set_fp32_view(force)
         assert ctx.nb_index_bytes == width, name
         ctx.assemble_k(pr.young, pr.nu, precision=1)
         x = np.random.default_rng(2).normal(size=pr.n_dof)
@@ -410,6 +440,65 @@ def test_fp32_views_vs_oracle(ctx, orc):
             ref = u_nat
         else:
             assert np.linalg.norm(u_nat - ref) / np.linalg.norm(ref) < 1e-8, name
+
+
+def test_fp32_symdiag_view(ctx, orc):
+    """The symmetric block-diagonal FP32 view.  A beam one element thick in y
+    and z has 10 distinct neighbour offsets (fewer than the 13 the kernel
+    unrolls: the missing diagonals must contribute nothing); its SpMV and
+    mixed solve match the oracle.  On a cube the view is switched after the
+    FP32 assembly: the stencil-pattern view's K x agrees to FP32 rounding,
+    and switching back gives the symmetric view's K x bit for bit.  A
+    numbering without the diagonal structure rejects the view
+    (UFC_STATUS_UNSUPPORTED), an unknown view id is invalid, and the
+    rejections leave the view unchanged."""
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    pr = P.cantilever(nx=12, ny=1, nz=1)
+    assert _n_diagonals(pr.conn) == 10
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 2)
+    assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    x = np.random.default_rng(3).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    y32 = ctx.spmv(x, precision=1)
+    say("symdiag beam spmv", np.abs(y32 - yo).max() / np.abs(yo).max())
+    assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    u, info = ctx.pcg(precision=2, rtol=1e-10)
+    assert info.converged
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+
+    pr = P.cube_pressure(10, name="cube10")
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 2)
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    x = np.random.default_rng(4).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    y_sym = ctx.spmv(x, precision=1)
+    assert ctx.set_fp32_view(A.FP32_VIEW_PATTERN) == A.FP32_VIEW_PATTERN
+    y_pat = ctx.spmv(x, precision=1)
+    assert ctx.set_fp32_view(-1) == A.FP32_VIEW_SYMDIAG
+    y_sym2 = ctx.spmv(x, precision=1)
+    scale = np.abs(yo).max()
+    assert np.abs(y_sym - yo).max() <= 1e-5 * scale
+    assert np.abs(y_pat - yo).max() <= 1e-5 * scale
+    assert np.array_equal(y_sym, y_sym2)
+    with pytest.raises(A.UfcError) as e:
+        ctx.set_fp32_view(7)
+    assert e.value.code == A.STATUS_INVALID
+    assert ctx.info().fp32_view == A.FP32_VIEW_SYMDIAG
+
+    perm = np.random.default_rng(6).permutation(pr.coords.shape[0])
+    pr = _renumbered(P.cube_pressure(10, name="cube10"), perm)
+    P.setup_context(ctx, pr, 2)
+    assert ctx.fp32_view == A.FP32_VIEW_OFF16
+    with pytest.raises(A.UfcError) as e:
+        ctx.set_fp32_view(A.FP32_VIEW_SYMDIAG)
+    assert e.value.code == A.STATUS_UNSUPPORTED
+    assert ctx.info().fp32_view == A.FP32_VIEW_OFF16
 
 
 @pytest.mark.parametrize("precision", [0, 2])

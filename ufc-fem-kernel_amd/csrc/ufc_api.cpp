@@ -3,6 +3,7 @@
 // the solver orchestration (FP64 / FP32 PCG, FP32-in-FP64 iterative
 // refinement, adjoint).  Compiled as HIP host code.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -149,9 +150,82 @@ void apply_fp64_operator(ufc_ctx* c, const double* u, double* y) {
     ufc::launch_apply_elem64(c, 0, u, y);
 }
 
-// Width of the FP32 view's neighbour index: 1 (one stencil-pattern byte per
-// node), 2 (16-bit offset per neighbour) or 4 (node id per neighbour).
-int32_t nb_index_bytes(const ufc_ctx* c) { return c->n_pat > 0 ? 1 : (c->have_off16 ? 2 : 4); }
+// Width of the FP32 view's neighbour index: 0 (symmetric block diagonals:
+// the offsets are kernel arguments), 1 (one stencil-pattern byte per node),
+// 2 (16-bit offset per neighbour) or 4 (node id per neighbour).
+int32_t nb_index_bytes(const ufc_ctx* c) {
+  switch (c->fp32_view) {
+    case ufc::kFp32ViewSymDiag:
+      return 0;
+    case ufc::kFp32ViewPattern:
+      return 1;
+    case ufc::kFp32ViewOff16:
+      return 2;
+    default:
+      return 4;
+  }
+}
+
+bool fp32_view_available(const ufc_ctx* c, int view) {
+  switch (view) {
+    case ufc::kFp32ViewSymDiag:
+      return c->n_diag > 0;
+    case ufc::kFp32ViewPattern:
+      return c->n_pat > 0;
+    case ufc::kFp32ViewOff16:
+      return c->have_off16;
+    case ufc::kFp32ViewId32:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Offsets of the symmetric block-diagonal view: the distinct m - n > 0 over
+// all neighbour pairs (n, m).  More than kMaxDiags of them: no such view.
+// Chunks of nodes collect their offsets in parallel and stop early once a
+// chunk alone has too many.
+void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx) {
+  c->n_diag = 0;
+  c->diag_a.release();
+  c->diag_b.release();
+  c->diag_c.release();
+  const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
+  constexpr int64_t kChunk = 4096;
+  const int64_t nchunk = (nn + kChunk - 1) / kChunk;
+  constexpr int kCap = ufc::kMaxDiags + 1;
+  std::vector<std::array<int32_t, kCap>> found(static_cast<size_t>(nchunk));
+  std::vector<int> count(static_cast<size_t>(nchunk), 0);
+  parallel_for(nchunk, [&](int64_t ch) {
+    auto& f = found[size_t(ch)];
+    int& k = count[size_t(ch)];
+    for (int64_t n = ch * kChunk; n < std::min(nn, (ch + 1) * kChunk) && k < kCap; ++n)
+      for (int32_t q = c->h_nb_ptr[size_t(n)]; q < c->h_nb_ptr[size_t(n) + 1] && k < kCap; ++q) {
+        const int64_t o = int64_t(nb_idx[size_t(q)]) - n;
+        if (o <= 0) continue;
+        if (std::find(f.begin(), f.begin() + k, int32_t(o)) == f.begin() + k) f[size_t(k++)] = int32_t(o);
+      }
+  });
+  std::vector<int32_t> all;
+  for (int64_t ch = 0; ch < nchunk; ++ch) {
+    for (int j = 0; j < count[size_t(ch)]; ++j)
+      if (std::find(all.begin(), all.end(), found[size_t(ch)][size_t(j)]) == all.end())
+        all.push_back(found[size_t(ch)][size_t(j)]);
+    if (int(all.size()) > ufc::kMaxDiags) return;
+  }
+  if (all.empty()) return;
+  std::sort(all.begin(), all.end());
+  // front zeros >= the largest offset, rounded to 64 entries; each diagonal
+  // a whole number of 64-entry blocks; the SpMV indexes the entries with
+  // 32-bit integers
+  const int64_t pad = (int64_t(all.back()) + 63) / 64 * 64;
+  const int64_t stride = (pad + nn + 63) / 64 * 64;
+  if (int64_t(all.size()) * stride >= (int64_t(1) << 31)) return;
+  c->n_diag = int32_t(all.size());
+  for (int s = 0; s < ufc::kMaxDiags; ++s) c->h_diag_off[s] = s < c->n_diag ? all[size_t(s)] : 0;
+  c->diag_pad = int32_t(pad);
+  c->diag_stride = stride;
+}
 
 // Stencil-pattern dictionary of the 16-bit view (off16, CSR pair order):
 // every node's (degree, offsets) list gets an id; with at most kMaxPatterns
@@ -226,6 +300,33 @@ void build_stencil_patterns(ufc_ctx* c, const std::vector<int16_t>& off16) {
 
 }  // namespace
 
+namespace ufc {
+
+// The fastest view first: symmetric block diagonals (each off-diagonal block
+// read once from HBM), then stencil patterns, 16-bit offsets, 4-byte ids.
+// The off-diagonal copy of the pattern view is dropped when another view is
+// chosen, and the block diagonals likewise.
+void select_fp32_view(ufc_ctx* c) {
+  int v = c->fp32_view_request;
+  if (v < 0 || !fp32_view_available(c, v)) {
+    v = kFp32ViewId32;
+    for (int w : {kFp32ViewSymDiag, kFp32ViewPattern, kFp32ViewOff16})
+      if (fp32_view_available(c, w)) {
+        v = w;
+        break;
+      }
+  }
+  c->fp32_view = v;
+  if (v != kFp32ViewPattern) c->val32m.release();
+  if (v != kFp32ViewSymDiag) {
+    c->diag_a.release();
+    c->diag_b.release();
+    c->diag_c.release();
+  }
+}
+
+}  // namespace ufc
+
 extern "C" {
 
 // ======================================================================= ctx
@@ -289,6 +390,28 @@ int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
     info->n_elems = c->n_elems;
     info->max_row_nnz = c->max_row_nnz;
     info->nb_index_bytes = nb_index_bytes(c);
+    info->fp32_view = c->fp32_view;
+    return UFC_STATUS_OK;
+  });
+}
+
+int ufc_rt_asm_set_fp32_view(ufc_ctx* c, int32_t view, ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_set_fp32_view", [&](std::string& msg) {
+    check_mesh(c);
+    UFC_REQUIRE(view >= -1 && view <= ufc::kFp32ViewSymDiag, UFC_STATUS_INVALID,
+                "view must be -1 (fastest) or 0-3");
+    UFC_REQUIRE(view < 0 || fp32_view_available(c, view), UFC_STATUS_UNSUPPORTED,
+                "the mesh does not allow FP32 view " + std::to_string(view));
+    UFC_HIP_CHECK(hipSetDevice(c->device));
+    c->fp32_view_request = view;
+    const int before = c->fp32_view;
+    ufc::select_fp32_view(c);
+    // FP32 values already assembled: build the new view's layout from them
+    if (c->have32 && c->fp32_view != before) {
+      ufc::launch_fp32_offdiag(c);
+      UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    msg = "FP32 view " + std::to_string(c->fp32_view);
     return UFC_STATUS_OK;
   });
 }
@@ -626,6 +749,9 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       c->nb_off16.release();
     }
     build_stencil_patterns(c, off16);
+    build_sym_diagonals(c, nb_idx);
+    c->fp32_view_request = -1;
+    ufc::select_fp32_view(c);
     up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
     up(c->diag_pos.get(), diag_pos.data(), c->diag_pos.bytes());
     up(c->slot_elem.get(), s_elem.data(), c->slot_elem.bytes());
@@ -647,6 +773,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       info->n_elems = ne;
       info->max_row_nnz = c->max_row_nnz;
       info->nb_index_bytes = nb_index_bytes(c);
+      info->fp32_view = c->fp32_view;
     }
     msg = "pattern built: nnz " + std::to_string(nnz);
     return UFC_STATUS_OK;

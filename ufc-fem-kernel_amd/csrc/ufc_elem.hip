@@ -618,6 +618,47 @@ __global__ __launch_bounds__(256) void k_offdiag_compact(int32_t n_nodes,
   }
 }
 
+// Symmetric block-diagonal values from the difference-form val32: a 32-lane
+// group per node, lane k on neighbour m = nb_idx[q0 + k] > n copies block
+// K(n, m) (node n's rows d, columns 3 m + c) to entry pad + n of diagonal s,
+// o_s = m - n, as float4 (0,0) (0,1) (0,2) (1,0), float4 (1,1) (1,2) (2,0)
+// (2,1) and float (2,2).  The pairs of the pattern are fixed, so the entries
+// never written stay zero from the allocation.
+__global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
+                                                   const int32_t* __restrict__ nb_ptr,
+                                                   const int32_t* __restrict__ nb_idx,
+                                                   const float* __restrict__ val, DiagOffsets off,
+                                                   int nd, int32_t pad, int64_t stride,
+                                                   float4* __restrict__ da,
+                                                   float4* __restrict__ db,
+                                                   float* __restrict__ dc) {
+  const int k = threadIdx.x & 31;
+  const int64_t ng = int64_t(gridDim.x) * (blockDim.x / 32);
+  for (int64_t n = int64_t(blockIdx.x) * (blockDim.x / 32) + threadIdx.x / 32; n < n_nodes;
+       n += ng) {
+    const int64_t q0 = nb_ptr[n];
+    const int deg = int(nb_ptr[n + 1] - q0);
+    if (k >= deg) continue;
+    const int64_t o = int64_t(nb_idx[q0 + k]) - n;
+    if (o <= 0) continue;
+    int s = -1;
+#pragma unroll
+    for (int j = 0; j < kMaxDiags; ++j)
+      if (j < nd && off.o[j] == o) s = j;
+    if (s < 0) continue;  // not one of the pattern's offsets (cannot occur)
+    const int64_t src = 9 * q0 + 3 * k, s3 = 3 * deg;
+    float v[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) v[3 * d + cc] = val[src + d * s3 + cc];
+    const int64_t e = int64_t(s) * stride + pad + n;
+    da[e] = make_float4(v[0], v[1], v[2], v[3]);
+    db[e] = make_float4(v[4], v[5], v[6], v[7]);
+    dc[e] = v[8];
+  }
+}
+
 // pen32 = the self-block diagonal of the difference-form values (the
 // penalty), pen_flag bit j of word w = node 32 w + j has a nonzero one: one
 // thread per node, the bits of a wave's 64 nodes formed by a ballot (waves
@@ -1155,15 +1196,35 @@ void launch_fp32_difference_form(ufc_ctx* c) {
 }
 
 void launch_fp32_offdiag(ufc_ctx* c) {
-  if (c->n_pat == 0) return;
-  const int64_t nv = c->nnz - 9 * int64_t(c->n_nodes);
-  c->val32m.resize(size_t(nv));
+  const bool pattern = c->fp32_view == kFp32ViewPattern, sym = c->fp32_view == kFp32ViewSymDiag;
+  if (!pattern && !sym) return;
   c->pen32.resize(size_t(c->n_dof));
   const int64_t nwords = int64_t(c->n_nodes) / 32 + 1;
   c->pen_flag.resize(size_t(nwords));
   const unsigned cgrid = unsigned(std::min<int64_t>(blocks_for(c->n_nodes, 8), 32768));
-  hipLaunchKernelGGL(k_offdiag_compact, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
-                     c->nb_ptr.get(), c->diag_pos.get(), c->val32.get(), c->val32m.get());
+  if (pattern) {
+    const int64_t nv = c->nnz - 9 * int64_t(c->n_nodes);
+    c->val32m.resize(size_t(nv));
+    hipLaunchKernelGGL(k_offdiag_compact, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
+                       c->nb_ptr.get(), c->diag_pos.get(), c->val32.get(), c->val32m.get());
+  } else {
+    const size_t entries = size_t(c->n_diag) * size_t(c->diag_stride);
+    if (!c->diag_c.get()) {
+      c->diag_a.resize(4 * entries);
+      c->diag_b.resize(4 * entries);
+      c->diag_c.resize(entries);
+      UFC_HIP_CHECK(hipMemsetAsync(c->diag_a.get(), 0, c->diag_a.bytes(), c->stream));
+      UFC_HIP_CHECK(hipMemsetAsync(c->diag_b.get(), 0, c->diag_b.bytes(), c->stream));
+      UFC_HIP_CHECK(hipMemsetAsync(c->diag_c.get(), 0, c->diag_c.bytes(), c->stream));
+    }
+    DiagOffsets off;
+    for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
+    hipLaunchKernelGGL(k_diag_fill, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
+                       c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), off, int(c->n_diag),
+                       c->diag_pad, c->diag_stride,
+                       reinterpret_cast<float4*>(c->diag_a.get()),
+                       reinterpret_cast<float4*>(c->diag_b.get()), c->diag_c.get());
+  }
   hipLaunchKernelGGL(k_offdiag_penalty, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
                      c->n_nodes, c->diag_pos.get(), c->val32.get(), c->pen32.get(),
                      c->pen_flag.get());

@@ -88,6 +88,15 @@ constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vecto
 constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
 constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
 constexpr int kMaxPatterns = 256;   // stencil patterns of the FP32 view (one-byte ids)
+constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP32 view
+constexpr int kDiagThreads = 256;   // symmetric FP32 SpMV: one node per thread
+// FP32 SpMV views (ufc_rt_asm_info.fp32_view)
+constexpr int kFp32ViewId32 = 0, kFp32ViewOff16 = 1, kFp32ViewPattern = 2, kFp32ViewSymDiag = 3;
+
+// offsets o_s of the symmetric FP32 view's block diagonals (kernel argument)
+struct DiagOffsets {
+  int32_t o[kMaxDiags];
+};
 
 // C3D8 variants, PH_ELEM_C3D8_VARIANT_* (PH_Elem_C3D8.f90:111-116)
 constexpr int kVarFull = 1, kVarReduced = 2, kVarSelective = 3, kVarIncompat = 4,
@@ -176,6 +185,24 @@ struct ufc_ctx {
   // node itself (ascending), then 0 at position deg - 1, zero padded
   ufc::DBuf<int16_t> pat_off;
   int32_t n_pat = 0;
+  // Symmetric block-diagonal view: when the offsets m - n > 0 of every node's
+  // upper neighbours take at most kMaxDiags distinct values o_0 < ... (a
+  // lexicographically numbered structured box has 13), each off-diagonal
+  // block is stored once, for its upper node: K(n, n + o_s) at entry
+  // diag_pad + n of diagonal s, zero where n + o_s is no neighbour.  Node n's
+  // lower block K(n, n - o_s) = K(n - o_s, n)^T is entry diag_pad + n - o_s;
+  // diag_pad >= o_last zero entries in front keep that read in bounds.
+  // Entries: diag_a float4 (0,0) (0,1) (0,2) (1,0), diag_b float4 (1,1)
+  // (1,2) (2,0) (2,1), diag_c float (2,2) -- three loads per block.
+  int32_t n_diag = 0;
+  int32_t h_diag_off[ufc::kMaxDiags] = {};
+  int32_t diag_pad = 0;
+  int64_t diag_stride = 0;      // entries per diagonal (diag_pad + n_nodes, rounded up)
+  ufc::DBuf<float> diag_a;      // (n_diag, diag_stride, 4)
+  ufc::DBuf<float> diag_b;      // (n_diag, diag_stride, 4)
+  ufc::DBuf<float> diag_c;      // (n_diag, diag_stride)
+  int32_t fp32_view = -1;       // view of the FP32 SpMV (kFp32View*); -1 before the pattern
+  int32_t fp32_view_request = -1;  // -1: the fastest available
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)
   ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
   ufc::DBuf<int32_t> diag_pos;    // (n_dof) position of K(i,i) in values
@@ -286,8 +313,11 @@ void launch_penalty(ufc_ctx* c, bool fp32, double sign);
 // by the penalty diagonal, diag32 = kdiag32 + penalty
 void launch_fp32_difference_form(ufc_ctx* c);
 void launch_diag32_penalty(ufc_ctx* c);
-// val32m, pen32 and pen_flag from val32 (stencil-pattern view only)
+// the view-specific FP32 values from val32: val32m (stencil-pattern view) or
+// the block diagonals (symmetric view), with pen32 and pen_flag
 void launch_fp32_offdiag(ufc_ctx* c);
+// the FP32 view the SpMV uses: the requested one, else the fastest the mesh allows
+void select_fp32_view(ufc_ctx* c);
 // y = K u in FP64 without penalties: element forces then node gather.
 // mode 0: elastic with c->young; mode 1: J2 tangent D_gp; mode 2: J2 trial stress (u unused)
 void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y);

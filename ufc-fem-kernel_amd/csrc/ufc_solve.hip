@@ -468,6 +468,136 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
   }
 }
 
+// the three values of a node as one 12-byte access (global_load_dwordx3 /
+// global_store_dwordx3)
+__device__ __forceinline__ float3 load3(const float* __restrict__ p) {
+  float3 v;
+  __builtin_memcpy(&v, p, 12);
+  return v;
+}
+__device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, float c) {
+  const float3 v = make_float3(a, b, c);
+  __builtin_memcpy(p, &v, 12);
+}
+
+// y = K x in the symmetric block-diagonal view, one node per thread:
+//   y_n = sum_s K(n, n + o_s) (x_{n+o_s} - x_n) + K(n - o_s, n)^T (x_{n-o_s} - x_n)
+//         + P_n x_n,
+// the difference form of the assembled K, whose blocks are symmetric
+// (K(n, m) = K(m, n)^T), so each off-diagonal block is read from HBM once,
+// for its upper node, and its second use (as the lower block of the other
+// node) hits in cache.  A neighbour index outside [0, n_nodes) is replaced
+// by n: its block is zero, and so is x_n - x_n.  Each block is three loads
+// (float4, float4, float); measured 0.26 ms at 128^3 against 0.38 ms for
+// the same one-node-per-thread kernel over all 26 stored blocks.
+// XCD order: workgroup b runs on XCD b % 8 and takes the 256-node tiles of
+// that XCD's contiguous share [x tpx, (x + 1) tpx) in steps of gridDim / 8
+// (gridDim a multiple of 8), so an XCD's lower reads (a node row and a node
+// plane back in a structured box) find blocks it fetched itself (measured
+// 0.301 -> 0.279 ms against the plain tile order).
+// pap != null: the workgroup partials of p.Ap as in k_spmv (fixed order,
+// barrier-free LDS combine).
+// diagonals whose loads are issued together (128^3: 4 and 7 0.266 ms, 13
+// 0.272 ms; one at a time, as the compiler schedules it unstaged, drains the
+// loads after every diagonal)
+constexpr int kDiagBatch = 4;
+__global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
+    int32_t n_nodes, DiagOffsets off, int nd, int32_t pad, int64_t stride,
+    const float4* __restrict__ da, const float4* __restrict__ db, const float* __restrict__ dc,
+    const float* __restrict__ pen, const uint32_t* __restrict__ pen_flag,
+    const float* __restrict__ x, float* __restrict__ y, const PcgScalars* __restrict__ sc,
+    double* __restrict__ pap) {
+  if (sc && sc->done) return;
+  constexpr int kWaves = kDiagThreads / WAVE;
+  __shared__ double slot[kWaves];
+  __shared__ int arrived;
+  if (pap) {
+    if (threadIdx.x == 0) arrived = 0;
+    __syncthreads();
+  }
+  const int64_t ntiles = (int64_t(n_nodes) + kDiagThreads - 1) / kDiagThreads;
+  const int64_t tpx = (ntiles + 7) / 8;
+  const int64_t lo = int64_t(blockIdx.x % 8) * tpx;
+  const int64_t hi = lo + tpx < ntiles ? lo + tpx : ntiles;
+  const int64_t step = gridDim.x / 8;
+  double dot = 0.0;
+  for (int64_t t = lo + blockIdx.x / 8; t < hi; t += step) {
+    // 32-bit indices: n < n_nodes < 2^31 and n_diag * stride < 2^31 (the
+    // host checks), which keeps the address arithmetic off 64-bit VALU pairs
+    const int32_t n = int32_t(t) * kDiagThreads + int32_t(threadIdx.x);
+    if (n >= n_nodes) continue;
+    const float3 xn = load3(x + 3 * n);
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    // Branch-free over kMaxDiags diagonals, in batches of kDiagBatch whose
+    // loads are all issued before the batch's arithmetic (without the
+    // staging the compiler waits for each diagonal's loads in turn): a
+    // diagonal s >= nd re-reads diagonal nd - 1 at offset 0, whose
+    // differences x_n - x_n make its terms exactly zero.
+#pragma unroll
+    for (int s0 = 0; s0 < kMaxDiags; s0 += kDiagBatch) {
+      constexpr int G = kDiagBatch;
+      float3 xu[G], xl[G];
+      float4 ua[G], ub[G], va[G], vb[G];
+      float u8[G], v8[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int s = s0 + g < kMaxDiags ? s0 + g : kMaxDiags - 1;
+        const int32_t se = s < nd ? s : nd - 1;
+        const int32_t o = (s0 + g < kMaxDiags && s < nd) ? off.o[s] : 0;
+        const int32_t mu = n < n_nodes - o ? n + o : n;
+        const int32_t ml = n >= o ? n - o : n;
+        xu[g] = load3(x + 3 * mu);
+        xl[g] = load3(x + 3 * ml);
+        const int32_t e = se * int32_t(stride) + pad + n;
+        ua[g] = da[e];
+        ub[g] = db[e];
+        u8[g] = dc[e];
+        va[g] = da[e - o];
+        vb[g] = db[e - o];
+        v8[g] = dc[e - o];
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const float d0 = xu[g].x - xn.x, d1 = xu[g].y - xn.y, d2 = xu[g].z - xn.z;
+        const float l0 = xl[g].x - xn.x, l1 = xl[g].y - xn.y, l2 = xl[g].z - xn.z;
+        // upper block rows (ua.xyz), (ua.w ub.xy), (ub.zw u8); the lower
+        // block is the transpose of the entry o_s back: columns of (va vb v8)
+        a0 += ua[g].x * d0 + ua[g].y * d1 + ua[g].z * d2 + va[g].x * l0 + va[g].w * l1 +
+              vb[g].z * l2;
+        a1 += ua[g].w * d0 + ub[g].x * d1 + ub[g].y * d2 + va[g].y * l0 + vb[g].x * l1 +
+              vb[g].w * l2;
+        a2 += ub[g].z * d0 + ub[g].w * d1 + u8[g] * d2 + va[g].z * l0 + vb[g].y * l1 +
+              v8[g] * l2;
+      }
+    }
+    if ((pen_flag[n >> 5] >> (n & 31)) & 1u) {
+      const float3 pn = load3(pen + 3 * n);
+      a0 += pn.x * xn.x;
+      a1 += pn.y * xn.y;
+      a2 += pn.z * xn.z;
+    }
+    store3(y + 3 * n, a0, a1, a2);
+    dot += (double(xn.x) * double(a0) + double(xn.y) * double(a1)) + double(xn.z) * double(a2);
+  }
+  if (pap) {
+    dot = sum_lanes(dot, WAVE);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = int(threadIdx.x) / WAVE;
+    if (lane == 0) {
+      slot[w] = dot;
+      const int before = __hip_atomic_fetch_add(&arrived, 1, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (before == kWaves - 1) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < kWaves; ++i)
+          t += __hip_atomic_load(&slot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        pap[blockIdx.x] = t;
+      }
+    }
+  }
+}
+
 // 16-byte accesses to V consecutive elements starting at a multiple of V
 // (every solver vector is a context buffer, 256-byte aligned); a 4-byte load
 // per lane streams ~40% slower than a 16-byte one on gfx950
@@ -944,6 +1074,19 @@ int spmv_blocks(const ufc_ctx* c) {
   return int(need < cap ? need : cap);
 }
 
+// symmetric block-diagonal SpMV grid: 8 x the tiles per XCD, capped (a
+// multiple of 8 either way)
+int sym_blocks(const ufc_ctx* c) {
+  const int64_t ntiles = (int64_t(c->n_nodes) + kDiagThreads - 1) / kDiagThreads;
+  const int64_t tpx = (ntiles + 7) / 8;
+  return int(8 * std::min<int64_t>(tpx, kSpmvBlocks32 / 8));
+}
+
+// workgroup partials the FP32 SpMV writes (its grid)
+int fp32_spmv_blocks(const ufc_ctx* c) {
+  return c->fp32_view == kFp32ViewSymDiag ? sym_blocks(c) : spmv_blocks<float>(c);
+}
+
 // pap: workgroup partials of x.y (FP32 view only), or null
 template <typename T, bool BLOCK_INDEX>
 void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
@@ -951,13 +1094,23 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
   const PcgScalars* sc = in_pcg ? c->scalars.get() : nullptr;
   const dim3 grid(spmv_blocks<T>(c)), block(kSpmvThreads);
   if constexpr (BLOCK_INDEX) {
+    if (c->fp32_view == kFp32ViewSymDiag) {
+      DiagOffsets off;
+      for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
+      hipLaunchKernelGGL(k_spmv_sym, dim3(sym_blocks(c)), dim3(kDiagThreads), 0, c->stream,
+                         c->n_nodes, off, int(c->n_diag), c->diag_pad, c->diag_stride,
+                         reinterpret_cast<const float4*>(c->diag_a.get()),
+                         reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
+                         c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
+      return;
+    }
     // the stencil-pattern view reads the off-diagonal copy of val (val32m)
-    if (c->n_pat > 0)
+    if (c->fp32_view == kFp32ViewPattern)
       hipLaunchKernelGGL((k_spmv<T, true, kViewPattern>), grid, block, 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
                          c->pat_off.get(), c->col_idx.get(), c->val32m.get(), c->pen32.get(),
                          c->pen_flag.get(), x, y, sc, pap);
-    else if (c->have_off16)
+    else if (c->fp32_view == kFp32ViewOff16)
       hipLaunchKernelGGL((k_spmv<T, true, kViewOff16>), grid, block, 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
                          c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
@@ -1016,7 +1169,7 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
       launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
       spmv_event_end(c);
       hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream,
-                         c->spmv_partials.get(), spmv_blocks<T>(c), pap_piece, sc);
+                         c->spmv_partials.get(), fp32_spmv_blocks(c), pap_piece, sc);
     } else {
       launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
       spmv_event_end(c);

@@ -29,6 +29,11 @@ PREC_MIXED = 2
 KRYLOV_PCG = 0
 KRYLOV_GMRES = 1
 
+FP32_VIEW_ID32 = 0
+FP32_VIEW_OFF16 = 1
+FP32_VIEW_PATTERN = 2
+FP32_VIEW_SYMDIAG = 3
+
 MAT_ELASTIC = 0
 MAT_J2 = 1
 
@@ -57,7 +62,8 @@ class MeshDesc(C.Structure):
 class AsmInfo(C.Structure):
     _fields_ = [("n_dof", C.c_int64), ("nnz", C.c_int64), ("n_nodes", C.c_int32),
                 ("n_elems", C.c_int32), ("max_row_nnz", C.c_int32),
-                ("nb_index_bytes", C.c_int32)]
+                ("nb_index_bytes", C.c_int32), ("fp32_view", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class KeAlgo(C.Structure):
@@ -163,6 +169,7 @@ _SIGS = {
     "ufc_nm_pcg": [C.c_void_p, P(SolverAlgo), P(SolverState), P(Status)],
     "ufc_nm_spmv_t": [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_rt_asm_set_values": [C.c_void_p, C.c_int32, C.c_void_p, P(Status)],
+    "ufc_rt_asm_set_fp32_view": [C.c_void_p, C.c_int32, P(Status)],
     "ufc_nm_adjoint_solve": [C.c_void_p, P(SolverAlgo), C.c_void_p, P(SolverState), P(Status)],
     "ufc_mesh_box": [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                      C.c_double, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],

@@ -161,7 +161,17 @@ class Context:
         self.n_dof, self.nnz = info.n_dof, info.nnz
         self.n_nodes, self.n_elems = info.n_nodes, info.n_elems
         self.nb_index_bytes = info.nb_index_bytes
+        self.fp32_view = info.fp32_view
         return info
+
+    def set_fp32_view(self, view):
+        """Select the FP32 SpMV view (A.FP32_VIEW_*, or -1 for the fastest the
+        mesh allows); raises on UFC_STATUS_UNSUPPORTED when the mesh does not
+        allow it.  Updates fp32_view and nb_index_bytes."""
+        self._call("ufc_rt_asm_set_fp32_view", int(view))
+        i = self.info()
+        self.fp32_view, self.nb_index_bytes = i.fp32_view, i.nb_index_bytes
+        return self.fp32_view
 
     def get_pattern(self, index_base=0):
         rp = np.empty(self.n_dof + 1, np.int32)
