@@ -215,16 +215,16 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx) {
   }
   if (all.empty()) return;
   std::sort(all.begin(), all.end());
-  // front zeros >= the largest offset, rounded to 64 entries; each diagonal
-  // a whole number of 64-entry blocks; the SpMV indexes the entries with
-  // 32-bit integers
-  const int64_t pad = (int64_t(all.back()) + 63) / 64 * 64;
-  const int64_t stride = (pad + nn + 63) / 64 * 64;
-  if (int64_t(all.size()) * stride >= (int64_t(1) << 31)) return;
+  // tile-major entries with ceil(o_last / kDiagTile) zero tiles in front; the
+  // SpMV indexes them with 32-bit integers
+  const int64_t pad_tiles = (int64_t(all.back()) + ufc::kDiagTile - 1) / ufc::kDiagTile;
+  const int64_t tiles = (nn + ufc::kDiagTile - 1) / ufc::kDiagTile + pad_tiles;
+  const int64_t entries = tiles * int64_t(all.size()) * ufc::kDiagTile;
+  if (entries >= (int64_t(1) << 31)) return;
   c->n_diag = int32_t(all.size());
   for (int s = 0; s < ufc::kMaxDiags; ++s) c->h_diag_off[s] = s < c->n_diag ? all[size_t(s)] : 0;
-  c->diag_pad = int32_t(pad);
-  c->diag_stride = stride;
+  c->diag_pad_tiles = int32_t(pad_tiles);
+  c->diag_entries = entries;
 }
 
 // Stencil-pattern dictionary of the 16-bit view (off16, CSR pair order):

@@ -620,15 +620,16 @@ __global__ __launch_bounds__(256) void k_offdiag_compact(int32_t n_nodes,
 
 // Symmetric block-diagonal values from the difference-form val32: a 32-lane
 // group per node, lane k on neighbour m = nb_idx[q0 + k] > n copies block
-// K(n, m) (node n's rows d, columns 3 m + c) to entry pad + n of diagonal s,
-// o_s = m - n, as float4 (0,0) (0,1) (0,2) (1,0), float4 (1,1) (1,2) (2,0)
-// (2,1) and float (2,2).  The pairs of the pattern are fixed, so the entries
-// never written stay zero from the allocation.
+// K(n, m) (node n's rows d, columns 3 m + c) to entry (s, n), o_s = m - n:
+// tile-major, ((n / kDiagTile + pad_tiles) nd + s) kDiagTile + n % kDiagTile,
+// as float4 (0,0) (0,1) (0,2) (1,0), float4 (1,1) (1,2) (2,0) (2,1) and
+// float (2,2).  The pairs of the pattern are fixed, so the entries never
+// written stay zero from the allocation.
 __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
                                                    const int32_t* __restrict__ nb_ptr,
                                                    const int32_t* __restrict__ nb_idx,
                                                    const float* __restrict__ val, DiagOffsets off,
-                                                   int nd, int32_t pad, int64_t stride,
+                                                   int nd, int32_t pad_tiles,
                                                    float4* __restrict__ da,
                                                    float4* __restrict__ db,
                                                    float* __restrict__ dc) {
@@ -652,7 +653,7 @@ __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) v[3 * d + cc] = val[src + d * s3 + cc];
-    const int64_t e = int64_t(s) * stride + pad + n;
+    const int64_t e = ((n / kDiagTile + pad_tiles) * nd + s) * kDiagTile + n % kDiagTile;
     da[e] = make_float4(v[0], v[1], v[2], v[3]);
     db[e] = make_float4(v[4], v[5], v[6], v[7]);
     dc[e] = v[8];
@@ -1208,7 +1209,7 @@ void launch_fp32_offdiag(ufc_ctx* c) {
     hipLaunchKernelGGL(k_offdiag_compact, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->diag_pos.get(), c->val32.get(), c->val32m.get());
   } else {
-    const size_t entries = size_t(c->n_diag) * size_t(c->diag_stride);
+    const size_t entries = size_t(c->diag_entries);
     if (!c->diag_c.get()) {
       c->diag_a.resize(4 * entries);
       c->diag_b.resize(4 * entries);
@@ -1221,7 +1222,7 @@ void launch_fp32_offdiag(ufc_ctx* c) {
     for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
     hipLaunchKernelGGL(k_diag_fill, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), off, int(c->n_diag),
-                       c->diag_pad, c->diag_stride,
+                       c->diag_pad_tiles,
                        reinterpret_cast<float4*>(c->diag_a.get()),
                        reinterpret_cast<float4*>(c->diag_b.get()), c->diag_c.get());
   }

@@ -90,6 +90,8 @@ constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
 constexpr int kMaxPatterns = 256;   // stencil patterns of the FP32 view (one-byte ids)
 constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP32 view
 constexpr int kDiagThreads = 256;   // symmetric FP32 SpMV: one node per thread
+constexpr int kDiagTile = 256;      // nodes per tile of the block-diagonal layout (= kDiagThreads)
+constexpr int kDiagPlanes = 16;     // tiles one "plane" (ceil(o_last / kDiagTile) tiles) apart run in a row
 // FP32 SpMV views (ufc_rt_asm_info.fp32_view)
 constexpr int kFp32ViewId32 = 0, kFp32ViewOff16 = 1, kFp32ViewPattern = 2, kFp32ViewSymDiag = 3;
 
@@ -188,19 +190,22 @@ struct ufc_ctx {
   // Symmetric block-diagonal view: when the offsets m - n > 0 of every node's
   // upper neighbours take at most kMaxDiags distinct values o_0 < ... (a
   // lexicographically numbered structured box has 13), each off-diagonal
-  // block is stored once, for its upper node: K(n, n + o_s) at entry
-  // diag_pad + n of diagonal s, zero where n + o_s is no neighbour.  Node n's
-  // lower block K(n, n - o_s) = K(n - o_s, n)^T is entry diag_pad + n - o_s;
-  // diag_pad >= o_last zero entries in front keep that read in bounds.
+  // block is stored once, for its upper node: K(n, n + o_s) is entry (s, n),
+  // zero where n + o_s is no neighbour.  Node n's lower block
+  // K(n, n - o_s) = K(n - o_s, n)^T is entry (s, n - o_s).  Tile-major: the
+  // entries of the kDiagTile nodes of tile T = n / kDiagTile + diag_pad_tiles
+  // for diagonal s are the kDiagTile consecutive entries from
+  // (T n_diag + s) kDiagTile; diag_pad_tiles = ceil(o_last / kDiagTile) zero
+  // tiles in front keep the lower reads of the first nodes in bounds.
   // Entries: diag_a float4 (0,0) (0,1) (0,2) (1,0), diag_b float4 (1,1)
   // (1,2) (2,0) (2,1), diag_c float (2,2) -- three loads per block.
   int32_t n_diag = 0;
   int32_t h_diag_off[ufc::kMaxDiags] = {};
-  int32_t diag_pad = 0;
-  int64_t diag_stride = 0;      // entries per diagonal (diag_pad + n_nodes, rounded up)
-  ufc::DBuf<float> diag_a;      // (n_diag, diag_stride, 4)
-  ufc::DBuf<float> diag_b;      // (n_diag, diag_stride, 4)
-  ufc::DBuf<float> diag_c;      // (n_diag, diag_stride)
+  int32_t diag_pad_tiles = 0;
+  int64_t diag_entries = 0;     // (n_nodes / kDiagTile + 1 + diag_pad_tiles) n_diag kDiagTile
+  ufc::DBuf<float> diag_a;      // (diag_entries, 4)
+  ufc::DBuf<float> diag_b;      // (diag_entries, 4)
+  ufc::DBuf<float> diag_c;      // (diag_entries)
   int32_t fp32_view = -1;       // view of the FP32 SpMV (kFp32View*); -1 before the pattern
   int32_t fp32_view_request = -1;  // -1: the fastest available
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)

@@ -486,15 +486,21 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 // the difference form of the assembled K, whose blocks are symmetric
 // (K(n, m) = K(m, n)^T), so each off-diagonal block is read from HBM once,
 // for its upper node, and its second use (as the lower block of the other
-// node) hits in cache.  A neighbour index outside [0, n_nodes) is replaced
-// by n: its block is zero, and so is x_n - x_n.  Each block is three loads
-// (float4, float4, float); measured 0.26 ms at 128^3 against 0.38 ms for
-// the same one-node-per-thread kernel over all 26 stored blocks.
-// XCD order: workgroup b runs on XCD b % 8 and takes the 256-node tiles of
-// that XCD's contiguous share [x tpx, (x + 1) tpx) in steps of gridDim / 8
-// (gridDim a multiple of 8), so an XCD's lower reads (a node row and a node
-// plane back in a structured box) find blocks it fetched itself (measured
-// 0.301 -> 0.279 ms against the plain tile order).
+// node) is mostly a cache re-read.  A neighbour index outside [0, n_nodes)
+// is replaced by n: its block is zero, and so is x_n - x_n.  Each block is
+// three loads (float4, float4, float); measured 0.26 ms at 128^3 against
+// 0.38 ms for the same one-node-per-thread kernel over all 26 stored blocks.
+// Tile order: a workgroup takes one kDiagTile-node tile t at a time.  Tiles
+// are visited in blocks of kDiagPlanes "planes" of F = diag_pad_tiles tiles
+// (F tiles span the largest offset, one node plane of a structured box):
+// within a block, the tiles p, p + F, p + 2F, ... run one after another, so
+// the lower blocks one plane back were fetched one tile earlier and those one
+// node row back kDiagPlanes tiles earlier -- both still in L2 -- and each
+// tile's entries are one contiguous range of the tile-major layout.  XCD
+// order on top: workgroup b runs on XCD b % 8 and takes the positions of that
+// XCD's contiguous share in steps of gridDim / 8 (gridDim a multiple of 8),
+// so the blocks an XCD re-reads are in its own L2 (in a probe: 0.301 ms plain
+// tile order, 0.279 XCD order, 0.250 with the blocked tile-major order).
 // pap != null: the workgroup partials of p.Ap as in k_spmv (fixed order,
 // barrier-free LDS combine).
 // diagonals whose loads are issued together (128^3: 4 and 7 0.266 ms, 13
@@ -502,7 +508,7 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 // loads after every diagonal)
 constexpr int kDiagBatch = 4;
 __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
-    int32_t n_nodes, DiagOffsets off, int nd, int32_t pad, int64_t stride,
+    int32_t n_nodes, DiagOffsets off, int nd, int32_t pad_tiles,
     const float4* __restrict__ da, const float4* __restrict__ db, const float* __restrict__ dc,
     const float* __restrict__ pen, const uint32_t* __restrict__ pen_flag,
     const float* __restrict__ x, float* __restrict__ y, const PcgScalars* __restrict__ sc,
@@ -515,17 +521,24 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
     if (threadIdx.x == 0) arrived = 0;
     __syncthreads();
   }
-  const int64_t ntiles = (int64_t(n_nodes) + kDiagThreads - 1) / kDiagThreads;
-  const int64_t tpx = (ntiles + 7) / 8;
-  const int64_t lo = int64_t(blockIdx.x % 8) * tpx;
-  const int64_t hi = lo + tpx < ntiles ? lo + tpx : ntiles;
-  const int64_t step = gridDim.x / 8;
+  static_assert(kDiagTile == kDiagThreads, "one node per thread and tile entry");
+  const int32_t ntiles = (n_nodes + kDiagTile - 1) / kDiagTile;
+  const int32_t blk = pad_tiles * kDiagPlanes;
+  const int32_t npos = (ntiles + blk - 1) / blk * blk;
+  const int32_t ppx = (npos + 7) / 8;
+  const int32_t lo = int32_t(blockIdx.x % 8) * ppx;
+  const int32_t hi = lo + ppx < npos ? lo + ppx : npos;
+  const int32_t step = int32_t(gridDim.x / 8);
   double dot = 0.0;
-  for (int64_t t = lo + blockIdx.x / 8; t < hi; t += step) {
-    // 32-bit indices: n < n_nodes < 2^31 and n_diag * stride < 2^31 (the
-    // host checks), which keeps the address arithmetic off 64-bit VALU pairs
-    const int32_t n = int32_t(t) * kDiagThreads + int32_t(threadIdx.x);
-    if (n >= n_nodes) continue;
+  for (int32_t w = lo + int32_t(blockIdx.x / 8); w < hi; w += step) {
+    const int32_t zb = w / blk, r = w - zb * blk;
+    const int32_t p = r / kDiagPlanes, zi = r - p * kDiagPlanes;
+    const int32_t t = (zb * kDiagPlanes + zi) * pad_tiles + p;
+    // 32-bit indices: n < n_nodes < 2^31 and the entries < 2^31 (the host
+    // checks), which keeps the address arithmetic off 64-bit VALU pairs
+    const int32_t n = t * kDiagTile + int32_t(threadIdx.x);
+    if (t >= ntiles || n >= n_nodes) continue;
+    const int32_t tu = (t + pad_tiles) * nd;  // tile row of the upper entries
     const float3 xn = load3(x + 3 * n);
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
     // Branch-free over kMaxDiags diagonals, in batches of kDiagBatch whose
@@ -548,13 +561,15 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
         const int32_t ml = n >= o ? n - o : n;
         xu[g] = load3(x + 3 * mu);
         xl[g] = load3(x + 3 * ml);
-        const int32_t e = se * int32_t(stride) + pad + n;
+        const int32_t e = (tu + se) * kDiagTile + int32_t(threadIdx.x);
+        const int32_t m = n - o + pad_tiles * kDiagTile;  // >= 0: o <= pad_tiles kDiagTile
+        const int32_t el = ((m / kDiagTile) * nd + se) * kDiagTile + m % kDiagTile;
         ua[g] = da[e];
         ub[g] = db[e];
         u8[g] = dc[e];
-        va[g] = da[e - o];
-        vb[g] = db[e - o];
-        v8[g] = dc[e - o];
+        va[g] = da[el];
+        vb[g] = db[el];
+        v8[g] = dc[el];
       }
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -1074,12 +1089,14 @@ int spmv_blocks(const ufc_ctx* c) {
   return int(need < cap ? need : cap);
 }
 
-// symmetric block-diagonal SpMV grid: 8 x the tiles per XCD, capped (a
+// symmetric block-diagonal SpMV grid: 8 x the tile positions per XCD, capped (a
 // multiple of 8 either way)
 int sym_blocks(const ufc_ctx* c) {
-  const int64_t ntiles = (int64_t(c->n_nodes) + kDiagThreads - 1) / kDiagThreads;
-  const int64_t tpx = (ntiles + 7) / 8;
-  return int(8 * std::min<int64_t>(tpx, kSpmvBlocks32 / 8));
+  const int64_t ntiles = (int64_t(c->n_nodes) + kDiagTile - 1) / kDiagTile;
+  const int64_t blk = int64_t(c->diag_pad_tiles) * kDiagPlanes;
+  const int64_t npos = (ntiles + blk - 1) / blk * blk;
+  const int64_t ppx = (npos + 7) / 8;
+  return int(8 * std::min<int64_t>(ppx, kSpmvBlocks32 / 8));
 }
 
 // workgroup partials the FP32 SpMV writes (its grid)
@@ -1098,7 +1115,7 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
       DiagOffsets off;
       for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
       hipLaunchKernelGGL(k_spmv_sym, dim3(sym_blocks(c)), dim3(kDiagThreads), 0, c->stream,
-                         c->n_nodes, off, int(c->n_diag), c->diag_pad, c->diag_stride,
+                         c->n_nodes, off, int(c->n_diag), c->diag_pad_tiles,
                          reinterpret_cast<const float4*>(c->diag_a.get()),
                          reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
                          c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
