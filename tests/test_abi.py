@@ -39,6 +39,35 @@ def test_status_codes_mirror_reference():
     assert C.sizeof(A.Status) == 4 * 4 + 512 + 64
 
 
+def test_info_struct_layout_matches_header(tmp_path):
+    """ufc_rt_asm_info as the C compiler lays it out from include/ufc_hip.h
+    (size and every field offset) equals the ctypes mirror, and the FP32
+    view constants agree."""
+    import shutil
+    import subprocess
+    from ufc_hip import _abi as A
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    fields = [f[0] for f in A.AsmInfo._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include \"ufc_hip.h\"\nint main(void) {\n"
+        '  printf("%zu\\n", sizeof(ufc_rt_asm_info));\n'
+        + "".join(f'  printf("%zu\\n", offsetof(ufc_rt_asm_info, {f}));\n' for f in fields)
+        + '  printf("%d %d %d %d\\n", UFC_FP32_VIEW_ID32, UFC_FP32_VIEW_OFF16, '
+          'UFC_FP32_VIEW_PATTERN, UFC_FP32_VIEW_SYMDIAG);\n  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert int(out[0]) == C.sizeof(A.AsmInfo)
+    for i, f in enumerate(fields):
+        assert int(out[1 + i]) == getattr(A.AsmInfo, f).offset, f
+    assert [int(v) for v in out[1 + len(fields)].split()] == [
+        A.FP32_VIEW_ID32, A.FP32_VIEW_OFF16, A.FP32_VIEW_PATTERN, A.FP32_VIEW_SYMDIAG]
+
+
 def test_generators_match_oracle(orc):
     import ufc_hip
     for (nx, ny, nz, jit) in [(3, 2, 4, 0.0), (5, 4, 3, 0.1)]:
