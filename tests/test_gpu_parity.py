@@ -501,6 +501,29 @@ def test_fp32_symdiag_view(ctx, orc):
     assert ctx.info().fp32_view == A.FP32_VIEW_OFF16
 
 
+def test_fp32_symdiag_view_tile_blocks(ctx, orc):
+    """A 24^3 cube: 62 tiles of 256 nodes, tile blocks of 16 planes of 3 tiles
+    (48), so the symmetric view's tile order runs a full block and a partial
+    one; SpMV and the mixed solve match the oracle."""
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(24, name="cube24")
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 2)
+    assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    x = np.random.default_rng(8).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    y32 = ctx.spmv(x, precision=1)
+    say("symdiag cube24 spmv", np.abs(y32 - yo).max() / np.abs(yo).max())
+    assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    u, info = ctx.pcg(precision=2, rtol=1e-10)
+    assert info.converged
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+
+
 @pytest.mark.parametrize("precision", [0, 2])
 def test_pcg_progress_record_matches_polling(ctx, precision):
     """The default host loop (the GPU reports each iteration into pinned host

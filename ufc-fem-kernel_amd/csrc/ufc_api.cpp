@@ -191,7 +191,7 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx) {
   c->diag_b.release();
   c->diag_c.release();
   const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
-  constexpr int64_t kChunk = 4096;
+  constexpr int64_t kChunk = 32;  // enough chunks for parallel_for's threads on large meshes
   const int64_t nchunk = (nn + kChunk - 1) / kChunk;
   constexpr int kCap = ufc::kMaxDiags + 1;
   std::vector<std::array<int32_t, kCap>> found(static_cast<size_t>(nchunk));
