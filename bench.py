@@ -14,9 +14,10 @@ Adjoint: the 96^3 SIMP cube (rho ~ U[0.3,1], E = 1e-3 + rho^3 1e9): forward
 solve, adjoint solve and the element sensitivity kernel; "adjoint grad ms" is
 adjoint solve + sensitivity.
 
-roofline: the dominant kernel of the mixed step (the FP32 node-block SpMV
-fused with p.Ap), algorithmic bytes per launch / mean launch time from HIP
-events on the library's stream.
+roofline: the dominant kernel of the mixed step (the FP32 SpMV fused with
+p.Ap; the symmetric block-diagonal view on this workload), algorithmic bytes
+per launch / mean launch time from HIP events on the library's stream, with
+the HBM traffic from the committed rocprofv3 counter summary beside it.
 cpu_baseline: the FP64 CPU oracle (C, OpenMP) on the same 128^3 problem:
 pattern + assembly in full and a bounded number of PCG iterations, the solve
 projected to the FP64 GPU iteration count.
@@ -42,7 +43,7 @@ def log(*a):
 
 
 def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4, n_penalty=0):
-    """Algorithmic bytes of one FP32 node-block SpMV.  Stencil-pattern view
+    """Algorithmic bytes of one FP32 SpMV.  Stencil-pattern view
     (index_bytes 1): the off-diagonal values (nnz - 9 n_nodes, the self blocks
     are not stored) + one pattern byte per node + the penalty diagonal of the
     n_penalty clamped DOFs; 16-bit offset / 4-byte id views: all nnz values +
@@ -349,6 +350,13 @@ def main():
                        "bytes_per_launch": b32, "ms_per_launch": spmv_ms,
                        "traffic_source": "profiles/spmv_pcg128_hbm_traffic.md (rocprofv3 "
                                          "FETCH_SIZE / WRITE_SIZE, copy-calibrated)"}
+    traffic = out["roofline"]["traffic"]
+    if traffic:
+        # what the kernel moves over HBM / the fabric per launch, at this run's
+        # launch time (the symmetric view re-reads each block once more, partly
+        # from L2, so this exceeds the algorithmic rate)
+        out["roofline"]["traffic_gbs"] = traffic / (spmv_ms * 1e6)
+        out["roofline"]["traffic_frac"] = traffic / (spmv_ms * 1e6) / HBM_PEAK_GBS
     # element integration + atomics-free scatter: VALU-bound; FLOPs counted in
     # the reference's dense B^T D B form (76,432 flop per element)
     # MI355X vector peaks: FP64 78.6 TFLOP/s; FP32 157.3 TFLOP/s with packed
