@@ -91,7 +91,9 @@ constexpr int kMaxPatterns = 256;   // stencil patterns of the FP32 view (one-by
 constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP32 view
 constexpr int kDiagThreads = 256;   // symmetric FP32 SpMV: one node per thread
 constexpr int kDiagTile = 256;      // nodes per tile of the block-diagonal layout (= kDiagThreads)
-constexpr int kDiagPlanes = 16;     // tiles one "plane" (ceil(o_last / kDiagTile) tiles) apart run in a row
+// tiles one "plane" (ceil(o_last / kDiagTile) tiles) apart that run in a row
+// (128^3 FP32 SpMV: 8 0.276 ms, 16 0.263, 32 0.270)
+constexpr int kDiagPlanes = 16;
 // FP32 SpMV views (ufc_rt_asm_info.fp32_view)
 constexpr int kFp32ViewId32 = 0, kFp32ViewOff16 = 1, kFp32ViewPattern = 2, kFp32ViewSymDiag = 3;
 
