@@ -391,6 +391,7 @@ int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
     info->max_row_nnz = c->max_row_nnz;
     info->nb_index_bytes = nb_index_bytes(c);
     info->fp32_view = c->fp32_view;
+    info->reserved = 0;
     return UFC_STATUS_OK;
   });
 }
@@ -774,6 +775,8 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       info->max_row_nnz = c->max_row_nnz;
       info->nb_index_bytes = nb_index_bytes(c);
       info->fp32_view = c->fp32_view;
+      info->reserved = 0;
+    info->reserved = 0;
     }
     msg = "pattern built: nnz " + std::to_string(nnz);
     return UFC_STATUS_OK;
