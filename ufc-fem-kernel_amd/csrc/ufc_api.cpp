@@ -776,7 +776,6 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       info->nb_index_bytes = nb_index_bytes(c);
       info->fp32_view = c->fp32_view;
       info->reserved = 0;
-    info->reserved = 0;
     }
     msg = "pattern built: nnz " + std::to_string(nnz);
     return UFC_STATUS_OK;
