@@ -184,13 +184,41 @@ typedef struct ufc_ph_sens_args {
 } ufc_ph_sens_args;
 int ufc_ph_c3d8_sensitivity(ufc_ctx* ctx, const ufc_ph_sens_args* args, ufc_status* st);
 
-/* J2 props (PH_J2_Props, PH_Mat_Plast_J2_Iso_Core.f90:52-91), linear
- * isotropic hardening only: [E, nu, sigma_y0, H]. */
+/* Isotropic hardening laws of PH_J2_ComputeHardening /
+ * PH_J2_ComputeHardeningTangent (PH_Mat_Plast_J2_Iso_Core.f90:549-605), the
+ * PH_MAT_J2_HARD_* values (:33-36). */
+#define UFC_J2_HARD_LINEAR 1 /* sigma_y = sigma_y0 + H peeq  (0 selects it too) */
+#define UFC_J2_HARD_SWIFT 2  /* sigma_y = K (eps0 + peeq)^n                    */
+#define UFC_J2_HARD_VOCE 3   /* sigma_y = sigma_y0 + Q (1 - exp(-delta peeq))   */
+#define UFC_J2_HARD_AF 4     /* Armstrong-Frederick backstress: not supported   */
+
+/* J2 props (PH_J2_Props, PH_Mat_Plast_J2_Iso_Core.f90:52-91: PH_J2_Cfg_Elastic,
+ * _Yield, _Harden, _Control).  The radial return runs the reference's local
+ * Newton on R(dg) = q_trial - 3G dg - sigma_y(peeq_n + dg) (:355-456) from
+ * dg = f_trial / (3G + H'(peeq_n)), to |R| < 1e-10 sigma_y0, for at most
+ * max_local_iter iterations; no convergence is a local failure (the update
+ * returns UFC_STATUS_ERROR, the reference's pnewdt = 0.25 cut-back signal,
+ * :419-427).  The tangent uses H'(peeq_n + dg).  The reference's
+ * Armstrong-Frederick backstress (:449-453, C_af, gamma_af) is updated but
+ * never subtracted from the trial deviator (:254-317), so it cannot change a
+ * result; UFC_J2_HARD_AF or a non-zero af_c / af_gamma returns
+ * UFC_STATUS_UNSUPPORTED from ufc_ph_j2_init rather than pretend to model
+ * kinematic hardening. */
 typedef struct ufc_ph_j2_props {
   double young;
   double nu;
   double sigma_y0;
-  double hardening;
+  double hardening;       /* H of UFC_J2_HARD_LINEAR                        */
+  int32_t hardening_type; /* UFC_J2_HARD_*; 0 = UFC_J2_HARD_LINEAR          */
+  int32_t max_local_iter; /* local Newton iterations; <= 0: 25              */
+                          /* (PH_MAT_J2_MAX_LOCAL_ITER, :41)                */
+  double swift_k;         /* K_swift  (> 0 for UFC_J2_HARD_SWIFT)           */
+  double swift_n;         /* n_swift  (>= 0)                                */
+  double swift_eps0;      /* eps0_swift (> 0)                               */
+  double voce_q;          /* sigma_inf of UFC_J2_HARD_VOCE                  */
+  double voce_delta;      /* delta_voce (>= 0)                              */
+  double af_c;            /* C_af: must be 0                                */
+  double af_gamma;        /* gamma_af: must be 0                            */
 } ufc_ph_j2_props;
 
 /* Allocates the per-Gauss-point FP64 state of every element (13 doubles per

@@ -79,12 +79,17 @@ def lib():
         L.orc_pcg.restype = C.c_int
         L.orc_pcg.argtypes = [C.c_int32, vp, vp, vp, vp, vp, C.c_int32, C.c_double, vp, vp,
                               C.c_int]
+        L.orc_pcg_mark.restype = C.c_int
+        L.orc_pcg_mark.argtypes = [C.c_int32, vp, vp, vp, vp, vp, C.c_int32, C.c_double,
+                                   C.c_double, vp, vp, vp, vp, C.c_int, C.c_int32]
         L.orc_cg_transpose.restype = C.c_int
         L.orc_cg_transpose.argtypes = [C.c_int32, vp, vp, vp, vp, vp, C.c_double, C.c_int32,
                                        vp, vp]
         L.orc_sensitivity.argtypes = [C.c_int32, vp, vp, C.c_double, vp, vp, vp, vp]
         L.orc_j2_update.restype = C.c_int
         L.orc_j2_update.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_j2_update_ex.restype = C.c_int
+        L.orc_j2_update_ex.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_j2_newton_step.restype = C.c_int
         L.orc_j2_newton_step.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_int32, vp, vp,
                                          C.c_double, vp, vp, vp, C.c_int32, C.c_double,
@@ -305,6 +310,19 @@ def pcg(rp, ci, val, b, x0=None, max_iter=100000, rtol=1e-10, nthreads=1):
     return x, st, ni.value, rn.value
 
 
+def pcg_mark(rp, ci, val, b, rtol, rtol_mark, max_iter=1000000, nthreads=1, report_every=0):
+    """orc_pcg_mark: PCG from x = 0 to rtol; also returns the iteration and
+    wall seconds at which ||r||/||b|| first fell below rtol_mark (progress
+    on stderr every report_every iterations when > 0)."""
+    n = rp.size - 1
+    x = np.zeros(n)
+    ni, rn, im, sm = C.c_int32(), C.c_double(), C.c_int32(), C.c_double()
+    st = lib().orc_pcg_mark(n, _p(rp), _p(ci), _p(val), _p(_f64(b)), _p(x), max_iter, rtol,
+                            rtol_mark, C.byref(ni), C.byref(rn), C.byref(im), C.byref(sm),
+                            nthreads, report_every)
+    return x, st, ni.value, rn.value, im.value, sm.value
+
+
 def cg_transpose(rp, ci, val, rhs, tol=1e-8, max_iter=1000):
     n = rp.size - 1
     lam = np.zeros(n)
@@ -324,15 +342,39 @@ def sensitivity(conn, coords, nu, dEdrho, u, lam):
 
 
 # ----------------------------------------------------------------- J2
-def j2_update(props, deps, sigma, peeq, eps_p):
-    """One material-point update; returns (status, sigma, peeq, eps_p, Dt, yielded)."""
-    props, deps = _f64(props), _f64(deps)
+J2_PROPS_LEN = 11  # E, nu, sigma_y0, H, type, max_local_iter, K, n, eps0, Q, delta
+
+
+def j2_props(props) -> np.ndarray:
+    """The oracle's 11-entry props array from a shorter tuple (zero-padded:
+    hardening type 0 = linear, max_local_iter 0 = 25) or from a dict of the
+    ufc_ph_j2_props field names."""
+    out = np.zeros(J2_PROPS_LEN)
+    if isinstance(props, dict):
+        keys = ["young", "nu", "sigma_y0", "hardening", "hardening_type", "max_local_iter",
+                "swift_k", "swift_n", "swift_eps0", "voce_q", "voce_delta"]
+        for i, k in enumerate(keys):
+            out[i] = float(props.get(k, 0.0))
+        return out
+    p = np.asarray(props, np.float64).ravel()
+    out[:p.size] = p
+    return out
+
+
+def j2_update(props, deps, sigma, peeq, eps_p, with_iters=False):
+    """One material-point update; returns (status, sigma, peeq, eps_p, Dt,
+    yielded) and, with_iters, the local Newton iterations used."""
+    props, deps = j2_props(props), _f64(deps)
     sig = _f64(sigma).copy()
     pe = np.array([peeq], np.float64)
     ep = _f64(eps_p).copy()
     Dt = np.empty((6, 6))
     y = C.c_int32()
-    st = lib().orc_j2_update(_p(props), _p(deps), _p(sig), _p(pe), _p(ep), _p(Dt), C.byref(y))
+    li = C.c_int32()
+    st = lib().orc_j2_update_ex(_p(props), _p(deps), _p(sig), _p(pe), _p(ep), _p(Dt),
+                                C.byref(y), C.byref(li))
+    if with_iters:
+        return st, sig, pe[0], ep, Dt.T.copy(), y.value, li.value
     return st, sig, pe[0], ep, Dt.T.copy(), y.value
 
 
@@ -345,7 +387,7 @@ def j2_newton_step(n_nodes, conn, coords, props, bc_eq, bc_val, penalty, state_n
     u = _f64(u).copy()
     nn = C.c_int32()
     rn = C.c_double()
-    st = lib().orc_j2_newton_step(n_nodes, conn.shape[0], _p(conn), _p(coords), _p(_f64(props)),
+    st = lib().orc_j2_newton_step(n_nodes, conn.shape[0], _p(conn), _p(coords), _p(j2_props(props)),
                                   bc_eq.size, _p(bc_eq), _p(bc_val), penalty, _p(state_n),
                                   _p(state), _p(u), max_newton, tol_r, pcg_rtol, C.byref(nn),
                                   C.byref(rn))

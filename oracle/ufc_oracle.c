@@ -12,9 +12,12 @@
  *   - CSR is merged and column-sorted                  (RT_Solv_Sparse.f90:336-447)
  *   - J2 tangent is the textbook consistent tangent    (PH_Mat_Plast_J2_Iso_Core.f90:467-539)
  */
+#define _POSIX_C_SOURCE 199309L  /* clock_gettime */
 #include "ufc_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -848,9 +851,23 @@ static double dot(int32_t n, const double* a, const double* b, int nthreads) {
  * else z = r; stop at the first k with ||r||/||b|| < rtol (||b|| < 1e-30 ->
  * 1).  Returns 0 (IF_STATUS_OK) when converged, 1 (IF_STATUS_INVALID, the
  * reference's non-convergence code) otherwise, 2 on p.Ap breakdown. */
-int orc_pcg(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
-            const double* b, double* x, int32_t max_iter, double rtol,
-            int32_t* niter, double* rnorm, int nthreads) {
+static double wall_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1.0e-9 * (double)ts.tv_nsec;
+}
+
+/* The PCG loop of orc_pcg.  rtol_mark > 0: the first iteration whose
+ * ||r||/||b|| falls below rtol_mark, and the wall time from entry to the end
+ * of that iteration, go to *iter_mark / *sec_mark (the iteration continues to
+ * rtol). */
+static int pcg_core(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                    const double* b, double* x, int32_t max_iter, double rtol,
+                    int32_t* niter, double* rnorm, int nthreads, double rtol_mark,
+                    int32_t* iter_mark, double* sec_mark, int32_t report_every) {
+  const double t_start = wall_seconds();
+  if (iter_mark) *iter_mark = -1;
+  if (sec_mark) *sec_mark = -1.0;
   double* r = (double*)malloc(sizeof(double) * (size_t)n);
   double* z = (double*)malloc(sizeof(double) * (size_t)n);
   double* p = (double*)malloc(sizeof(double) * (size_t)n);
@@ -892,6 +909,15 @@ int orc_pcg(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const dou
       r[i] -= alpha * Ap[i];
     }
     rn = sqrt(dot(n, r, r, nthreads)) / bn;
+    if (rtol_mark > 0.0 && rn < rtol_mark && iter_mark && *iter_mark < 0) {
+      *iter_mark = k;
+      if (sec_mark) *sec_mark = wall_seconds() - t_start;
+    }
+    if (report_every > 0 && k % report_every == 0) {
+      fprintf(stderr, "[oracle pcg] iteration %d ||r||/||b|| %.3e %.1f s\n", (int)k, rn,
+              wall_seconds() - t_start);
+      fflush(stderr);
+    }
     if (rn < rtol) { status = 0; break; }
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nt)
@@ -909,6 +935,26 @@ int orc_pcg(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const dou
   if (rnorm) *rnorm = rn;
   free(r); free(z); free(p); free(Ap); free(dg);
   return status;
+}
+
+int orc_pcg(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+            const double* b, double* x, int32_t max_iter, double rtol,
+            int32_t* niter, double* rnorm, int nthreads) {
+  return pcg_core(n, row_ptr, col_idx, val, b, x, max_iter, rtol, niter, rnorm, nthreads, 0.0,
+                  NULL, NULL, 0);
+}
+
+/* orc_pcg that also reports when ||r||/||b|| first fell below rtol_mark
+ * (iteration and wall seconds from entry): one run to a tight rtol gives the
+ * solution for error checks and the time to the benchmark's tolerance.
+ * report_every > 0 prints a progress line to stderr every that many
+ * iterations (long runs). */
+int orc_pcg_mark(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                 const double* b, double* x, int32_t max_iter, double rtol, double rtol_mark,
+                 int32_t* niter, double* rnorm, int32_t* iter_mark, double* sec_mark,
+                 int nthreads, int32_t report_every) {
+  return pcg_core(n, row_ptr, col_idx, val, b, x, max_iter, rtol, niter, rnorm, nthreads,
+                  rtol_mark, iter_mark, sec_mark, report_every);
 }
 
 /* NM_CG_Solve_Transpose, NM_Solv_GMRESTranspose.f90:118-224: unpreconditioned
@@ -982,23 +1028,61 @@ void orc_sensitivity(int32_t n_elems, const int32_t* conn, const double* coords,
 }
 
 /* ========================================================================= */
-/* J2 plasticity with linear isotropic hardening.                            */
-/* props = [E, nu, sigma_y0, H].  Stress update per PH_J2_ComputeStress_Core  */
-/* (PH_Mat_Plast_J2_Iso_Core.f90:146-207): trial stress (:254-317), yield    */
-/* check, radial return with the local Newton of :355-456.  The tangent is   */
-/* the textbook consistent tangent (Simo & Hughes 1998 sec. 3.3; de Souza    */
-/* Neto et al. 2008 eq. 7.120) in engineering-shear Voigt form:              */
+/* J2 plasticity with isotropic hardening.                                   */
+/* props = [E, nu, sigma_y0, H, type, max_local_iter, K_swift, n_swift,      */
+/*          eps0_swift, sigma_inf, delta_voce]; a shorter array from the     */
+/* Python front end is zero-padded (type 0 = linear, max_local_iter 0 = 25). */
+/* Hardening PH_J2_ComputeHardening / _Tangent (PH_Mat_Plast_J2_Iso_Core.f90 */
+/* :549-605): 1 linear sigma_y0 + H e, 2 Swift K (eps0 + e)^n, 3 Voce        */
+/* sigma_y0 + Q (1 - exp(-delta e)).  Stress update per                      */
+/* PH_J2_ComputeStress_Core (:146-207): trial stress (:254-317), yield check */
+/* against sigma_y(peeq_n), radial return with the local Newton of :355-456  */
+/* (dg0 = f / (3G + H'(peeq_n)); R = q - 3G dg - sigma_y(peeq_n + dg);       */
+/* dg -= R / -(3G + H'(peeq_n + dg)), clipped at 0; |R| < 1e-10 sigma_y0;     */
+/* at most max_local_iter iterations).  The tangent is the textbook          */
+/* consistent tangent (Simo & Hughes 1998 sec. 3.3; de Souza Neto et al.     */
+/* 2008 eq. 7.120) in engineering-shear Voigt form with H' at the updated    */
+/* peeq:                                                                     */
 /*   Dt = K 1x1 + 2G th1 (I_s - 1x1/3) - 2G (th1 + th2 - 1) n x n,           */
-/*   th1 = 1 - 3G dg/q_tr, th2 = 3G/(3G+H), I_s = diag(1,1,1,1/2,1/2,1/2),   */
+/*   th1 = 1 - 3G dg/q_tr, th2 = 3G/(3G+H'), I_s = diag(1,1,1,1/2,1/2,1/2),  */
 /* replacing the reference's :509-537, which is not the consistent tangent.  */
-/* Returns 0 OK, 2 (IF_STATUS_ERROR) for a degenerate or non-converged      */
-/* local Newton.                                                            */
+/* Returns 0 OK, 2 (IF_STATUS_ERROR) for a degenerate trial state, a zero    */
+/* Jacobian or a local Newton that did not converge (*local_iters, when not  */
+/* NULL, gets the iterations the local Newton used).                         */
 /* ========================================================================= */
-int orc_j2_update(const double* props, const double* deps, double* sigma, double* peeq,
-                  double* eps_p, double* Dt, int32_t* yielded) {
-  const double E = props[0], nu = props[1], sy0 = props[2], H = props[3];
+static double j2_yield(const double* pr, double e) {
+  const int type = (int)pr[4];
+  if (type == 2) return pr[6] * pow(pr[8] + e, pr[7]);
+  if (type == 3) return pr[2] + pr[9] * (1.0 - exp(-pr[10] * e));
+  return pr[2] + pr[3] * e;
+}
+
+static double j2_hard_tan(const double* pr, double e) {
+  const int type = (int)pr[4];
+  if (type == 2) return pr[6] * pr[7] * pow(pr[8] + e, pr[7] - 1.0);
+  if (type == 3) return pr[9] * pr[10] * exp(-pr[10] * e);
+  return pr[3];
+}
+
+/* Dt = K 1x1 + 2G th1 (I_s - 1x1/3) - 2G (th1 + th2 - 1) n x n (column-major) */
+static void j2_tangent(double K, double G, double th1, double th2, const double* n, double* Dt) {
+  const double cn = 2.0 * G * (th1 + th2 - 1.0);
+  for (int j = 0; j < 6; ++j)
+    for (int i = 0; i < 6; ++i) {
+      double v = 0.0;
+      if (i < 3 && j < 3) v += K - 2.0 * G * th1 / 3.0;
+      if (i == j) v += 2.0 * G * th1 * (i < 3 ? 1.0 : 0.5);
+      v -= cn * n[i] * n[j];
+      Dt[i + 6 * j] = v;
+    }
+}
+
+int orc_j2_update_ex(const double* props, const double* deps, double* sigma, double* peeq,
+                     double* eps_p, double* Dt, int32_t* yielded, int32_t* local_iters) {
+  const double E = props[0], nu = props[1], sy0 = props[2];
   const double mu = E / (2.0 * (1.0 + nu));
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  if (local_iters) *local_iters = 0;
   double De[36];
   memset(De, 0, sizeof De);
   for (int i = 0; i < 3; ++i) {
@@ -1019,26 +1103,43 @@ int orc_j2_update(const double* props, const double* deps, double* sigma, double
   double q = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] +
                   2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
   q = sqrt(1.5) * q;
-  const double f = q - (sy0 + H * (*peeq));
+  const double peeq_n = *peeq;
+  const double f = q - j2_yield(props, peeq_n);
+  const double G = mu, K = E / (3.0 * (1.0 - 2.0 * nu));
   if (f <= 0.0) {
     for (int i = 0; i < 6; ++i) sigma[i] = st[i];
-    if (Dt) memcpy(Dt, De, sizeof De);
     if (yielded) *yielded = 0;
+    if (!Dt) return 0;
+    if (peeq_n > 0.0 && f > -1.0e-8 * sy0 && q >= 1.0e-30) {
+      /* on the yield surface of a point that has yielded (the strain
+       * increment of the first Newton iteration of an increment is zero):
+       * the plastic branch's tangent at dg = 0, th1 = 1, th2 = 3G/(3G+H'),
+       * the direction a loading step takes (D_el would make that iteration
+       * an elastic predictor) */
+      double n0[6];
+      for (int i = 0; i < 6; ++i) n0[i] = s[i] / (q / sqrt(1.5));
+      j2_tangent(K, G, 1.0, 3.0 * G / (3.0 * G + j2_hard_tan(props, peeq_n)), n0, Dt);
+    } else {
+      memcpy(Dt, De, sizeof De);
+    }
     return 0;
   }
   if (q < 1.0e-30) return 2;
-  const double G = mu;
   double n[6];
   for (int i = 0; i < 6; ++i) n[i] = s[i] / (q / sqrt(1.5));
-  double dg = f / (3.0 * G + H);
+  double dg = f / (3.0 * G + j2_hard_tan(props, peeq_n));
+  const int max_it = props[5] > 0.0 ? (int)props[5] : 25;
   int it;
-  for (it = 1; it <= 25; ++it) {
-    double R = q - 3.0 * G * dg - (sy0 + H * (*peeq + dg));
+  for (it = 1; it <= max_it; ++it) {
+    double R = q - 3.0 * G * dg - j2_yield(props, peeq_n + dg);
     if (fabs(R) < 1.0e-10 * sy0) break;
-    dg = dg - R / (-(3.0 * G + H));
+    double dR = -(3.0 * G + j2_hard_tan(props, peeq_n + dg));
+    if (fabs(dR) < 1.0e-30) return 2;
+    dg = dg - R / dR;
     if (dg < 0.0) dg = 0.0;
   }
-  if (it > 25) return 2;
+  if (local_iters) *local_iters = it;
+  if (it > max_it) return 2;
   const double beta = 1.0 - 3.0 * G * dg / q;
   sigma[0] = beta * s[0] + pm;
   sigma[1] = beta * s[1] + pm;
@@ -1046,23 +1147,16 @@ int orc_j2_update(const double* props, const double* deps, double* sigma, double
   sigma[3] = beta * s[3];
   sigma[4] = beta * s[4];
   sigma[5] = beta * s[5];
-  *peeq += dg;
+  *peeq = peeq_n + dg;
   for (int i = 0; i < 6; ++i) eps_p[i] += dg * n[i];
   if (yielded) *yielded = 1;
-  if (Dt) {
-    const double K = E / (3.0 * (1.0 - 2.0 * nu));
-    const double th1 = beta, th2 = 3.0 * G / (3.0 * G + H);
-    const double cn = 2.0 * G * (th1 + th2 - 1.0);
-    for (int j = 0; j < 6; ++j)
-      for (int i = 0; i < 6; ++i) {
-        double v = 0.0;
-        if (i < 3 && j < 3) v += K - 2.0 * G * th1 / 3.0;
-        if (i == j) v += 2.0 * G * th1 * (i < 3 ? 1.0 : 0.5);
-        v -= cn * n[i] * n[j];
-        Dt[i + 6 * j] = v;
-      }
-  }
+  if (Dt) j2_tangent(K, G, beta, 3.0 * G / (3.0 * G + j2_hard_tan(props, *peeq)), n, Dt);
   return 0;
+}
+
+int orc_j2_update(const double* props, const double* deps, double* sigma, double* peeq,
+                  double* eps_p, double* Dt, int32_t* yielded) {
+  return orc_j2_update_ex(props, deps, sigma, peeq, eps_p, Dt, yielded, NULL);
 }
 
 /* One load increment of the small-strain static Newton loop
@@ -1076,7 +1170,9 @@ int orc_j2_update(const double* props, const double* deps, double* sigma, double
  * RT_Step_Exec.f90:380-382, which are meaningless in SI units).
  * state (13 doubles per Gauss point: sigma(6), peeq, eps_p(6)) holds the
  * trial state on return; state_n is the committed state; u holds u_n on
- * entry.  Returns 0 converged, 4 (IF_STATUS_NOT_CONVERGED) otherwise. */
+ * entry.  Returns 0 converged, 4 (IF_STATUS_NOT_CONVERGED) otherwise, 2 when
+ * a Gauss point's local Newton failed (the reference's pnewdt cut-back
+ * signal, PH_Mat_Plast_J2_Iso_Core.f90:419-427). */
 int orc_j2_newton_step(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
                        const double* coords, const double* props,
                        int32_t n_bc, const int32_t* bc_eq, const double* bc_val, double penalty,
@@ -1098,6 +1194,7 @@ int orc_j2_newton_step(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
   int status = 4;
   int32_t it;
   double rn = 0.0;
+  int64_t local_fail = 0;
   for (it = 1; it <= max_newton; ++it) {
     memset(Fi, 0, sizeof(double) * (size_t)n);
     for (int64_t e = 0; e < n_elems; ++e) {
@@ -1112,12 +1209,15 @@ int orc_j2_newton_step(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
         const double* sn = state_n + 13 * (8 * e + g);
         double* sc = state + 13 * (8 * e + g);
         memcpy(sc, sn, 13 * sizeof(double));
-        orc_j2_update(props, deps + 6 * g, sc, sc + 6, sc + 7, Dgp + 288 * e + 36 * g, NULL);
+        if (orc_j2_update(props, deps + 6 * g, sc, sc + 6, sc + 7, Dgp + 288 * e + 36 * g,
+                          NULL) != 0)
+          local_fail++;
         memcpy(sig + 6 * g, sc, 6 * sizeof(double));
       }
       orc_c3d8_fint(xe, sig, Fe);
       for (int j = 0; j < 24; ++j) Fi[3 * (int64_t)conn[8 * e + j / 3] + j % 3] += Fe[j];
     }
+    if (local_fail) { status = 2; break; }
     for (int32_t i = 0; i < n; ++i) R[i] = -Fi[i];
     for (int32_t b = 0; b < n_bc; ++b) R[bc_eq[b]] += penalty * (bc_val[b] - u[bc_eq[b]]);
     double fn = sqrt(dot(n, Fi, Fi, 1));

@@ -77,6 +77,10 @@ void orc_spmv_t(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const
 int  orc_pcg(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
              const double* b, double* x, int32_t max_iter, double rtol,
              int32_t* niter, double* rnorm, int nthreads);
+int  orc_pcg_mark(int32_t n, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                  const double* b, double* x, int32_t max_iter, double rtol, double rtol_mark,
+                  int32_t* niter, double* rnorm, int32_t* iter_mark, double* sec_mark,
+                  int nthreads, int32_t report_every);
 int  orc_cg_transpose(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
                       const double* val, const double* rhs, double* lam, double tol,
                       int32_t max_iter, int32_t* niter, double* rel);
@@ -88,6 +92,11 @@ void orc_sensitivity(int32_t n_elems, const int32_t* conn, const double* coords,
 /* ---- J2 plasticity -------------------------------------------------------- */
 int  orc_j2_update(const double* props, const double* deps, double* sigma, double* peeq,
                    double* eps_p, double* Dt, int32_t* yielded);
+/* props (11): E, nu, sigma_y0, H, hardening type (1 linear, 2 Swift, 3 Voce),
+ * max local iterations (0: 25), K_swift, n_swift, eps0_swift, sigma_inf,
+ * delta_voce; local_iters receives the local Newton iterations used */
+int  orc_j2_update_ex(const double* props, const double* deps, double* sigma, double* peeq,
+                      double* eps_p, double* Dt, int32_t* yielded, int32_t* local_iters);
 int  orc_j2_newton_step(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
                         const double* coords, const double* props,
                         int32_t n_bc, const int32_t* bc_eq, const double* bc_val, double penalty,

@@ -139,3 +139,43 @@ def test_j2_driver_checkpoint_and_resume(tmp_path, orc):
     so = state.reshape(-1, 13)
     assert np.abs(sg[:, :6] - so[:, :6]).max() < 1e-6 * np.abs(so[:, :6]).max()
     assert np.abs(sg[:, 6] - so[:, 6]).max() < 1e-6 * so[:, 6].max()
+
+
+@pytest.mark.gpu
+def test_j2_driver_cutback_and_hardening_law(tmp_path, orc):
+    """The Fortran J2 driver with Voce hardening (hard=voce) and a local
+    Newton limit of 2: its first increment, to half the load, fails locally
+    and is cut back by 0.25 from the committed state (RT_Step_Exec.f90:
+    455-471, 575-590); the job then runs eight increments of 0.125 and ends
+    equal to an uncut eight-increment job and to the oracle's."""
+    from ufc_hip import problems as P
+    law = ["hard=voce", "sy0=250e6", "q=150e6", "delta=30"]
+    cut, uncut = tmp_path / "cut.bin", tmp_path / "uncut.bin"
+    print("[gpu-test] fortran j2 driver cut-back", flush=True)
+    r = subprocess.run([_exe(EXE_J2), "4", "fp64", "2", str(cut), "local_iter=2"] + law,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "cutback at lambda  0.50000" in r.stdout and "reason local" in r.stdout
+    assert r.stdout.count("increment ") == 8 and " cutbacks 1 " in r.stdout
+    r2 = subprocess.run([_exe(EXE_J2), "4", "fp64", "8", str(uncut)] + law,
+                        capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert " cutbacks 0 " in r2.stdout
+    u_cut, u_ref = np.fromfile(cut, dtype=np.float64), np.fromfile(uncut, dtype=np.float64)
+    assert np.linalg.norm(u_cut - u_ref) / np.linalg.norm(u_ref) < 1e-12
+    pr = P.j2_cube(4)
+    target = np.zeros(pr.bc_eq.size)
+    target[pr.extra["n_clamp"]:] = pr.extra["strain"]
+    props = dict(young=200e9, nu=0.3, sigma_y0=250e6, hardening_type=3, voce_q=150e6,
+                 voce_delta=30.0)
+    state = np.zeros(pr.n_elems * 8 * 13)
+    uo = np.zeros(pr.n_dof)
+    for k in range(1, 9):
+        st, uo, state, _, _ = orc.j2_newton_step(pr.coords.shape[0], pr.conn, pr.coords, props,
+                                                 pr.bc_eq, target * (0.125 * k), 1e12, state,
+                                                 uo)
+        assert st == 0
+    err = np.linalg.norm(u_cut - uo) / np.linalg.norm(uo)
+    print(f"[gpu-test] fortran j2 cut-back u err {err:.2e}", flush=True)
+    assert err < 1e-7

@@ -419,6 +419,121 @@ def test_j2_consistent_tangent_by_finite_differences(orc):
     assert np.abs(Dt - Dt.T).max() < 1e-6 * np.abs(Dt).max()
 
 
+# hardening laws of PH_J2_ComputeHardening (PH_Mat_Plast_J2_Iso_Core.f90:549-605)
+# as 11-entry oracle props: E, nu, sigma_y0, H, type, max_local_iter, K, n,
+# eps0, sigma_inf, delta
+J2_LAWS = {
+    "linear": [200e9, 0.3, 250e6, 2e9, 1, 0, 0, 0, 0, 0, 0],
+    "swift": [200e9, 0.3, 500e6 * 0.01 ** 0.2, 0, 2, 0, 500e6, 0.2, 0.01, 0, 0],
+    "voce": [200e9, 0.3, 250e6, 0, 3, 0, 0, 0, 0, 150e6, 30.0],
+}
+
+
+def _sigma_y(props, e):
+    E, nu, sy0, H, t, _, K, n, e0, Q, d = props
+    if t == 2:
+        return K * (e0 + e) ** n
+    if t == 3:
+        return sy0 + Q * (1.0 - math.exp(-d * e))
+    return sy0 + H * e
+
+
+def _uniaxial_closed_form(props, e11):
+    """Monotonic uniaxial stress: sigma = sigma_y(peeq) with
+    e11 = sigma / E + peeq beyond yield, found by bisection on
+    g(sigma) = sigma - sigma_y(e11 - sigma / E), which increases with sigma;
+    sigma = E e11 below yield."""
+    E = props[0]
+    if E * e11 <= _sigma_y(props, 0.0):
+        return E * e11, 0.0
+    lo, hi = 0.0, E * e11
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid - _sigma_y(props, e11 - mid / E) > 0.0:
+            hi = mid
+        else:
+            lo = mid
+    sig = 0.5 * (lo + hi)
+    return sig, e11 - sig / E
+
+
+@pytest.mark.parametrize("law", ["linear", "swift", "voce"])
+def test_j2_hardening_uniaxial_closed_form(orc, law):
+    """Each hardening law under uniaxial stress against the closed form of
+    the law itself (independent bisection, not the radial return): beyond
+    yield sigma_11 = sigma_y(peeq) and eps_11 = sigma_11 / E + peeq."""
+    props = J2_LAWS[law]
+    path = [0.0005, 0.002, 0.004, 0.008, 0.02, 0.05]
+    res = _uniaxial_stress(orc, props, path)
+    for e11, (sig, pe) in zip(path, res):
+        s_cf, pe_cf = _uniaxial_closed_form(props, e11)
+        assert abs(sig[0] - s_cf) < 1e-9 * s_cf, (law, e11, sig[0], s_cf)
+        assert abs(pe - pe_cf) < 1e-9 * max(pe_cf, 1e-3), (law, e11, pe, pe_cf)
+        assert np.abs(sig[1:]).max() < 1e-6 * s_cf
+    # the laws really differ past yield
+    assert res[-1][0][0] > 1.05 * _sigma_y(props, 0.0)
+
+
+@pytest.mark.parametrize("law", ["linear", "swift", "voce"])
+def test_j2_hardening_consistent_tangent_fd(orc, law):
+    """The consistent tangent with H'(peeq_n + dg) against central
+    differences of the stress update, for each law, from a plastic state."""
+    props = J2_LAWS[law]
+    sig0 = np.array([180e6, -40e6, 30e6, 60e6, -25e6, 10e6])
+    pe0, ep0 = 0.01, np.zeros(6)
+    deps = np.array([1.5e-3, -4e-4, 2e-4, 9e-4, -3e-4, 5e-4])
+    st, sig, pe, ep, Dt, y, li = orc.j2_update(props, deps, sig0, pe0, ep0, with_iters=True)
+    assert st == 0 and y == 1
+    h = 1e-9
+    fd = np.zeros((6, 6))
+    for j in range(6):
+        dp, dm = deps.copy(), deps.copy()
+        dp[j] += h
+        dm[j] -= h
+        sp = orc.j2_update(props, dp, sig0, pe0, ep0)[1]
+        sm = orc.j2_update(props, dm, sig0, pe0, ep0)[1]
+        fd[:, j] = (sp - sm) / (2 * h)
+    assert np.abs(fd - Dt).max() < 1e-5 * np.abs(Dt).max(), law
+    assert np.abs(Dt - Dt.T).max() < 1e-6 * np.abs(Dt).max()
+    # a tangent with H' at peeq_n instead of peeq_n + dg fails the same check
+    if law != "linear":
+        G = props[0] / (2 * (1 + props[1]))
+        s = sig.copy()
+        s[:3] -= s[:3].mean()
+        nrm = math.sqrt(s[0] ** 2 + s[1] ** 2 + s[2] ** 2 + 2 * (s[3] ** 2 + s[4] ** 2 +
+                                                                 s[5] ** 2))
+        nv = s / nrm
+        def htan(e):
+            return (_sigma_y(props, e + 1e-7) - _sigma_y(props, e - 1e-7)) / 2e-7
+        th2_new = 3 * G / (3 * G + htan(pe))
+        th2_old = 3 * G / (3 * G + htan(pe0))
+        wrong = Dt + 2 * G * (th2_new - th2_old) * np.outer(nv, nv)
+        assert np.abs(fd - wrong).max() > 1e-4 * np.abs(Dt).max()
+
+
+def test_j2_local_newton_iterations_and_limit(orc):
+    """Linear hardening returns in one local iteration (the first guess is
+    exact); Voce and Swift iterate; a local limit below what a step needs is
+    a failure (status 2, the reference's pnewdt cut-back signal,
+    PH_Mat_Plast_J2_Iso_Core.f90:419-427), and the same limit passes a
+    step small enough."""
+    deps = np.array([4e-3, -1.2e-3, -1.2e-3, 0, 0, 0])
+    z = np.zeros(6)
+    for law, lo in (("linear", 1), ("swift", 3), ("voce", 3)):
+        st, sig, pe, ep, Dt, y, li = orc.j2_update(J2_LAWS[law], deps, z, 0.0, z,
+                                                   with_iters=True)
+        assert st == 0 and y == 1
+        assert (li == 1) if law == "linear" else (li >= lo), (law, li)
+    limited = list(J2_LAWS["voce"])
+    limited[5] = 2
+    st, *_ = orc.j2_update(limited, deps, z, 0.0, z)
+    assert st == 2
+    # a step just past yield converges within two local iterations
+    small = np.array([1.4e-3, -0.42e-3, -0.42e-3, 0, 0, 0])
+    st, sig, pe, ep, Dt, y, li = orc.j2_update(limited, small, z, 0.0, z, with_iters=True)
+    assert st == 0 and y == 1 and li <= 2
+
+
 def test_j2_newton_converges_quadratically(orc):
     coords, conn = orc.mesh_box(2, 2, 2, jitter=0.1, seed=5)
     bottom = np.nonzero(coords[:, 2] == 0)[0]

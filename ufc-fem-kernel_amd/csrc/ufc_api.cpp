@@ -506,9 +506,27 @@ int ufc_ph_j2_init(ufc_ctx* c, const ufc_ph_j2_props* p, ufc_status* st) {
     UFC_REQUIRE(p->young > 0.0 && p->nu >= 0.0 && p->nu < 0.5, UFC_STATUS_ERROR,
                 "Invalid elastic parameters (E<=0 or nu out of range)");
     UFC_REQUIRE(p->sigma_y0 > 0.0, UFC_STATUS_ERROR, "sigma_y0 must be positive");
+    ufc_ph_j2_props law = *p;
+    if (law.hardening_type == 0) law.hardening_type = UFC_J2_HARD_LINEAR;
+    UFC_REQUIRE(law.hardening_type >= UFC_J2_HARD_LINEAR && law.hardening_type <= UFC_J2_HARD_AF,
+                UFC_STATUS_INVALID, "unknown hardening_type");
+    // the reference updates the Armstrong-Frederick backstress
+    // (PH_Mat_Plast_J2_Iso_Core.f90:449-453) but never subtracts it from the
+    // trial deviator (:254-317): it cannot change a result, so it is refused
+    UFC_REQUIRE(law.hardening_type != UFC_J2_HARD_AF && law.af_c == 0.0 && law.af_gamma == 0.0,
+                UFC_STATUS_UNSUPPORTED,
+                "Armstrong-Frederick kinematic hardening is not supported (the reference's "
+                "backstress never enters its yield function)");
+    if (law.hardening_type == UFC_J2_HARD_SWIFT)
+      UFC_REQUIRE(law.swift_k > 0.0 && law.swift_n >= 0.0 && law.swift_eps0 > 0.0,
+                  UFC_STATUS_ERROR, "Swift hardening needs K > 0, n >= 0 and eps0 > 0");
+    if (law.hardening_type == UFC_J2_HARD_VOCE)
+      UFC_REQUIRE(law.voce_delta >= 0.0 && std::isfinite(law.voce_q), UFC_STATUS_ERROR,
+                  "Voce hardening needs delta >= 0 and a finite sigma_inf");
+    UFC_REQUIRE(law.max_local_iter <= 1000, UFC_STATUS_INVALID, "max_local_iter > 1000");
     UFC_HIP_CHECK(hipSetDevice(c->device));
     const size_t ngp = size_t(c->n_elems) * 8;
-    c->j2 = *p;
+    c->j2 = law;
     c->j2_commit.resize(ngp * 13);
     c->j2_trial.resize(ngp * 13);
     c->j2_tangent.resize(ngp * 36);

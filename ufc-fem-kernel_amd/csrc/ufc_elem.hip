@@ -923,15 +923,56 @@ __global__ __launch_bounds__(128) void k_sensitivity(int32_t n_elems,
 }
 
 // ---------------------------------------------------------------- J2
-// PH_J2_ComputeStress_Core (PH_Mat_Plast_J2_Iso_Core.f90:146-207) for linear
-// isotropic hardening: trial stress (:254-317), yield check, radial return
-// with the local Newton of :355-456; the tangent is the consistent tangent
+// sigma_y(peeq) and H'(peeq) of PH_J2_ComputeHardening /
+// PH_J2_ComputeHardeningTangent (PH_Mat_Plast_J2_Iso_Core.f90:549-605); the
+// law was validated and normalised (0 -> linear) by ufc_ph_j2_init
+__device__ __forceinline__ double j2_yield(const ufc_ph_j2_props& p, double e) {
+  if (p.hardening_type == UFC_J2_HARD_SWIFT) return p.swift_k * pow(p.swift_eps0 + e, p.swift_n);
+  if (p.hardening_type == UFC_J2_HARD_VOCE)
+    return p.sigma_y0 + p.voce_q * (1.0 - exp(-p.voce_delta * e));
+  return p.sigma_y0 + p.hardening * e;
+}
+__device__ __forceinline__ double j2_hard_tan(const ufc_ph_j2_props& p, double e) {
+  if (p.hardening_type == UFC_J2_HARD_SWIFT)
+    return p.swift_k * p.swift_n * pow(p.swift_eps0 + e, p.swift_n - 1.0);
+  if (p.hardening_type == UFC_J2_HARD_VOCE)
+    return p.voce_q * p.voce_delta * exp(-p.voce_delta * e);
+  return p.hardening;
+}
+
+// PH_J2_ComputeStress_Core (PH_Mat_Plast_J2_Iso_Core.f90:146-207): trial
+// stress (:254-317), yield check against sigma_y(peeq_n), radial return with
+// the local Newton of :355-456 -- dg from f / (3G + H'(peeq_n)), then
+// R = q - 3G dg - sigma_y(peeq_n + dg), dg -= R / -(3G + H'), dg >= 0, until
+// |R| < 1e-10 sigma_y0 within max_local_iter iterations (25 by default);
+// the tangent is the consistent tangent
 //   Dt = K 1x1 + 2G th1 (I_s - 1x1/3) - 2G (th1 + th2 - 1) n x n
-// (th1 = 1 - 3G dg/q, th2 = 3G/(3G+H), I_s = diag(1,1,1,1/2,1/2,1/2)),
-// which replaces the reference's :509-537.  st: sigma(6), peeq, eps_p(6).
-// Returns 0 elastic, 1 plastic, 2 failure.
-__device__ int j2_point(double E, double nu, double sy0, double H, const double* deps,
-                        double* st, double* Dt) {
+// (th1 = 1 - 3G dg/q, th2 = 3G/(3G + H'(peeq_n + dg)),
+// I_s = diag(1,1,1,1/2,1/2,1/2)), which replaces the reference's :509-537.
+// A point that has yielded and whose trial state lies on the yield surface
+// (f > -1e-8 sigma_y0: the zero strain increment of an increment's first
+// Newton iteration) gets the plastic branch's tangent at dg = 0 (th1 = 1,
+// th2 = 3G/(3G + H'(peeq_n))), the direction a loading step takes, instead of
+// D_el, which would make that iteration an elastic predictor; the stress is
+// the trial stress either way.
+// st: sigma(6), peeq, eps_p(6).  Returns 0 elastic, 1 plastic, 2 failure
+// (degenerate trial state, zero Jacobian or no local convergence: st and Dt
+// are then not meaningful).
+__device__ __forceinline__ void j2_tangent(double K, double G, double th1, double th2,
+                                           const double (&n)[6], double* Dt) {
+  const double cn = 2.0 * G * (th1 + th2 - 1.0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double v = (i < 3 && j < 3) ? K - 2.0 * G * th1 / 3.0 : 0.0;
+      if (i == j) v += 2.0 * G * th1 * (i < 3 ? 1.0 : 0.5);
+      Dt[i + 6 * j] = v - cn * n[i] * n[j];
+    }
+}
+
+__device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* st, double* Dt) {
+  const double E = pr.young, nu = pr.nu;
   const double mu = E / (2.0 * (1.0 + nu));
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   double strial[6];
@@ -946,32 +987,44 @@ __device__ int j2_point(double E, double nu, double sy0, double H, const double*
   double s[6] = {strial[0] - pm, strial[1] - pm, strial[2] - pm, strial[3], strial[4], strial[5]};
   const double q = sqrt(1.5) * sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] +
                                     2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
-  const double f = q - (sy0 + H * st[6]);
+  const double peeq_n = st[6];
+  const double f = q - j2_yield(pr, peeq_n);
+  const double G = mu, K = E / (3.0 * (1.0 - 2.0 * nu));
   if (f <= 0.0) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) st[i] = strial[i];
+    if (peeq_n > 0.0 && f > -1.0e-8 * pr.sigma_y0 && q >= 1.0e-30) {
+      double n0[6];
+      const double sn0 = q / sqrt(1.5);
 #pragma unroll
-    for (int j = 0; j < 6; ++j)
+      for (int i = 0; i < 6; ++i) n0[i] = s[i] / sn0;
+      j2_tangent(K, G, 1.0, 3.0 * G / (3.0 * G + j2_hard_tan(pr, peeq_n)), n0, Dt);
+    } else {
 #pragma unroll
-      for (int i = 0; i < 6; ++i)
-        Dt[i + 6 * j] = (i < 3 && j < 3 ? lam : 0.0) + (i == j ? (i < 3 ? 2.0 * mu : mu) : 0.0);
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+          Dt[i + 6 * j] = (i < 3 && j < 3 ? lam : 0.0) + (i == j ? (i < 3 ? 2.0 * mu : mu) : 0.0);
+    }
     return 0;
   }
   if (q < 1.0e-30) return 2;
-  const double G = mu;
   double n[6];
   const double snorm = q / sqrt(1.5);
 #pragma unroll
   for (int i = 0; i < 6; ++i) n[i] = s[i] / snorm;
-  double dg = f / (3.0 * G + H);
+  double dg = f / (3.0 * G + j2_hard_tan(pr, peeq_n));
+  const int max_it = pr.max_local_iter > 0 ? pr.max_local_iter : 25;
   int it;
-  for (it = 1; it <= 25; ++it) {
-    const double R = q - 3.0 * G * dg - (sy0 + H * (st[6] + dg));
-    if (fabs(R) < 1.0e-10 * sy0) break;
-    dg = dg - R / (-(3.0 * G + H));
+  for (it = 1; it <= max_it; ++it) {
+    const double R = q - 3.0 * G * dg - j2_yield(pr, peeq_n + dg);
+    if (fabs(R) < 1.0e-10 * pr.sigma_y0) break;
+    const double dR = -(3.0 * G + j2_hard_tan(pr, peeq_n + dg));
+    if (fabs(dR) < 1.0e-30) return 2;
+    dg = dg - R / dR;
     if (dg < 0.0) dg = 0.0;
   }
-  if (it > 25) return 2;
+  if (it > max_it) return 2;
   const double beta = 1.0 - 3.0 * G * dg / q;
   st[0] = beta * s[0] + pm;
   st[1] = beta * s[1] + pm;
@@ -979,20 +1032,10 @@ __device__ int j2_point(double E, double nu, double sy0, double H, const double*
   st[3] = beta * s[3];
   st[4] = beta * s[4];
   st[5] = beta * s[5];
-  st[6] += dg;
+  st[6] = peeq_n + dg;
 #pragma unroll
   for (int i = 0; i < 6; ++i) st[7 + i] += dg * n[i];
-  const double K = E / (3.0 * (1.0 - 2.0 * nu));
-  const double th1 = beta, th2 = 3.0 * G / (3.0 * G + H);
-  const double cn = 2.0 * G * (th1 + th2 - 1.0);
-#pragma unroll
-  for (int j = 0; j < 6; ++j)
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double v = (i < 3 && j < 3) ? K - 2.0 * G * th1 / 3.0 : 0.0;
-      if (i == j) v += 2.0 * G * th1 * (i < 3 ? 1.0 : 0.5);
-      Dt[i + 6 * j] = v - cn * n[i] * n[j];
-    }
+  j2_tangent(K, G, beta, 3.0 * G / (3.0 * G + j2_hard_tan(pr, st[6])), n, Dt);
   return 1;
 }
 
@@ -1034,7 +1077,7 @@ __global__ __launch_bounds__(128) void k_j2_update(int32_t n_elems,
 #pragma unroll
   for (int k = 0; k < 13; ++k) st[k] = commit[13 * idx + k];
   double Dt[36];
-  const int r = j2_point(pr.young, pr.nu, pr.sigma_y0, pr.hardening, deps, st, Dt);
+  const int r = j2_point(pr, deps, st, Dt);
 #pragma unroll
   for (int k = 0; k < 13; ++k) trial[13 * idx + k] = st[k];
 #pragma unroll

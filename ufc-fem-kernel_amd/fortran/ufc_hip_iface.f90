@@ -22,6 +22,8 @@ MODULE ufc_hip_iface
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_PREC_FP64 = 0, UFC_PREC_FP32 = 1, &
        UFC_PREC_MIXED = 2
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_MAT_ELASTIC = 0, UFC_MAT_J2 = 1
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_J2_HARD_LINEAR = 1, UFC_J2_HARD_SWIFT = 2, &
+                                           UFC_J2_HARD_VOCE = 3, UFC_J2_HARD_AF = 4
   ! C3D8 variants (PH_ELEM_C3D8_VARIANT_*, PH_Elem_C3D8.f90:111-116)
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_VARIANT_STANDARD = 1, UFC_VARIANT_REDUCED = 2, &
     UFC_VARIANT_SELECTIVE = 3, UFC_VARIANT_INCOMPAT = 4, UFC_VARIANT_27PT = 6
@@ -113,11 +115,22 @@ MODULE ufc_hip_iface
     INTEGER(C_INT32_T) :: x0_zero = 1
   END TYPE ufc_nm_solver_state
 
+  ! PH_J2_Props (PH_Mat_Plast_J2_Iso_Core.f90:52-91); hardening_type is one
+  ! of the PH_MAT_J2_HARD_* values (:33-36), 0 meaning linear
   TYPE, BIND(C), PUBLIC :: ufc_ph_j2_props
     REAL(C_DOUBLE) :: young = 0.0_C_DOUBLE
     REAL(C_DOUBLE) :: nu = 0.0_C_DOUBLE
     REAL(C_DOUBLE) :: sigma_y0 = 0.0_C_DOUBLE
     REAL(C_DOUBLE) :: hardening = 0.0_C_DOUBLE
+    INTEGER(C_INT32_T) :: hardening_type = UFC_J2_HARD_LINEAR
+    INTEGER(C_INT32_T) :: max_local_iter = 0       ! <= 0: 25
+    REAL(C_DOUBLE) :: swift_k = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: swift_n = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: swift_eps0 = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: voce_q = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: voce_delta = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: af_c = 0.0_C_DOUBLE
+    REAL(C_DOUBLE) :: af_gamma = 0.0_C_DOUBLE
   END TYPE ufc_ph_j2_props
 
   TYPE, BIND(C), PUBLIC :: ufc_timing

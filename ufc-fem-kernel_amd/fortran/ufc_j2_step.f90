@@ -21,16 +21,23 @@
 ! test here is relative to the internal force, which is scale-free.
 !
 ! Usage: ufc_j2_step <n> <precision> <increments> <u_out> [restart [mode [stop]]]
+!                    [key=value ...]
 !   unit cube of n^3 C3D8, interior nodes jittered by U[-0.1, 0.1] h (seeded),
 !   E = 200e9, nu = 0.3, sigma_y0 = 250e6, H = 2e9; z = 0 clamped, u_z on
 !   z = 1 raised to 0.005 (0.5% strain) in <increments> equal increments.
+!   key=value options (anywhere on the line): hard=linear|swift|voce (the
+!   PH_MAT_J2_HARD_* law; swift: sigma_y = K (eps0 + e)^n, voce: sigma_y =
+!   sigma_y0 + Q (1 - exp(-delta e))), sy0=, h=, k=, n=, eps0=, q=, delta=
+!   (law parameters), local_iter= (local Newton limit, 0: 25), max_newton=
+!   (Newton iterations per attempt, default 25).
 !   precision fp64 | mixed (the linear solve); u_out: n_dof doubles, stream.
 !   restart: checkpoint file written after every converged increment with u,
 !   the step data and the J2 state (ufc_rt_restart_save; RT_Step_Exec.f90:
 !   551-558); '-' for none.  mode: new (default) | resume (continue from the
 !   checkpoint at its load factor; increments resume at the nominal size).
 !   stop: end after that many converged increments in total (0: run to 1).
-! Prints one line per increment and a summary; exits 0 when every increment
+! Prints one line per increment, one per cut-back attempt (with its reason)
+! and a summary with the number of cut-backs; exits 0 when every increment
 ! converged, 1 otherwise, 2 on a C-ABI error.
 PROGRAM ufc_j2_step
   USE, INTRINSIC :: ISO_C_BINDING
@@ -39,7 +46,10 @@ PROGRAM ufc_j2_step
 
   INTEGER(C_INT64_T), PARAMETER :: mesh_seed = 20251205_C_INT64_T
   REAL(C_DOUBLE), PARAMETER :: strain = 0.005_C_DOUBLE, tol_r = 1.0E-8_C_DOUBLE
-  INTEGER, PARAMETER :: max_newton = 25, max_cutbacks = 4
+  INTEGER, PARAMETER :: max_cutbacks = 4
+  INTEGER :: max_newton, n_cut, npos, ia, eqpos
+  CHARACTER(LEN=64) :: key, reason
+  CHARACTER(LEN=512) :: pos(7)
 
   CHARACTER(LEN=64) :: arg, prec_name
   CHARACTER(LEN=512) :: outfile, rstfile
@@ -67,19 +77,64 @@ PROGRAM ufc_j2_step
   TYPE(ufc_nm_solver_state) :: sstate
   TYPE(ufc_ph_j2_props) :: props
 
-  CALL GET_COMMAND_ARGUMENT(1, arg); READ(arg, *) n
-  CALL GET_COMMAND_ARGUMENT(2, prec_name)
-  CALL GET_COMMAND_ARGUMENT(3, arg); READ(arg, *) n_inc
-  CALL GET_COMMAND_ARGUMENT(4, outfile)
+  ! material of the step (PH_J2_Props); key=value options may change it
+  props%young = 200.0E9_C_DOUBLE
+  props%nu = 0.3_C_DOUBLE
+  props%sigma_y0 = 250.0E6_C_DOUBLE
+  props%hardening = 2.0E9_C_DOUBLE
+  props%hardening_type = UFC_J2_HARD_LINEAR
+  max_newton = 25
+  npos = 0
+  pos = ' '
+  DO ia = 1, COMMAND_ARGUMENT_COUNT()
+    CALL GET_COMMAND_ARGUMENT(ia, outfile)
+    eqpos = INDEX(outfile, '=')
+    IF (eqpos == 0) THEN
+      npos = MIN(npos + 1, 7)
+      pos(npos) = outfile
+      CYCLE
+    END IF
+    key = outfile(1:eqpos - 1)
+    arg = outfile(eqpos + 1:)
+    SELECT CASE (TRIM(key))
+    CASE ('hard')
+      SELECT CASE (TRIM(arg))
+      CASE ('linear'); props%hardening_type = UFC_J2_HARD_LINEAR
+      CASE ('swift'); props%hardening_type = UFC_J2_HARD_SWIFT
+      CASE ('voce'); props%hardening_type = UFC_J2_HARD_VOCE
+      CASE DEFAULT
+        WRITE(*, '(A,A)') 'unknown hardening law ', TRIM(arg)
+        CALL EXIT(2)
+      END SELECT
+    CASE ('sy0'); READ(arg, *) props%sigma_y0
+    CASE ('h'); READ(arg, *) props%hardening
+    CASE ('k'); READ(arg, *) props%swift_k
+    CASE ('n'); READ(arg, *) props%swift_n
+    CASE ('eps0'); READ(arg, *) props%swift_eps0
+    CASE ('q'); READ(arg, *) props%voce_q
+    CASE ('delta'); READ(arg, *) props%voce_delta
+    CASE ('local_iter'); READ(arg, *) props%max_local_iter
+    CASE ('max_newton'); READ(arg, *) max_newton
+    CASE DEFAULT
+      WRITE(*, '(A,A)') 'unknown option ', TRIM(key)
+      CALL EXIT(2)
+    END SELECT
+  END DO
+  IF (npos < 4) THEN
+    WRITE(*, '(A)') 'usage: ufc_j2_step <n> <precision> <increments> <u_out> ' // &
+                    '[restart [mode [stop]]] [key=value ...]'
+    CALL EXIT(2)
+  END IF
+  READ(pos(1), *) n
+  prec_name = pos(2)
+  READ(pos(3), *) n_inc
+  outfile = pos(4)
   rstfile = '-'
   rstmode = 'new'
   stop_after = 0
-  IF (COMMAND_ARGUMENT_COUNT() >= 5) CALL GET_COMMAND_ARGUMENT(5, rstfile)
-  IF (COMMAND_ARGUMENT_COUNT() >= 6) CALL GET_COMMAND_ARGUMENT(6, rstmode)
-  IF (COMMAND_ARGUMENT_COUNT() >= 7) THEN
-    CALL GET_COMMAND_ARGUMENT(7, arg)
-    READ(arg, *) stop_after
-  END IF
+  IF (npos >= 5) rstfile = pos(5)
+  IF (npos >= 6) rstmode = pos(6)
+  IF (npos >= 7) READ(pos(7), *) stop_after
   prec = UFC_PREC_FP64
   IF (TRIM(prec_name) == 'mixed') prec = UFC_PREC_MIXED
 
@@ -121,10 +176,6 @@ PROGRAM ufc_j2_step
   mesh%conn = C_LOC(conn)
   rc = ufc_rt_asm_build_pattern(ctx, mesh, info, st)
   CALL check('ufc_rt_asm_build_pattern')
-  props%young = 200.0E9_C_DOUBLE
-  props%nu = 0.3_C_DOUBLE
-  props%sigma_y0 = 250.0E6_C_DOUBLE
-  props%hardening = 2.0E9_C_DOUBLE
   rc = ufc_ph_j2_init(ctx, props, st)
   CALL check('ufc_ph_j2_init')
 
@@ -142,6 +193,7 @@ PROGRAM ufc_j2_step
   lam = 0.0_C_DOUBLE
   step = 1.0_C_DOUBLE / REAL(n_inc, C_DOUBLE)
   cutbacks = 0
+  n_cut = 0
   n_done = 0
   total_newton = 0
   total_linear = 0
@@ -171,10 +223,12 @@ PROGRAM ufc_j2_step
     rrel = HUGE(1.0_C_DOUBLE)
     r_prev = -1.0_C_DOUBLE
     eta_prev = -1.0_C_DOUBLE
+    reason = 'newton'
     DO it = 1, max_newton
       rc = ufc_ph_j2_update(ctx, C_LOC(u_try), C_LOC(u_n), n_yield, st)
       IF (rc /= UFC_STATUS_OK) THEN            ! local Newton failure: cut back
         WRITE(*, '(A,I0,A,A)') 'ufc_ph_j2_update status ', rc, ': ', TRIM(ufc_status_message(st))
+        reason = 'local'
         EXIT
       END IF
       rc = ufc_rt_residual(ctx, C_LOC(u_try), C_LOC(r), C_LOC(norms), st)
@@ -211,6 +265,7 @@ PROGRAM ufc_j2_step
       lin = lin + sstate%niter
       IF (rc /= UFC_STATUS_OK) THEN
         WRITE(*, '(A,I0,A,A)') 'ufc_nm_pcg status ', rc, ': ', TRIM(ufc_status_message(st))
+        reason = 'linear'
         EXIT
       END IF
       u_try = u_try + du
@@ -238,9 +293,12 @@ PROGRAM ufc_j2_step
         CALL check('ufc_rt_restart_save')
       END IF
     ELSE
-      WRITE(*, '(A,F8.5,A,I0,A,ES10.3)') 'cutback at lambda ', target, ' after newton ', it, &
-           ' rrel ', rrel
+      ! the trial state is discarded: the next attempt starts again from the
+      ! committed state and u_n (RT_Step_Exec.f90:455-471)
+      WRITE(*, '(A,F8.5,A,I0,A,ES10.3,A,A)') 'cutback at lambda ', target, ' after newton ', &
+           MIN(it, max_newton), ' rrel ', rrel, ' reason ', TRIM(reason)
       cutbacks = cutbacks + 1
+      n_cut = n_cut + 1
       IF (cutbacks > max_cutbacks) THEN
         failed = .TRUE.
         EXIT
@@ -253,8 +311,9 @@ PROGRAM ufc_j2_step
        STATUS='REPLACE', ACTION='WRITE')
   WRITE(u_unit) u
   CLOSE(u_unit)
-  WRITE(*, '(A,I0,A,I0,A,I0,A,I0,A,L1)') 'ufc_j2_step n_dof ', info%n_dof, ' increments ', &
-       n_done, ' newton ', total_newton, ' linear ', total_linear, ' converged ', .NOT. failed
+  WRITE(*, '(A,I0,A,I0,A,I0,A,I0,A,I0,A,L1)') 'ufc_j2_step n_dof ', info%n_dof, &
+       ' increments ', n_done, ' newton ', total_newton, ' linear ', total_linear, &
+       ' cutbacks ', n_cut, ' converged ', .NOT. failed
   rc = ufc_ctx_destroy(ctx, st)
   IF (failed) CALL EXIT(1)
 

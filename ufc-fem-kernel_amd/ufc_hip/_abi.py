@@ -44,6 +44,12 @@ VARIANT_SELECTIVE = 3
 VARIANT_INCOMPAT = 4
 VARIANT_27PT = 6
 
+# J2 hardening laws (PH_MAT_J2_HARD_*, PH_Mat_Plast_J2_Iso_Core.f90:33-36)
+J2_HARD_LINEAR = 1
+J2_HARD_SWIFT = 2
+J2_HARD_VOCE = 3
+J2_HARD_AF = 4
+
 LOAD_TRACTION = 0
 LOAD_PRESSURE = 1
 
@@ -82,7 +88,10 @@ class SensArgs(C.Structure):
 
 class J2Props(C.Structure):
     _fields_ = [("young", C.c_double), ("nu", C.c_double), ("sigma_y0", C.c_double),
-                ("hardening", C.c_double)]
+                ("hardening", C.c_double), ("hardening_type", C.c_int32),
+                ("max_local_iter", C.c_int32), ("swift_k", C.c_double),
+                ("swift_n", C.c_double), ("swift_eps0", C.c_double), ("voce_q", C.c_double),
+                ("voce_delta", C.c_double), ("af_c", C.c_double), ("af_gamma", C.c_double)]
 
 
 class AsmAlgo(C.Structure):

@@ -39,6 +39,7 @@ class IncrementInfo:
     linear_iters: int
     rnorm_rel: float
     n_yield: int
+    history: list = field(default_factory=list)  # ||R|| / ||F_int|| per Newton iteration
 
 
 @dataclass
@@ -46,6 +47,8 @@ class NewtonResult:
     converged: bool
     increments: list = field(default_factory=list)
     u: np.ndarray = None
+    cutbacks: int = 0          # failed attempts cut back by 0.25 over the job
+    failures: list = field(default_factory=list)  # (load factor tried, reason) per cut
 
 
 def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
@@ -73,6 +76,11 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
               restart_file=None, checkpoint_freq=1, resume_from=None, stop_after=None):
     """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
     in n_increments equal increments.  ctx must hold the mesh (build_pattern).
+    props: (E, nu, sigma_y0, H) or a dict of Context.j2_init keywords (the
+    hardening law, max_local_iter).  A failed attempt (local-Newton failure,
+    no Newton convergence in max_newton iterations, a failed linear solve) is
+    retried from the committed state with the step cut by 0.25, at most
+    max_cutbacks times in a row; res.cutbacks counts them over the job.
     lin_rtol None: inexact Newton, each linear solve to the forcing term of
     forcing_term(); a number: every linear solve to that relative tolerance.
 
@@ -84,7 +92,12 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     in total (a job that ends early, for restarts)."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
-    ctx.j2_init(*props)
+    if isinstance(props, dict):
+        ctx.j2_init(**props)
+        nu = props["nu"]
+    else:
+        ctx.j2_init(*props)
+        nu = props[1]
     n = ctx.n_dof
     u = np.zeros(n)
     lam = 0.0
@@ -101,17 +114,21 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
         ctx.apply_bc(bc_eq, bc_target * target, penalty)
         u_n = u.copy()
         ok, its, lin, rrel, ny, rabs = False, 0, 0, np.inf, 0, np.inf
+        hist = []
         u_try = u_n.copy()
         r_prev = eta = None
+        reason = f"no convergence in {max_newton} Newton iterations"
         for it in range(1, max_newton + 1):
             try:
                 ny = ctx.j2_update(u_try, u_n)
-            except A.UfcError:
-                break  # local Newton failure: cut back
+            except A.UfcError as e:
+                reason = f"local Newton failure ({e.message})"
+                break  # cut back
             r, norms = ctx.residual(u_try)
             its = it
             rabs = norms[0]
             rrel = norms[0] / norms[1] if norms[1] > 0.0 else float("inf")
+            hist.append(rrel)
             if it > 1 and norms[0] <= tol_r * max(norms[1], 1e-300):
                 ok = True
                 break
@@ -120,10 +137,11 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             else:
                 eta = lin_rtol
             r_prev = norms[0]
-            ctx.assemble_k(None, props[1], precision=precision, material=A.MAT_J2)
+            ctx.assemble_k(None, nu, precision=precision, material=A.MAT_J2)
             du, info = ctx.pcg(precision=precision, rtol=eta, b=r)
             lin += info.niter
             if not info.converged:
+                reason = f"linear solve status {info.status}"
                 break
             u_try = u_try + du
         if ok:
@@ -131,13 +149,17 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             u = u_try
             lam = target
             count += 1
-            res.increments.append(IncrementInfo(lam, its, lin, rrel, ny))
+            res.increments.append(IncrementInfo(lam, its, lin, rrel, ny, hist))
             cutbacks = 0
             if restart_file is not None and count % checkpoint_freq == 0:
                 R.save(restart_file, u, R.StepState(lam, count, lam, rabs, its, True), ctx=ctx,
                        with_state=True)
         else:
+            # the trial state is discarded: the next attempt starts again from
+            # the committed state and u_n (RT_Step_Exec.f90:455-471)
             cutbacks += 1
+            res.cutbacks += 1
+            res.failures.append((target, reason))
             if cutbacks > max_cutbacks:
                 res.converged = False
                 break

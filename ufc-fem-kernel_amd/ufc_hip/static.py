@@ -259,8 +259,15 @@ class Context:
         self._call("ufc_ph_c3d8_sensitivity", C.byref(a))
         return out
 
-    def j2_init(self, young, nu, sigma_y0, hardening):
-        p = A.J2Props(young, nu, sigma_y0, hardening)
+    def j2_init(self, young, nu, sigma_y0, hardening=0.0, hardening_type=A.J2_HARD_LINEAR,
+                max_local_iter=0, swift_k=0.0, swift_n=0.0, swift_eps0=0.0, voce_q=0.0,
+                voce_delta=0.0, af_c=0.0, af_gamma=0.0):
+        """ufc_ph_j2_init: zero J2 state with the hardening law of
+        PH_J2_ComputeHardening (linear H, Swift K (eps0 + e)^n or Voce
+        sigma_y0 + Q (1 - exp(-delta e))); max_local_iter bounds the local
+        Newton (0: 25)."""
+        p = A.J2Props(young, nu, sigma_y0, hardening, int(hardening_type), int(max_local_iter),
+                      swift_k, swift_n, swift_eps0, voce_q, voce_delta, af_c, af_gamma)
         self._call("ufc_ph_j2_init", C.byref(p))
 
     def j2_update(self, u, u_n):
