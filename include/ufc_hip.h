@@ -168,6 +168,20 @@ typedef struct ufc_ph_ke_args {
 int ufc_ph_c3d8_eval_ke(ufc_ctx* ctx, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* args,
                         ufc_status* st);
 
+/* ufc_ph_c3d8_eval_ke without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  args->coords, young and
+ * ke must be device memory of the context's GPU (host arrays return
+ * UFC_STATUS_INVALID: staging them would allocate).  The incompatible-mode
+ * element's condensation factor (24 x 9 doubles per element) goes to a
+ * static 16 MiB device array that each GPU holds until the process ends, so
+ * a UFC_VARIANT_INCOMPAT batch is limited to 9709 elements per call
+ * (UFC_STATUS_INVALID beyond); the other variants need no scratch and take
+ * any n_elems.  Results are bitwise those of ufc_ph_c3d8_eval_ke.  Calls that
+ * overlap on one GPU (several contexts or threads) race on the array. */
+int ufc_ph_c3d8_eval_ke_noalloc(ufc_ctx* ctx, const ufc_ph_ke_algo* algo, ufc_ph_ke_args* args,
+                                ufc_status* st);
+
 /* Adjoint element sensitivities for E_e = E_e(rho_e):
  *   grad[e] = -dEdrho[e] * lambda_e^T K_e(E=1) u_e,
  * recomputing K_e(E=1) instead of storing it (after an elastic assembly

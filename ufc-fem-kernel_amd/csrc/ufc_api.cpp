@@ -337,6 +337,7 @@ int ufc_ctx_create(int32_t device, ufc_ctx** ctx, ufc_status* st) {
     UFC_HIP_CHECK(hipGetDeviceCount(&ndev));
     UFC_REQUIRE(device >= 0 && device < ndev, UFC_STATUS_INVALID, "no such HIP device");
     UFC_HIP_CHECK(hipSetDevice(device));
+    ufc::noalloc_setup(device);
     auto* c = new ufc_ctx();
     c->device = device;
     try {

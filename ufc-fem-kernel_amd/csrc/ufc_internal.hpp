@@ -306,6 +306,9 @@ struct ufc_ctx {
 namespace ufc {
 
 // ---------------------------------------------------------------- launchers
+// device address of the allocation-free entry point's scratch array on the
+// current device, resolved once per device (ufc_noalloc.hip)
+void noalloc_setup(int device);
 // element kernels (ufc_elem.hip)
 void launch_eval_ke(int32_t n_elems, const double* coords, const double* young, double nu,
                     void* ke, bool fp32, int variant, double* inc_c, hipStream_t s);

@@ -159,6 +159,7 @@ _SIGS = {
     "ufc_ctx_timing": [C.c_void_p, P(Timing), P(Status)],
     "ufc_ctx_enable_timing": [C.c_void_p, C.c_int32, P(Status)],
     "ufc_ph_c3d8_eval_ke": [C.c_void_p, P(KeAlgo), P(KeArgs), P(Status)],
+    "ufc_ph_c3d8_eval_ke_noalloc": [C.c_void_p, P(KeAlgo), P(KeArgs), P(Status)],
     "ufc_ph_c3d8_sensitivity": [C.c_void_p, P(SensArgs), P(Status)],
     "ufc_ph_j2_init": [C.c_void_p, P(J2Props), P(Status)],
     "ufc_ph_j2_update": [C.c_void_p, C.c_void_p, C.c_void_p, P(C.c_int64), P(Status)],

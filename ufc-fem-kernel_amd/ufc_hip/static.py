@@ -251,6 +251,22 @@ class Context:
         self._call("ufc_ph_c3d8_eval_ke", C.byref(algo), C.byref(args))
         return np.transpose(ke, (0, 2, 1))  # column-major per element
 
+    def eval_ke_noalloc(self, xe, young, nu, ke, precision=A.PREC_FP64,
+                        variant=A.VARIANT_STANDARD):
+        """ufc_ph_c3d8_eval_ke_noalloc on device tensors: xe (n,8,3) float64,
+        young (n,) float64 and ke (n,24,24) float64/float32, all contiguous on
+        the context's GPU; Ke is written column-major per element, as
+        eval_ke's buffer.  Allocates nothing on the device."""
+        for t in (xe, young, ke):
+            assert _is_tensor(t) and t.is_cuda and t.is_contiguous(), \
+                "eval_ke_noalloc takes contiguous device tensors"
+        torch.cuda.current_stream(ke.device).synchronize()
+        algo = A.KeAlgo(precision, variant)
+        args = A.KeArgs(int(xe.shape[0]), 0, C.c_void_p(xe.data_ptr()),
+                        C.c_void_p(young.data_ptr()), nu, C.c_void_p(ke.data_ptr()))
+        self._call("ufc_ph_c3d8_eval_ke_noalloc", C.byref(algo), C.byref(args))
+        return ke
+
     def sensitivity(self, u, lam, dedrho, nu, out=None):
         arg = _Arg()
         out = np.empty(self.n_elems) if out is None else out
