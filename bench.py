@@ -193,6 +193,106 @@ def cpu_baseline(pr, gpu_sols, repeats=5, rtol_full=1e-12):
     return out, errs
 
 
+# J2 update per Gauss point, algorithmic: committed state read and trial
+# state written (13 doubles each), tangent written (36 doubles), flag (4 B);
+# per element its connectivity, per node coordinates, u and u_n
+J2_BYTES_PER_GP = 13 * 8 * 2 + 36 * 8 + 4
+# FLOPs per Gauss point: J = dN/dxi x (9 x 8 FMA = 144), det + inverse (50),
+# dN/dx = J^-T dN/dxi (8 x 3 x 5 = 120), strain increment from 8 nodes
+# (8 x 27 = 216), elastic predictor, deviator and q (40), radial return and
+# state update (40), consistent tangent (36 x 4 + 26 = 170)
+J2_FLOP_PER_GP = 144 + 50 + 120 + 216 + 40 + 40 + 170
+
+
+def j2_roofline(n_elems, n_nodes, ms):
+    """HBM roofline of one k_j2_update launch (the bound: ~0.6 FLOP per byte
+    against FP64 VALU's ~10 per byte of HBM)."""
+    ngp = 8 * n_elems
+    b = ngp * J2_BYTES_PER_GP + n_elems * 32 + n_nodes * 72
+    f = ngp * J2_FLOP_PER_GP
+    ach = b / (ms * 1e6)
+    return {"kernel": "k_j2_update", "bound": "hbm", "bytes_per_launch": b, "ms_per_launch": ms,
+            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "flop_per_launch": f, "fp64_tflops": f / (ms * 1e9),
+            "fp64_frac_of_78.6": f / (ms * 1e9) / 78.6}
+
+
+def cpu_adjoint_baseline(pa):
+    """Config-4 workload on the FP64 oracle (threads as cpu_baseline), not
+    sampled: assembly, forward Jacobi-PCG to 1e-10, the adjoint solve
+    K^T lambda = f (NM_Adjoint_Solve on symmetric K, Jacobi-PCG from 0 to
+    1e-10) and the element sensitivities."""
+    import numpy as np
+    from oracle import oracle as O
+    threads, aff = cpu_threads()
+    rp, ci = O.csr_pattern(pa.coords.shape[0], pa.conn)
+    t0 = time.perf_counter()
+    val = O.assemble(pa.conn, pa.coords, pa.young, pa.nu, rp, ci, nthreads=threads)
+    t1 = time.perf_counter()
+    F = np.zeros(pa.n_dof)
+    O.face_loads_pressure(pa.load_elem, pa.load_face, pa.conn, pa.coords, pa.load_value[0], F)
+    f0 = F.copy()
+    O.apply_penalty(pa.bc_eq, pa.bc_val, 1e12, rp, ci, val, F)
+    log("cpu adjoint workload: forward PCG")
+    u, st, it_f, _, _, _ = O.pcg_mark(rp, ci, val, F, 1e-10, 0.0, nthreads=threads,
+                                      report_every=500)
+    t2 = time.perf_counter()
+    log("cpu adjoint workload: adjoint PCG")
+    lam, st2, it_a, _, _, _ = O.pcg_mark(rp, ci, val, f0, 1e-10, 0.0, nthreads=threads,
+                                         report_every=500)
+    t3 = time.perf_counter()
+    rho = pa.extra["rho"]
+    g = O.sensitivity(pa.conn, pa.coords, pa.nu, 3.0 * rho ** 2 * 1e9, u, lam, nthreads=threads)
+    t4 = time.perf_counter()
+    if st != 0 or st2 != 0:
+        raise RuntimeError("CPU oracle adjoint workload did not converge")
+    return {"kind": "port", "cores": threads,
+            "sample": (f"FP64 C oracle ({threads} threads) on the same {pa.nx}^3 SIMP cube, not "
+                       "sampled: assembly, forward PCG to 1e-10, adjoint PCG to 1e-10, "
+                       "sensitivity kernel"),
+            "forward_ms": (t2 - t0) * 1e3, "grad_ms": (t4 - t2) * 1e3,
+            "assemble_ms": (t1 - t0) * 1e3, "forward_iters": int(it_f),
+            "adjoint_iters": int(it_a), "sensitivity_ms": (t4 - t3) * 1e3,
+            "grad_checksum": float(np.abs(g).sum())}
+
+
+def cpu_j2_baseline(pj, props, u_end, n_newton, n_linear):
+    """Config-5 workload on the FP64 oracle, a bounded sample projected to
+    the GPU job's counts: one J2 update + internal force over all Gauss
+    points (a strain increment from the unloaded state to the job's final
+    displacement, every point on the plastic branch), one tangent assembly
+    and 20 Jacobi-PCG iterations on that tangent, all with the CPU share of
+    threads; CPU job = n_newton x (update + assembly) + n_linear x PCG
+    iteration."""
+    import numpy as np
+    from oracle import oracle as O
+    threads, aff = cpu_threads()
+    nn = pj.coords.shape[0]
+    rp, ci = O.csr_pattern(nn, pj.conn)
+    s0 = np.zeros(pj.n_elems * 8 * 13)
+    t0 = time.perf_counter()
+    nf, st, Dgp, Fi = O.j2_internal_force(nn, pj.conn, pj.coords, props, u_end,
+                                          np.zeros(pj.n_dof), s0, nthreads=threads)
+    t1 = time.perf_counter()
+    val = O.assemble_dgp(pj.conn, pj.coords, Dgp, rp, ci, nthreads=threads)
+    t2 = time.perf_counter()
+    O.apply_penalty(pj.bc_eq, pj.bc_val, 1e12, rp, ci, val, None)
+    t3 = time.perf_counter()
+    _, _, it, _ = O.pcg(rp, ci, val, Fi, rtol=1e-30, max_iter=20, nthreads=threads)
+    t4 = time.perf_counter()
+    per_it = (t4 - t3) / max(it, 1)
+    job = n_newton * ((t1 - t0) + (t2 - t1)) + n_linear * per_it
+    return {"kind": "port", "cores": threads,
+            "sample": (f"FP64 C oracle ({threads} threads) on the same {pj.nx}^3 mesh: one J2 "
+                       f"update + internal force ({(t1 - t0) * 1e3:.0f} ms, "
+                       f"{int(np.count_nonzero(st.reshape(-1, 13)[:, 6]))} plastic points), one "
+                       f"tangent assembly ({(t2 - t1) * 1e3:.0f} ms), {it} PCG iterations "
+                       f"({per_it * 1e3:.2f} ms each); projected to the GPU job's "
+                       f"{n_newton} Newton and {n_linear} linear iterations"),
+            "update_ms": (t1 - t0) * 1e3, "assemble_ms": (t2 - t1) * 1e3,
+            "pcg_iter_ms": per_it * 1e3, "job_s_projected": job}
+
+
 def copy_bandwidth(torch, dev, nbytes=1 << 31):
     """Device-to-device copy of nbytes (reads and writes nbytes): the
     attainable-bandwidth denominator beside the 8 TB/s peak."""
@@ -519,6 +619,9 @@ def main():
             log(f"fd check: max rel err {max(errs):.2e}")
             m64.close()
         model.close()
+        if not args.no_cpu:
+            out["adjoint"]["cpu_baseline"] = cpu_adjoint_baseline(pa)
+            log(f"cpu adjoint: {out['adjoint']['cpu_baseline']}")
 
     if not args.no_j2:
         from ufc_hip.nonlinear import j2_static
@@ -568,6 +671,20 @@ def main():
                      "u_exact_vs_inexact_rel": float(np.linalg.norm(res.u - res_ex.u) /
                                                      np.linalg.norm(res_ex.u))}
         log(f"J2: {(t1 - t0):.2f} s, newton {out['j2']['newton_iters']}")
+        upd = res.j2_update_ms + res_ex.j2_update_ms
+        out["j2"]["roofline"] = j2_roofline(pj.n_elems, pj.coords.shape[0],
+                                            float(np.median(upd)))
+        out["j2"]["roofline"]["launches"] = len(upd)
+        out["j2"]["tangent_assembly_ms_median"] = float(np.median(res.assemble_ms))
+        log(f"J2 update {np.median(upd):.3f} ms, tangent assembly "
+            f"{np.median(res.assemble_ms):.3f} ms")
+        if not args.no_cpu:
+            n_newton = int(sum(i.newton_iters for i in res.increments))
+            cb = cpu_j2_baseline(pj, pj.extra["props"], res.u, n_newton,
+                                 out["j2"]["linear_iters"])
+            cb["value_job_s_gpu"] = (t1 - t0)
+            out["j2"]["cpu_baseline"] = cb
+            log(f"cpu J2: {cb}")
 
     if not args.no_cpu:
         log("cpu baseline + full-size errors against the oracle")

@@ -295,7 +295,11 @@ typedef struct ufc_rt_asm_args {
 int ufc_rt_asm_assemble_k(ufc_ctx* ctx, const ufc_rt_asm_algo* algo,
                           const ufc_rt_asm_args* args, ufc_status* st);
 
-/* Copies the values of precision `precision` (FP64 -> double, FP32 -> float). */
+/* Copies the values of precision `precision` (FP64 -> double, FP32 -> float)
+ * in CSR order.  FP64: K itself.  FP32: K in the difference form the FP32
+ * SpMV applies -- every node's 3x3 self block is zero apart from the penalty
+ * diagonal; the self blocks of K follow from K 1 = 0 per block row,
+ * K(n,n) = -sum_{m != n} K(n,m). */
 int ufc_rt_asm_get_values(ufc_ctx* ctx, int32_t precision, void* values, ufc_status* st);
 
 /* Dirichlet conditions as diagonal penalties, RT_Asm_ApplyBC _Idx path
@@ -373,8 +377,11 @@ typedef struct ufc_nm_solver_algo {
   double rtol;         /* ||r||/||b|| target (outer, FP64, for MIXED)        */
   double inner_rtol;   /* MIXED: inner FP32 PCG tolerance                    */
   int32_t max_outer;   /* MIXED: refinement steps                            */
-  int32_t check_every; /* host poll interval in iterations (0 = 32); the    */
-                       /* stopping iteration is decided on the device        */
+  int32_t check_every; /* 0: the GPU reports each finished iteration into   */
+                       /* pinned host memory and the host keeps 3 queued;    */
+                       /* > 0: the host polls after every check_every        */
+                       /* iterations; either way the device decides the      */
+                       /* stopping iteration                                 */
   double replace_rtol; /* MIXED: inside each inner solve the FP32 residual  */
                        /* is replaced by the FP64 b - K u every time it has  */
                        /* dropped by replace_rtol (<= 0: 0.1), and the inner */
@@ -425,8 +432,9 @@ int ufc_nm_adjoint_solve(ufc_ctx* ctx, const ufc_nm_solver_algo* algo, const dou
  * NM_CSR_Transpose + SpMV (NM_Mtx_SpMVCSRTranspose.f90:46-145). */
 int ufc_nm_spmv_t(ufc_ctx* ctx, int32_t precision, const double* x, double* y, ufc_status* st);
 
-/* Device time in ms of the last assembly and the last solve on this context
- * (HIP events on the context's stream), and the PCG iteration counts. */
+/* Device time in ms of the last assembly, the last solve and the last J2
+ * update on this context (HIP events on the context's stream), and the PCG
+ * iteration counts. */
 typedef struct ufc_timing {
   double assemble_ms;
   double solve_ms;
@@ -434,6 +442,7 @@ typedef struct ufc_timing {
   int32_t spmv_calls;
   int32_t n_replace;   /* MIXED: FP64 residual replacements inside the inner */
                        /* solves of the last solve                           */
+  double j2_update_ms; /* device time of the last ufc_ph_j2_update's kernel  */
 } ufc_timing;
 int ufc_ctx_timing(ufc_ctx* ctx, ufc_timing* t, ufc_status* st);
 /* per_kernel = N > 0 brackets every N-th SpMV launch with a HIP event pair

@@ -68,7 +68,7 @@ def lib():
         L.orc_assemble.argtypes = [C.c_int32, vp, vp, vp, C.c_double, C.c_int32, vp, vp, vp,
                                    C.c_int]
         L.orc_assemble_dgp.restype = C.c_int
-        L.orc_assemble_dgp.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, vp, vp, vp]
+        L.orc_assemble_dgp.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, vp, vp, vp, C.c_int]
         L.orc_apply_penalty.restype = C.c_int
         L.orc_apply_penalty.argtypes = [C.c_int32, vp, vp, C.c_double, vp, vp, vp, vp]
         L.orc_face_loads_traction.argtypes = [C.c_int32, vp, vp, vp, vp, vp, vp]
@@ -85,7 +85,10 @@ def lib():
         L.orc_cg_transpose.restype = C.c_int
         L.orc_cg_transpose.argtypes = [C.c_int32, vp, vp, vp, vp, vp, C.c_double, C.c_int32,
                                        vp, vp]
-        L.orc_sensitivity.argtypes = [C.c_int32, vp, vp, C.c_double, vp, vp, vp, vp]
+        L.orc_sensitivity.argtypes = [C.c_int32, vp, vp, C.c_double, vp, vp, vp, vp, C.c_int]
+        L.orc_j2_internal_force.restype = C.c_int64
+        L.orc_j2_internal_force.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp,
+                                            vp, vp, C.c_int]
         L.orc_j2_update.restype = C.c_int
         L.orc_j2_update.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_j2_update_ex.restype = C.c_int
@@ -259,11 +262,11 @@ def assemble(conn, coords, E, nu, rp, ci, nthreads=1):
     return val
 
 
-def assemble_dgp(conn, coords, Dgp_colmajor, rp, ci):
+def assemble_dgp(conn, coords, Dgp_colmajor, rp, ci, nthreads=1):
     conn, coords, D = _i32(conn), _f64(coords), _f64(Dgp_colmajor)
     val = np.empty(ci.size)
     lib().orc_assemble_dgp(conn.shape[0], _p(conn), _p(coords), _p(D), rp.size - 1, _p(rp),
-                           _p(ci), _p(val))
+                           _p(ci), _p(val), nthreads)
     return val
 
 
@@ -333,12 +336,27 @@ def cg_transpose(rp, ci, val, rhs, tol=1e-8, max_iter=1000):
     return lam, st, ni.value, rel.value
 
 
-def sensitivity(conn, coords, nu, dEdrho, u, lam):
+def sensitivity(conn, coords, nu, dEdrho, u, lam, nthreads=1):
     conn, coords = _i32(conn), _f64(coords)
     g = np.empty(conn.shape[0])
-    lib().orc_sensitivity(conn.shape[0], _p(conn), _p(coords), nu, _p(_f64(dEdrho)),
-                          _p(_f64(u)), _p(_f64(lam)), _p(g))
+    dE, uu, ll = _f64(dEdrho), _f64(u), _f64(lam)
+    lib().orc_sensitivity(conn.shape[0], _p(conn), _p(coords), nu, _p(dE), _p(uu), _p(ll),
+                          _p(g), nthreads)
     return g
+
+
+def j2_internal_force(n_nodes, conn, coords, props, u, u_n, state_n, nthreads=1):
+    """orc_j2_internal_force: (local failures, trial state, Dgp (6,6,8,ne)
+    column-major per point, F_int)."""
+    conn, coords = _i32(conn), _f64(coords)
+    pr, uu, un, sn = j2_props(props), _f64(u), _f64(u_n), _f64(state_n)
+    state = np.empty_like(sn)
+    Dgp = np.empty(288 * conn.shape[0])
+    Fi = np.empty(3 * n_nodes)
+    nf = lib().orc_j2_internal_force(n_nodes, conn.shape[0], _p(conn), _p(coords), _p(pr),
+                                     _p(uu), _p(un), _p(sn), _p(state), _p(Dgp), _p(Fi),
+                                     nthreads)
+    return nf, state, Dgp, Fi
 
 
 # ----------------------------------------------------------------- J2
@@ -387,7 +405,8 @@ def j2_newton_step(n_nodes, conn, coords, props, bc_eq, bc_val, penalty, state_n
     u = _f64(u).copy()
     nn = C.c_int32()
     rn = C.c_double()
-    st = lib().orc_j2_newton_step(n_nodes, conn.shape[0], _p(conn), _p(coords), _p(j2_props(props)),
+    pr = j2_props(props)
+    st = lib().orc_j2_newton_step(n_nodes, conn.shape[0], _p(conn), _p(coords), _p(pr),
                                   bc_eq.size, _p(bc_eq), _p(bc_val), penalty, _p(state_n),
                                   _p(state), _p(u), max_newton, tol_r, pcg_rtol, C.byref(nn),
                                   C.byref(rn))

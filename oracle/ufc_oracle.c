@@ -743,13 +743,17 @@ int orc_assemble(int32_t n_elems, const int32_t* conn, const double* coords,
   return bad;
 }
 
-/* the same with one D per Gauss point: Dgp is (6,6,8,n_elems) */
+/* the same with one D per Gauss point: Dgp is (6,6,8,n_elems); nthreads as
+ * orc_assemble */
 int orc_assemble_dgp(int32_t n_elems, const int32_t* conn, const double* coords,
                      const double* Dgp, int32_t n_rows, const int32_t* row_ptr,
-                     const int32_t* col_idx, double* val) {
+                     const int32_t* col_idx, double* val, int nthreads) {
   int64_t nnz = row_ptr[n_rows];
   memset(val, 0, sizeof(double) * (size_t)nnz);
   int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : bad)
+#endif
   for (int64_t e = 0; e < n_elems; ++e) {
     double xe[24], Ke[576];
     gather_coords(conn, coords, e, xe);
@@ -760,7 +764,14 @@ int orc_assemble_dgp(int32_t n_elems, const int32_t* conn, const double* coords,
         int32_t ri = 3 * conn[8 * e + i / 3] + i % 3;
         int64_t s = find_slot(row_ptr, col_idx, ri, cj);
         if (s < 0) { bad++; continue; }
-        val[s] += Ke[i + 24 * j];
+        if (nthreads > 1) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+          val[s] += Ke[i + 24 * j];
+        } else {
+          val[s] += Ke[i + 24 * j];
+        }
       }
     }
   }
@@ -1006,7 +1017,11 @@ int orc_cg_transpose(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
 /* kernel exists in the reference; this is its definition.                  */
 /* ========================================================================= */
 void orc_sensitivity(int32_t n_elems, const int32_t* conn, const double* coords, double nu,
-                     const double* dEdrho, const double* u, const double* lam, double* g) {
+                     const double* dEdrho, const double* u, const double* lam, double* g,
+                     int nthreads) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
   for (int64_t e = 0; e < n_elems; ++e) {
     double xe[24], Ke[576], ue[24], le[24];
     gather_coords(conn, coords, e, xe);
@@ -1159,6 +1174,47 @@ int orc_j2_update(const double* props, const double* deps, double* sigma, double
   return orc_j2_update_ex(props, deps, sigma, peeq, eps_p, Dt, yielded, NULL);
 }
 
+/* Strain increments B(u - u_n) at every Gauss point, the J2 update from the
+ * committed state state_n into state (with the tangent into Dgp,
+ * (6,6,8,n_elems)), and the internal force F_int = sum_e B^T sigma into Fi
+ * (RT_Asm_ComputeResidual, RT_Asm_Solv.f90:1311-1426): elements in parallel
+ * (nthreads), element forces added in element order.  Returns the number of
+ * Gauss points whose local Newton failed. */
+int64_t orc_j2_internal_force(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
+                              const double* coords, const double* props, const double* u,
+                              const double* u_n, const double* state_n, double* state,
+                              double* Dgp, double* Fi, int nthreads) {
+  double* Fe_all = (double*)malloc(sizeof(double) * 24 * (size_t)n_elems);
+  int64_t local_fail = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : local_fail)
+#endif
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double xe[24], due[24], deps[48], sig[48];
+    gather_coords(conn, coords, e, xe);
+    for (int j = 0; j < 24; ++j) {
+      int64_t q = 3 * (int64_t)conn[8 * e + j / 3] + j % 3;
+      due[j] = u[q] - u_n[q];
+    }
+    orc_c3d8_strain(xe, due, deps);
+    for (int g = 0; g < 8; ++g) {
+      const double* sn = state_n + 13 * (8 * e + g);
+      double* sc = state + 13 * (8 * e + g);
+      memcpy(sc, sn, 13 * sizeof(double));
+      if (orc_j2_update(props, deps + 6 * g, sc, sc + 6, sc + 7, Dgp + 288 * e + 36 * g,
+                        NULL) != 0)
+        local_fail++;
+      memcpy(sig + 6 * g, sc, 6 * sizeof(double));
+    }
+    orc_c3d8_fint(xe, sig, Fe_all + 24 * e);
+  }
+  memset(Fi, 0, sizeof(double) * 3 * (size_t)n_nodes);
+  for (int64_t e = 0; e < n_elems; ++e)
+    for (int j = 0; j < 24; ++j) Fi[3 * (int64_t)conn[8 * e + j / 3] + j % 3] += Fe_all[24 * e + j];
+  free(Fe_all);
+  return local_fail;
+}
+
 /* One load increment of the small-strain static Newton loop
  * (RT_NLSolver_NewtonRaph, RT_Solv_Nonlin.f90:807-1000): per iteration
  * strain increments from u - u_n, J2 update from the committed state,
@@ -1194,36 +1250,18 @@ int orc_j2_newton_step(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
   int status = 4;
   int32_t it;
   double rn = 0.0;
-  int64_t local_fail = 0;
   for (it = 1; it <= max_newton; ++it) {
-    memset(Fi, 0, sizeof(double) * (size_t)n);
-    for (int64_t e = 0; e < n_elems; ++e) {
-      double xe[24], due[24], deps[48], sig[48], Fe[24];
-      gather_coords(conn, coords, e, xe);
-      for (int j = 0; j < 24; ++j) {
-        int64_t q = 3 * (int64_t)conn[8 * e + j / 3] + j % 3;
-        due[j] = u[q] - u_n[q];
-      }
-      orc_c3d8_strain(xe, due, deps);
-      for (int g = 0; g < 8; ++g) {
-        const double* sn = state_n + 13 * (8 * e + g);
-        double* sc = state + 13 * (8 * e + g);
-        memcpy(sc, sn, 13 * sizeof(double));
-        if (orc_j2_update(props, deps + 6 * g, sc, sc + 6, sc + 7, Dgp + 288 * e + 36 * g,
-                          NULL) != 0)
-          local_fail++;
-        memcpy(sig + 6 * g, sc, 6 * sizeof(double));
-      }
-      orc_c3d8_fint(xe, sig, Fe);
-      for (int j = 0; j < 24; ++j) Fi[3 * (int64_t)conn[8 * e + j / 3] + j % 3] += Fe[j];
+    if (orc_j2_internal_force(n_nodes, n_elems, conn, coords, props, u, u_n, state_n, state,
+                              Dgp, Fi, 1)) {
+      status = 2;
+      break;
     }
-    if (local_fail) { status = 2; break; }
     for (int32_t i = 0; i < n; ++i) R[i] = -Fi[i];
     for (int32_t b = 0; b < n_bc; ++b) R[bc_eq[b]] += penalty * (bc_val[b] - u[bc_eq[b]]);
     double fn = sqrt(dot(n, Fi, Fi, 1));
     rn = sqrt(dot(n, R, R, 1));
     if (it > 1 && rn <= tol_r * (fn > 0.0 ? fn : 1.0)) { status = 0; break; }
-    orc_assemble_dgp(n_elems, conn, coords, Dgp, n, row_ptr, col_idx, val);
+    orc_assemble_dgp(n_elems, conn, coords, Dgp, n, row_ptr, col_idx, val, 1);
     orc_apply_penalty(n_bc, bc_eq, NULL, penalty, row_ptr, col_idx, val, NULL);
     memset(du, 0, sizeof(double) * (size_t)n);
     int32_t ni;

@@ -60,7 +60,7 @@ int  orc_assemble(int32_t n_elems, const int32_t* conn, const double* coords,
                   const int32_t* col_idx, double* val, int nthreads);
 int  orc_assemble_dgp(int32_t n_elems, const int32_t* conn, const double* coords,
                       const double* Dgp, int32_t n_rows, const int32_t* row_ptr,
-                      const int32_t* col_idx, double* val);
+                      const int32_t* col_idx, double* val, int nthreads);
 int  orc_apply_penalty(int32_t n_bc, const int32_t* eq, const double* value, double penalty,
                        const int32_t* row_ptr, const int32_t* col_idx, double* val, double* F);
 void orc_face_loads_traction(int32_t n_faces, const int32_t* elem, const int32_t* face,
@@ -87,7 +87,8 @@ int  orc_cg_transpose(int32_t n, const int32_t* row_ptr, const int32_t* col_idx,
 
 /* ---- adjoint sensitivity ----------------------------------------------------- */
 void orc_sensitivity(int32_t n_elems, const int32_t* conn, const double* coords, double nu,
-                     const double* dEdrho, const double* u, const double* lam, double* g);
+                     const double* dEdrho, const double* u, const double* lam, double* g,
+                     int nthreads);
 
 /* ---- J2 plasticity -------------------------------------------------------- */
 int  orc_j2_update(const double* props, const double* deps, double* sigma, double* peeq,
@@ -97,6 +98,10 @@ int  orc_j2_update(const double* props, const double* deps, double* sigma, doubl
  * delta_voce; local_iters receives the local Newton iterations used */
 int  orc_j2_update_ex(const double* props, const double* deps, double* sigma, double* peeq,
                       double* eps_p, double* Dt, int32_t* yielded, int32_t* local_iters);
+int64_t orc_j2_internal_force(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
+                              const double* coords, const double* props, const double* u,
+                              const double* u_n, const double* state_n, double* state,
+                              double* Dgp, double* Fi, int nthreads);
 int  orc_j2_newton_step(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
                         const double* coords, const double* props,
                         int32_t n_bc, const int32_t* bc_eq, const double* bc_val, double penalty,

@@ -39,10 +39,21 @@ def test_status_codes_mirror_reference():
     assert C.sizeof(A.Status) == 4 * 4 + 512 + 64
 
 
+# ctypes mirror -> C struct, with the C names of fields whose Python names differ
+_STRUCTS = [("AsmInfo", "ufc_rt_asm_info", {}), ("J2Props", "ufc_ph_j2_props", {}),
+            ("Timing", "ufc_timing", {}), ("SolverAlgo", "ufc_nm_solver_algo", {}),
+            ("SolverState", "ufc_nm_solver_state", {}), ("KeArgs", "ufc_ph_ke_args", {}),
+            ("KeAlgo", "ufc_ph_ke_algo", {}), ("SensArgs", "ufc_ph_sens_args", {"lam": "lambda"}),
+            ("AsmAlgo", "ufc_rt_asm_algo", {}), ("AsmArgs", "ufc_rt_asm_args", {}),
+            ("BcArgs", "ufc_rt_bc_args", {}), ("LoadArgs", "ufc_rt_load_args", {}),
+            ("MeshDesc", "ufc_mesh_desc", {}), ("ApModel", "ufc_ap_model", {}),
+            ("RestartData", "ufc_rt_restart_data", {"lambda_": "lambda"})]
+
+
 def test_info_struct_layout_matches_header(tmp_path):
-    """ufc_rt_asm_info as the C compiler lays it out from include/ufc_hip.h
-    (size and every field offset) equals the ctypes mirror, and the FP32
-    view constants agree."""
+    """Every C-ABI struct as the C compiler lays it out from
+    include/ufc_hip.h (size and every field offset) equals its ctypes mirror,
+    and the FP32 view and J2 hardening constants agree."""
     import shutil
     import subprocess
     from ufc_hip import _abi as A
@@ -50,22 +61,34 @@ def test_info_struct_layout_matches_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
-    fields = [f[0] for f in A.AsmInfo._fields_]
+    body = ""
+    for py, cname, ren in _STRUCTS:
+        body += f'  printf("%zu\\n", sizeof({cname}));\n'
+        for f in getattr(A, py)._fields_:
+            body += f'  printf("%zu\\n", offsetof({cname}, {ren.get(f[0], f[0])}));\n'
     src = tmp_path / "layout.c"
     src.write_text(
         "#include <stdio.h>\n#include <stddef.h>\n#include \"ufc_hip.h\"\nint main(void) {\n"
-        '  printf("%zu\\n", sizeof(ufc_rt_asm_info));\n'
-        + "".join(f'  printf("%zu\\n", offsetof(ufc_rt_asm_info, {f}));\n' for f in fields)
+        + body
         + '  printf("%d %d %d %d\\n", UFC_FP32_VIEW_ID32, UFC_FP32_VIEW_OFF16, '
-          'UFC_FP32_VIEW_PATTERN, UFC_FP32_VIEW_SYMDIAG);\n  return 0;\n}\n')
+          'UFC_FP32_VIEW_PATTERN, UFC_FP32_VIEW_SYMDIAG);\n'
+        + '  printf("%d %d %d %d\\n", UFC_J2_HARD_LINEAR, UFC_J2_HARD_SWIFT, UFC_J2_HARD_VOCE, '
+          'UFC_J2_HARD_AF);\n  return 0;\n}\n')
     exe = tmp_path / "layout"
     subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
-    assert int(out[0]) == C.sizeof(A.AsmInfo)
-    for i, f in enumerate(fields):
-        assert int(out[1 + i]) == getattr(A.AsmInfo, f).offset, f
-    assert [int(v) for v in out[1 + len(fields)].split()] == [
+    k = 0
+    for py, cname, ren in _STRUCTS:
+        cls = getattr(A, py)
+        assert int(out[k]) == C.sizeof(cls), cname
+        k += 1
+        for f in cls._fields_:
+            assert int(out[k]) == getattr(cls, f[0]).offset, (cname, f[0])
+            k += 1
+    assert [int(v) for v in out[k].split()] == [
         A.FP32_VIEW_ID32, A.FP32_VIEW_OFF16, A.FP32_VIEW_PATTERN, A.FP32_VIEW_SYMDIAG]
+    assert [int(v) for v in out[k + 1].split()] == [
+        A.J2_HARD_LINEAR, A.J2_HARD_SWIFT, A.J2_HARD_VOCE, A.J2_HARD_AF]
 
 
 def test_generators_match_oracle(orc):

@@ -102,3 +102,27 @@ def test_load_errors_with_a_context(ctx, tmp_path):
     with pytest.raises(UfcError) as e:
         R.load(other, ctx=ctx)
     assert e.value.code == 1 and "holds 7 DOFs" in e.value.message
+
+
+def test_truncated_state_keeps_committed_state(ctx, tmp_path):
+    """A file that ends inside the J2 state records fails with
+    UFC_STATUS_ERROR and leaves the context's committed state unchanged
+    (the records are read into the trial copy first)."""
+    from ufc_hip import restart as R, UfcError
+    from ufc_hip.nonlinear import j2_static
+    pr, target, props = _j2(4)
+    ctx.build_pattern(pr.coords, pr.conn)
+    print("[gpu-test] j2 4^3 truncated state", flush=True)
+    res = j2_static(ctx, pr.bc_eq, target, props, n_increments=4, precision=0)
+    assert res.converged
+    path = str(tmp_path / "s.rst")
+    R.save(path, res.u, R.StepState(1.0, 4, 1.0, 0.0, 1, True), ctx=ctx, with_state=True)
+    data = open(path, "rb").read()
+    cut = str(tmp_path / "cut.rst")
+    open(cut, "wb").write(data[:len(data) - 4 - 8 * 13 * 8 * 5])  # inside the last record
+    known = np.random.default_rng(3).normal(size=pr.n_elems * 8 * 13)
+    ctx.j2_set_state(known)
+    with pytest.raises(UfcError) as e:
+        R.load(cut, ctx=ctx, with_state=True)
+    assert e.value.code == 2
+    assert np.array_equal(ctx.j2_get_state().ravel(), known)

@@ -436,6 +436,7 @@ int ufc_ctx_timing(ufc_ctx* c, ufc_timing* t, ufc_status* st) {
     t->spmv_ms_avg = c->spmv_ms_avg;
     t->spmv_calls = c->spmv_calls;
     t->n_replace = c->n_replace;
+    t->j2_update_ms = c->j2_update_ms;
     return UFC_STATUS_OK;
   });
 }
@@ -552,11 +553,16 @@ int ufc_ph_j2_update(ufc_ctx* c, const double* u, const double* u_n, int64_t* n_
     ufc::DBuf<double> tu, tn;
     const double* du = dev_in(c, u, size_t(c->n_dof), tu);
     const double* dn = dev_in(c, u_n, size_t(c->n_dof), tn);
+    UFC_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
     ufc::launch_j2_update(c, du, dn);
+    UFC_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
     std::vector<int32_t> fl(static_cast<size_t>(c->n_elems) * 8);
     UFC_HIP_CHECK(hipMemcpyAsync(fl.data(), c->j2_flags.get(), fl.size() * sizeof(int32_t),
                                  hipMemcpyDeviceToHost, c->stream));
     UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    UFC_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->j2_update_ms = ms;
     int64_t ny = 0, nfail = 0;
     for (int32_t f : fl) {
       ny += (f == 1);

@@ -298,7 +298,7 @@ struct ufc_ctx {
   std::vector<int64_t> spmv_pair_launch;  // per pair: index of the bracketed launch
   int64_t spmv_launches = 0;        // SpMV launches since the last reset
   bool spmv_sampled = false;        // the current launch is bracketed
-  double assemble_ms = 0.0, solve_ms = 0.0, spmv_ms_avg = 0.0;
+  double assemble_ms = 0.0, solve_ms = 0.0, spmv_ms_avg = 0.0, j2_update_ms = 0.0;
   int32_t spmv_calls = 0;
   int32_t n_replace = 0;  // FP64 residual replacements of the last MIXED solve
 };

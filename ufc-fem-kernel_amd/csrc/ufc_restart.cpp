@@ -272,19 +272,29 @@ int ufc_rt_restart_load(ufc_ctx* c, const char* path, ufc_rt_restart_data* d,
                   UFC_STATUS_INVALID,
                   "Restart restore: J2 state of " + std::to_string(sh[0]) +
                       " elements does not fit the mesh of " + std::to_string(ne));
+      // The records go to the trial copy first and replace the committed
+      // state only once every one of them has been read: a file that ends
+      // (or is damaged) inside the state leaves the committed state as it
+      // was, and the trial copy is reset to it.
       std::vector<double> stage(kStageDoubles);
-      for (int64_t e0 = 0; e0 < ne; e0 += kChunkElems) {
-        const int64_t vals = std::min(kChunkElems, ne - e0) * kPoints * kVals;
-        double* dst = c->j2_commit.get() + e0 * kPoints * kVals;
-        r.begin(vals * 8, "state");
-        for (int64_t q = 0; q < vals; q += int64_t(kStageDoubles)) {
-          const size_t m = size_t(std::min<int64_t>(int64_t(kStageDoubles), vals - q));
-          r.get(stage.data(), m * 8, "state");
-          UFC_HIP_CHECK(hipMemcpy(dst + q, stage.data(), m * 8, hipMemcpyHostToDevice));
+      try {
+        for (int64_t e0 = 0; e0 < ne; e0 += kChunkElems) {
+          const int64_t vals = std::min(kChunkElems, ne - e0) * kPoints * kVals;
+          double* dst = c->j2_trial.get() + e0 * kPoints * kVals;
+          r.begin(vals * 8, "state");
+          for (int64_t q = 0; q < vals; q += int64_t(kStageDoubles)) {
+            const size_t m = size_t(std::min<int64_t>(int64_t(kStageDoubles), vals - q));
+            r.get(stage.data(), m * 8, "state");
+            UFC_HIP_CHECK(hipMemcpy(dst + q, stage.data(), m * 8, hipMemcpyHostToDevice));
+          }
+          r.end("state");
         }
-        r.end("state");
+      } catch (...) {
+        (void)hipMemcpy(c->j2_trial.get(), c->j2_commit.get(), c->j2_commit.bytes(),
+                        hipMemcpyDeviceToDevice);
+        throw;
       }
-      UFC_HIP_CHECK(hipMemcpy(c->j2_trial.get(), c->j2_commit.get(), c->j2_commit.bytes(),
+      UFC_HIP_CHECK(hipMemcpy(c->j2_commit.get(), c->j2_trial.get(), c->j2_trial.bytes(),
                               hipMemcpyDeviceToDevice));
     }
     if (c)

@@ -139,6 +139,7 @@ MODULE ufc_hip_iface
     REAL(C_DOUBLE) :: spmv_ms_avg = 0.0_C_DOUBLE
     INTEGER(C_INT32_T) :: spmv_calls = 0
     INTEGER(C_INT32_T) :: n_replace = 0
+    REAL(C_DOUBLE) :: j2_update_ms = 0.0_C_DOUBLE
   END TYPE ufc_timing
 
   ! PH_Elem_Eval_Ke_Arg (PH_Elem_Def.f90:218-222) for a batch of C3D8

@@ -130,7 +130,7 @@ class SolverState(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("assemble_ms", C.c_double), ("solve_ms", C.c_double),
                 ("spmv_ms_avg", C.c_double), ("spmv_calls", C.c_int32),
-                ("n_replace", C.c_int32)]
+                ("n_replace", C.c_int32), ("j2_update_ms", C.c_double)]
 
 
 class ApModel(C.Structure):

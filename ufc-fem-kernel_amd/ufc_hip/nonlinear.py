@@ -49,6 +49,8 @@ class NewtonResult:
     u: np.ndarray = None
     cutbacks: int = 0          # failed attempts cut back by 0.25 over the job
     failures: list = field(default_factory=list)  # (load factor tried, reason) per cut
+    j2_update_ms: list = field(default_factory=list)  # device time of every J2 update
+    assemble_ms: list = field(default_factory=list)   # device time of every tangent assembly
 
 
 def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
@@ -124,6 +126,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             except A.UfcError as e:
                 reason = f"local Newton failure ({e.message})"
                 break  # cut back
+            res.j2_update_ms.append(ctx.timing().j2_update_ms)
             r, norms = ctx.residual(u_try)
             its = it
             rabs = norms[0]
@@ -138,6 +141,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
                 eta = lin_rtol
             r_prev = norms[0]
             ctx.assemble_k(None, nu, precision=precision, material=A.MAT_J2)
+            res.assemble_ms.append(ctx.timing().assemble_ms)
             du, info = ctx.pcg(precision=precision, rtol=eta, b=r)
             lin += info.niter
             if not info.converged:
