@@ -293,6 +293,53 @@ def cpu_j2_baseline(pj, props, u_end, n_newton, n_linear):
             "pcg_iter_ms": per_it * 1e3, "job_s_projected": job}
 
 
+def renumbered_run(args, torch, dev, ufc_hip, P, n, sols):
+    """The same cube with its node ids permuted at random (seeded): the
+    library recovers the box's lexicographic numbering for the symmetric
+    FP32 view; mixed and FP64 steps are timed as above (2 and 1 steps after
+    one warm-up each) and the solutions compared with the natural
+    numbering's after un-permuting."""
+    import numpy as np
+    pr = P.lognormal_cube(n)
+    perm = np.random.default_rng(P.SEED + 5).permutation(pr.coords.shape[0])
+    coords = np.empty_like(pr.coords)
+    coords[perm] = pr.coords
+    pr.coords = coords
+    pr.conn = perm[pr.conn].astype(np.int32)
+    pr.bc_eq = (3 * perm[pr.bc_eq // 3] + pr.bc_eq % 3).astype(np.int32)
+    log(f"renumbered {n}^3: setup")
+    t0 = time.perf_counter()
+    cr = ufc_hip.Context(0)
+    P.setup_context(cr, pr, ufc_hip.PREC_MIXED)
+    t1 = time.perf_counter()
+    pr.young_dev = torch.as_tensor(pr.young, device=dev)
+    x = torch.zeros(pr.n_dof, dtype=torch.float64, device=dev)
+    res = {"workload": (f"the {n}^3 cube with node ids permuted at random (seed "
+                        f"{P.SEED + 5}); the library finds the box numbering"),
+           "fp32_view": int(cr.fp32_view), "nb_index_bytes": int(cr.nb_index_bytes),
+           "setup_s": t1 - t0}
+    for name, prec, repl, k in (("mixed", ufc_hip.PREC_MIXED, args.replace_rtol, 2),
+                                ("fp64", ufc_hip.PREC_FP64, 0.0, 1)):
+        run_step(cr, pr, prec, 1e-10, torch, x, repl)
+        runs = []
+        for _ in range(k):
+            r, u = run_step(cr, pr, prec, 1e-10, torch, x, repl)
+            if not r["converged"] or not r["rnorm"] < 1e-10:
+                raise RuntimeError(f"renumbered {name} solve did not converge")
+            runs.append(r)
+        ms = float(np.mean([r["wall_ms"] for r in runs]))
+        u_nat = u.detach().cpu().numpy().reshape(-1, 3)[perm].ravel()
+        ref = sols.get(name)
+        res[name] = {"ms": ms, "iters": runs[-1]["niter"], "spmv_ms": runs[-1]["spmv_ms"]}
+        if ref is not None:
+            res[name]["vs_natural_numbering_rel"] = float(np.linalg.norm(u_nat - ref) /
+                                                          np.linalg.norm(ref))
+        log(f"renumbered {name}: {ms:.1f} ms, it {runs[-1]['niter']}")
+    res["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+    cr.close()
+    return res
+
+
 def copy_bandwidth(torch, dev, nbytes=1 << 31):
     """Device-to-device copy of nbytes (reads and writes nbytes): the
     attainable-bandwidth denominator beside the 8 TB/s peak."""
@@ -333,6 +380,8 @@ def main():
                     help="central-difference check of the adjoint gradient on N elements "
                          "(0: none)")
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton workload")
+    ap.add_argument("--no-renumbered", action="store_true",
+                    help="skip the randomly renumbered copy of the cube")
     ap.add_argument("--j2-size", type=int, default=64)
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 10 PCG iterations in each "
@@ -555,6 +604,9 @@ def main():
     out["assembly"] = asm
     ctx.close()
     del pr.young_dev
+
+    if not args.no_renumbered:
+        out["renumbered"] = renumbered_run(args, torch, dev, ufc_hip, P, n, sols)
 
     if not args.no_adjoint:
         na = args.adjoint_size

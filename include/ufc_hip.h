@@ -117,7 +117,10 @@ int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
  *   SYMDIAG: when the offsets m - n > 0 of all neighbour pairs take at most
  *            13 distinct values (a lexicographically numbered structured
  *            box), each off-diagonal block is stored once, for its upper
- *            node, and applied as K(n,m) and K(n,m)^T;
+ *            node, and applied as K(n,m) and K(n,m)^T; a structured box in
+ *            any other node numbering gets the view in its lexicographic
+ *            numbering (ufc_mesh_box_numbering), the FP32 inner solve then
+ *            running in that numbering;
  *   PATTERN: at most 256 distinct neighbour-offset lists, all offsets below
  *            2^15: one pattern byte per node;
  *   OFF16:   16-bit offset per neighbour;  ID32: node id per neighbour. */
@@ -464,6 +467,14 @@ int ufc_ctx_enable_timing(ufc_ctx* ctx, int32_t per_kernel, ufc_status* st);
 int ufc_mesh_box(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz,
                  double jitter, uint64_t seed, int32_t index_base, double* coords,
                  int32_t* conn, ufc_status* st);
+/* Lexicographic numbering of a mesh that is a structured box of C3D8 under
+ * any node numbering (host only, no GPU): perm[n] (0-based) = i + NX (j + NY
+ * k) for node n's lattice position, recovered from the element edges;
+ * UFC_STATUS_UNSUPPORTED when the mesh is not such a box.  The library uses
+ * it for the symmetric FP32 view of meshes whose own numbering is not
+ * lexicographic; a caller may use it to renumber its mesh. */
+int ufc_mesh_box_numbering(int32_t n_nodes, int32_t n_elems, int32_t index_base,
+                           const int32_t* conn, int32_t* perm, ufc_status* st);
 /* Per-element fields from a counter-based generator keyed by (seed, element):
  * kind 0: lognormal exp(a + b z); kind 1: uniform a + (b - a) u. */
 int ufc_field(int32_t kind, int64_t n, uint64_t seed, double a, double b, double* out,

@@ -354,9 +354,10 @@ def test_pattern_rejects_invalid_meshes(ctx):
 
 
 def test_fp32_view_wide_numbering_vs_oracle(ctx, orc):
-    """A 32^3 cube with randomly permuted node ids (bandwidth > 2^15): the FP32
-    view falls back to 4-byte neighbour ids; SpMV and the mixed solve match
-    the oracle as with 16-bit offsets."""
+    """A 32^3 cube with randomly permuted node ids (bandwidth > 2^15): by
+    default the symmetric view in the box's own lexicographic numbering;
+    forced to the views of the given numbering it takes 4-byte neighbour ids.
+    Both SpMVs and mixed solves match the oracle."""
     from ufc_hip import problems as P
     pr = P.cube_pressure(32, name="cube32perm")
     perm = np.random.default_rng(11).permutation(pr.coords.shape[0])
@@ -365,23 +366,26 @@ def test_fp32_view_wide_numbering_vs_oracle(ctx, orc):
     pr.conn = perm[pr.conn].astype(np.int32)
     pr.coords = coords
     pr.bc_eq = (3 * perm[pr.bc_eq // 3] + pr.bc_eq % 3).astype(np.int32)
+    from ufc_hip import _abi as A
     rp, ci, val, F, _ = _oracle_system(orc, pr)
-    P.setup_context(ctx, pr, 2)
-    assert ctx.nb_index_bytes == 4
-    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
     x = np.random.default_rng(2).normal(size=pr.n_dof)
     yo = orc.spmv(rp, ci, val, x)
-    y32 = ctx.spmv(x, precision=1)
-    assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
-    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
-    ctx.assemble_k(pr.young, pr.nu, precision=2)
-    say("wide numbering mixed solve")
-    u, info = ctx.pcg(precision=2, rtol=1e-10)
-    assert info.converged
-    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+    P.setup_context(ctx, pr, 2)
+    assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG and ctx.nb_index_bytes == 0
+    for view in (A.FP32_VIEW_SYMDIAG, A.FP32_VIEW_ID32):
+        assert ctx.set_fp32_view(view) == view
+        ctx.assemble_k(pr.young, pr.nu, precision=1)
+        y32 = ctx.spmv(x, precision=1)
+        assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max(), view
+        ctx.assemble_k(pr.young, pr.nu, precision=2)
+        say("wide numbering mixed solve, view", view)
+        u, info = ctx.pcg(precision=2, rtol=1e-10)
+        assert info.converged
+        assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8, view
+    assert ctx.nb_index_bytes == 4
     # the structured numbering takes the symmetric block-diagonal view (13
     # diagonals); the stencil-pattern view (27 patterns) is there on request
-    from ufc_hip import _abi as A
     P.setup_context(ctx, P.cube_pressure(8, name="cube8"), 2)
     assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG and ctx.nb_index_bytes == 0
     assert ctx.set_fp32_view(A.FP32_VIEW_PATTERN) == A.FP32_VIEW_PATTERN
@@ -402,10 +406,11 @@ def test_fp32_views_vs_oracle(ctx, orc):
     numbering (13 block diagonals, each off-diagonal block stored once; and,
     on request, one stencil-pattern byte per node, 27 patterns), a numbering
     shuffled inside windows of 64 ids and a random numbering (both more than
-    256 patterns, offsets below 2^15: 16-bit offsets; the 4-byte ids are
-    covered on a 32^3 cube above).  Each SpMV, and the mixed solve, matches
-    the oracle; the shuffled numberings give the structured solution back
-    after un-permuting."""
+    256 patterns, offsets below 2^15: 16-bit offsets on request; by default
+    the symmetric view in the box's lexicographic numbering; the 4-byte ids
+    are covered on a 32^3 cube above).  Each SpMV, and the mixed solve,
+    matches the oracle; the shuffled numberings give the structured solution
+    back after un-permuting."""
     from ufc_hip import _abi as A
     from ufc_hip import problems as P
     n = 16
@@ -413,9 +418,11 @@ def test_fp32_views_vs_oracle(ctx, orc):
     nn = base.coords.shape[0]
     rng = np.random.default_rng(5)
     window = np.concatenate([w + rng.permutation(min(64, nn - w)) for w in range(0, nn, 64)])
+    rand = rng.permutation(nn)
     cases = [("symdiag", np.arange(nn), 0, None),
              ("pattern", np.arange(nn), 1, A.FP32_VIEW_PATTERN),
-             ("window", window, 2, None), ("random", rng.permutation(nn), 2, None)]
+             ("window", window, 2, A.FP32_VIEW_OFF16), ("random", rand, 2, A.FP32_VIEW_OFF16),
+             ("window-box", window, 0, None), ("random-box", rand, 0, None)]
     ref = None
     for name, perm, width, force in cases:
         pr = _renumbered(P.cube_pressure(n, name="cube16"), perm)
@@ -448,8 +455,8 @@ def test_fp32_symdiag_view(ctx, orc):
     unrolls: the missing diagonals must contribute nothing); its SpMV and
     mixed solve match the oracle.  On a cube the view is switched after the
     FP32 assembly: the stencil-pattern view's K x agrees to FP32 rounding,
-    and switching back gives the symmetric view's K x bit for bit.  A
-    numbering without the diagonal structure rejects the view
+    and switching back gives the symmetric view's K x bit for bit.  A mesh
+    that is no box (notched, shuffled) rejects the view
     (UFC_STATUS_UNSUPPORTED), an unknown view id is invalid, and the
     rejections leave the view unchanged."""
     from ufc_hip import _abi as A
@@ -491,14 +498,14 @@ def test_fp32_symdiag_view(ctx, orc):
     assert e.value.code == A.STATUS_INVALID
     assert ctx.info().fp32_view == A.FP32_VIEW_SYMDIAG
 
-    perm = np.random.default_rng(6).permutation(pr.coords.shape[0])
-    pr = _renumbered(P.cube_pressure(10, name="cube10"), perm)
+    pr = _irregular()  # notched, shuffled: no box numbering exists
     P.setup_context(ctx, pr, 2)
-    assert ctx.fp32_view == A.FP32_VIEW_OFF16
+    before = ctx.fp32_view
+    assert before in (A.FP32_VIEW_OFF16, A.FP32_VIEW_PATTERN)
     with pytest.raises(A.UfcError) as e:
         ctx.set_fp32_view(A.FP32_VIEW_SYMDIAG)
     assert e.value.code == A.STATUS_UNSUPPORTED
-    assert ctx.info().fp32_view == A.FP32_VIEW_OFF16
+    assert ctx.info().fp32_view == before
 
 
 def test_fp32_symdiag_view_tile_blocks(ctx, orc):
@@ -549,3 +556,71 @@ def test_pcg_progress_record_matches_polling(ctx, precision):
         for ce in (0, 3):
             _, im = ctx.pcg(precision=0, rtol=1e-14, max_iter=11, check_every=ce)
             assert im.status == 1 and im.niter == 11
+
+
+def test_fp32_symdiag_box_numbering_vs_oracle(ctx, orc):
+    """A 12^3 cube with random node ids, shuffled element order and rotated
+    element node order: the library finds the box's lexicographic numbering
+    and runs the symmetric view in it.  FP32 SpMV, FP32 PCG and both mixed
+    forms match the oracle; a boundary-condition change after assembly
+    (penalties re-applied in the box numbering) solves again to the oracle;
+    switching to the 16-bit-offset view of the given numbering and back
+    gives K x bit for bit."""
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(12, name="cube12box")
+    nn = pr.coords.shape[0]
+    rng = np.random.default_rng(21)
+    perm = rng.permutation(nn)
+    pr = _renumbered(pr, perm)
+    eorder = rng.permutation(pr.n_elems)
+    rot = np.array([[0, 1, 2, 3], [1, 2, 3, 0], [2, 3, 0, 1], [3, 0, 1, 2]])
+    r = rng.integers(0, 4, pr.n_elems)
+    conn = pr.conn[eorder]
+    pr.conn = np.concatenate([np.take_along_axis(conn[:, :4], rot[r], 1),
+                              np.take_along_axis(conn[:, 4:], rot[r], 1)], 1).astype(np.int32)
+    pr.young = pr.young[eorder]
+    # the loaded faces follow the elements (face 1, zeta = +1, of the
+    # rotated element is the same face)
+    inv = np.empty_like(eorder)
+    inv[eorder] = np.arange(eorder.size)
+    pr.load_elem = inv[pr.load_elem].astype(np.int32)
+    assert _n_diagonals(pr.conn) > 13
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=200000)
+    P.setup_context(ctx, pr, 2)
+    assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG and ctx.nb_index_bytes == 0
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    x = np.random.default_rng(4).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    y32 = ctx.spmv(x, precision=1)
+    say("box-numbered symdiag spmv", np.abs(y32 - yo).max() / np.abs(yo).max())
+    assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
+    u1, i1 = ctx.pcg(precision=1, rtol=1e-5)
+    assert i1.converged
+    assert np.linalg.norm(u1 - uo) / np.linalg.norm(uo) < 1e-3
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    for repl in (1.0, 0.1):
+        u, info = ctx.pcg(precision=2, rtol=1e-10, replace_rtol=repl)
+        err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
+        say("box-numbered mixed", repl, info.niter, err)
+        assert info.converged and err < 1e-8
+    # penalties re-applied after assembly: clamp z = 0 and also x = 0
+    x0 = np.nonzero(pr.coords[:, 0] == 0.0)[0]
+    eq2 = np.unique(np.concatenate([pr.bc_eq, (3 * x0[:, None] + np.arange(3)).ravel()]))
+    ctx.apply_bc(eq2.astype(np.int32), None, 1e12)
+    rp2, ci2 = rp, ci
+    val2 = orc.assemble(pr.conn, pr.coords, pr.young, pr.nu, rp2, ci2)
+    F2 = np.zeros(pr.n_dof)
+    orc.face_loads_pressure(pr.load_elem, pr.load_face, pr.conn, pr.coords, pr.load_value[0], F2)
+    orc.apply_penalty(eq2.astype(np.int32), np.zeros(eq2.size), 1e12, rp2, ci2, val2, F2)
+    uo2, _, _, _ = orc.pcg(rp2, ci2, val2, F2, rtol=1e-13, max_iter=200000)
+    u2, info2 = ctx.pcg(precision=2, rtol=1e-10, replace_rtol=0.1)
+    assert info2.converged and np.linalg.norm(u2 - uo2) / np.linalg.norm(uo2) < 1e-8
+    y_box = ctx.spmv(x, precision=1)
+    assert ctx.set_fp32_view(A.FP32_VIEW_OFF16) == A.FP32_VIEW_OFF16
+    y16 = ctx.spmv(x, precision=1)
+    yo2 = orc.spmv(rp2, ci2, val2, x)
+    assert np.abs(y16 - yo2).max() <= 1e-5 * np.abs(yo2).max()
+    assert ctx.set_fp32_view(-1) == A.FP32_VIEW_SYMDIAG
+    assert np.array_equal(ctx.spmv(x, precision=1), y_box)

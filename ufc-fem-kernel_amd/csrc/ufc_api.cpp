@@ -181,15 +181,205 @@ bool fp32_view_available(const ufc_ctx* c, int view) {
   }
 }
 
+// Lexicographic numbering of a mesh that is a structured box of hexes under
+// some node numbering: perm[n] = i + NX (j + NY k) for the lattice position
+// (i, j, k) of node n, or an empty vector when the mesh is not such a box.
+// The lattice is recovered from the element edges (the 12 edges of the C3D8
+// node order): a corner node (one element, three edge neighbours) is the
+// origin and its edge neighbours the three axes; a lattice line continues
+// from (prev, cur) to the edge neighbour of cur that shares no element with
+// prev; every other position is the one common edge neighbour of two
+// labelled positions besides the one they already share.  The result is
+// accepted only if it labels every node once and every element is a unit
+// cell of the lattice.  node_elem: (n_nodes, 8) element ids, -1 padded.
+std::vector<int32_t> box_numbering(int32_t nn, const std::vector<int32_t>& conn,
+                                   const std::vector<int32_t>& node_elem) {
+  static const int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                   {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  const int64_t ne = int64_t(conn.size()) / 8;
+  std::vector<std::array<int32_t, 6>> gn(static_cast<size_t>(nn));
+  std::vector<uint8_t> gdeg(static_cast<size_t>(nn), 0);
+  for (int64_t e = 0; e < ne; ++e)
+    for (const auto& ed : kEdge) {
+      const int32_t a = conn[8 * e + ed[0]], b = conn[8 * e + ed[1]];
+      for (int t = 0; t < 2; ++t) {
+        const int32_t x = t ? b : a, y = t ? a : b;
+        auto& L = gn[size_t(x)];
+        const int d = gdeg[size_t(x)];
+        if (std::find(L.begin(), L.begin() + d, y) != L.begin() + d) continue;
+        if (d == 6) return {};  // more than 6 edge neighbours: no lattice
+        L[size_t(d)] = y;
+        gdeg[size_t(x)] = uint8_t(d + 1);
+      }
+    }
+  auto elems_of = [&](int32_t n) { return node_elem.data() + 8 * size_t(n); };
+  auto share_elem = [&](int32_t a, int32_t b) {
+    const int32_t* E = elems_of(a);
+    for (int s = 0; s < 8 && E[s] >= 0; ++s)
+      for (int q = 0; q < 8; ++q)
+        if (conn[8 * size_t(E[s]) + q] == b) return true;
+    return false;
+  };
+  auto n_elems_of = [&](int32_t n) {
+    int k = 0;
+    while (k < 8 && elems_of(n)[k] >= 0) ++k;
+    return k;
+  };
+  int32_t corner = -1;
+  for (int32_t n = 0; n < nn && corner < 0; ++n)
+    if (n_elems_of(n) == 1 && gdeg[size_t(n)] == 3) corner = n;
+  if (corner < 0) return {};
+  // the three axis lines from the corner
+  std::vector<int32_t> axis[3];
+  for (int a = 0; a < 3; ++a) {
+    std::vector<int32_t>& line = axis[a];
+    line.push_back(corner);
+    line.push_back(gn[size_t(corner)][size_t(a)]);
+    while (true) {
+      const int32_t prev = line[line.size() - 2], cur = line.back();
+      int32_t next = -1, cnt = 0;
+      for (int q = 0; q < gdeg[size_t(cur)]; ++q) {
+        const int32_t cand = gn[size_t(cur)][size_t(q)];
+        if (cand != prev && !share_elem(prev, cand)) {
+          next = cand;
+          ++cnt;
+        }
+      }
+      if (cnt == 0) break;
+      if (cnt > 1 || int64_t(line.size()) > nn) return {};
+      line.push_back(next);
+    }
+  }
+  const int64_t NX = int64_t(axis[0].size()), NY = int64_t(axis[1].size()),
+                NZ = int64_t(axis[2].size());
+  if (NX * NY * NZ != nn || (NX - 1) * (NY - 1) * (NZ - 1) != ne) return {};
+  std::vector<int32_t> lab(static_cast<size_t>(nn), -1);
+  auto at = [&](int64_t i, int64_t j, int64_t k) -> int32_t& {
+    return lab[size_t(i + NX * (j + NY * k))];
+  };
+  for (int64_t i = 0; i < NX; ++i) at(i, 0, 0) = axis[0][size_t(i)];
+  for (int64_t j = 0; j < NY; ++j) at(0, j, 0) = axis[1][size_t(j)];
+  for (int64_t k = 0; k < NZ; ++k) at(0, 0, k) = axis[2][size_t(k)];
+  // the one edge neighbour common to a and b other than x
+  auto common = [&](int32_t a, int32_t b, int32_t x) -> int32_t {
+    int32_t r = -1;
+    for (int p = 0; p < gdeg[size_t(a)]; ++p) {
+      const int32_t cand = gn[size_t(a)][size_t(p)];
+      if (cand == x) continue;
+      for (int q = 0; q < gdeg[size_t(b)]; ++q)
+        if (gn[size_t(b)][size_t(q)] == cand) {
+          if (r >= 0 && r != cand) return -2;
+          r = cand;
+        }
+    }
+    return r;
+  };
+  for (int64_t k = 0; k < NZ; ++k)
+    for (int64_t j = 0; j < NY; ++j)
+      for (int64_t i = 0; i < NX; ++i) {
+        if (at(i, j, k) >= 0) continue;
+        int32_t v;
+        if (k == 0)
+          v = common(at(i - 1, j, 0), at(i, j - 1, 0), at(i - 1, j - 1, 0));
+        else if (i > 0)
+          v = common(at(i - 1, j, k), at(i, j, k - 1), at(i - 1, j, k - 1));
+        else
+          v = common(at(0, j - 1, k), at(0, j, k - 1), at(0, j - 1, k - 1));
+        if (v < 0) return {};
+        at(i, j, k) = v;
+      }
+  std::vector<int32_t> perm(static_cast<size_t>(nn), -1);
+  for (int64_t p = 0; p < nn; ++p) {
+    const int32_t n = lab[size_t(p)];
+    if (perm[size_t(n)] >= 0) return {};  // a node labelled twice
+    perm[size_t(n)] = int32_t(p);
+  }
+  // every element a unit cell of the lattice
+  for (int64_t e = 0; e < ne; ++e) {
+    int64_t lo[3] = {NX, NY, NZ};
+    int64_t pos[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const int64_t p = perm[size_t(conn[8 * e + a])];
+      pos[a][0] = p % NX;
+      pos[a][1] = (p / NX) % NY;
+      pos[a][2] = p / (NX * NY);
+      for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], pos[a][d]);
+    }
+    int seen = 0;
+    for (int a = 0; a < 8; ++a) {
+      int bit = 0;
+      for (int d = 0; d < 3; ++d) {
+        const int64_t r = pos[a][d] - lo[d];
+        if (r > 1) return {};
+        bit |= int(r) << d;
+      }
+      seen |= 1 << bit;
+    }
+    if (seen != 0xff) return {};
+  }
+  return perm;
+}
+
 // Offsets of the symmetric block-diagonal view: the distinct m - n > 0 over
-// all neighbour pairs (n, m).  More than kMaxDiags of them: no such view.
+// all neighbour pairs (n, m), in the node numbering or, when that has too
+// many and the mesh is a structured box, in the box's lexicographic
+// numbering (c->sym_permuted).  More than kMaxDiags of them: no such view.
 // Chunks of nodes collect their offsets in parallel and stop early once a
 // chunk alone has too many.
-void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx) {
+bool collect_sym_offsets(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
+                         const std::vector<int32_t>* perm, std::vector<int32_t>& all);
+
+void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
+                         const std::vector<int32_t>& node_elem) {
   c->n_diag = 0;
+  c->sym_permuted = false;
+  c->h_sym_perm.clear();
+  c->sym_perm.release();
+  c->sym_iperm.release();
+  c->diag32p.release();
   c->diag_a.release();
   c->diag_b.release();
   c->diag_c.release();
+  const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
+  std::vector<int32_t> all;
+  if (!collect_sym_offsets(c, nb_idx, nullptr, all)) {
+    std::vector<int32_t> perm = box_numbering(int32_t(nn), c->h_conn, node_elem);
+    if (perm.empty() || !collect_sym_offsets(c, nb_idx, &perm, all)) return;
+    c->sym_permuted = true;
+    c->h_sym_perm = std::move(perm);
+  }
+  if (all.empty()) return;
+  std::sort(all.begin(), all.end());
+  // tile-major entries with ceil(o_last / kDiagTile) zero tiles in front; the
+  // SpMV indexes them with 32-bit integers
+  const int64_t pad_tiles = (int64_t(all.back()) + ufc::kDiagTile - 1) / ufc::kDiagTile;
+  const int64_t tiles = (nn + ufc::kDiagTile - 1) / ufc::kDiagTile + pad_tiles;
+  const int64_t entries = tiles * int64_t(all.size()) * ufc::kDiagTile;
+  if (entries >= (int64_t(1) << 31)) {
+    c->sym_permuted = false;
+    c->h_sym_perm.clear();
+    return;
+  }
+  c->n_diag = int32_t(all.size());
+  for (int s = 0; s < ufc::kMaxDiags; ++s) c->h_diag_off[s] = s < c->n_diag ? all[size_t(s)] : 0;
+  c->diag_pad_tiles = int32_t(pad_tiles);
+  c->diag_entries = entries;
+  if (c->sym_permuted) {
+    std::vector<int32_t> iperm(static_cast<size_t>(nn));
+    for (int64_t n = 0; n < nn; ++n) iperm[size_t(c->h_sym_perm[size_t(n)])] = int32_t(n);
+    c->sym_perm.resize(size_t(nn));
+    c->sym_iperm.resize(size_t(nn));
+    UFC_HIP_CHECK(hipMemcpyAsync(c->sym_perm.get(), c->h_sym_perm.data(), c->sym_perm.bytes(),
+                                 hipMemcpyHostToDevice, c->stream));
+    UFC_HIP_CHECK(hipMemcpyAsync(c->sym_iperm.get(), iperm.data(), c->sym_iperm.bytes(),
+                                 hipMemcpyHostToDevice, c->stream));
+    UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+}
+
+bool collect_sym_offsets(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
+                         const std::vector<int32_t>* perm, std::vector<int32_t>& all) {
+  all.clear();
   const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
   constexpr int64_t kChunk = 32;  // enough chunks for parallel_for's threads on large meshes
   const int64_t nchunk = (nn + kChunk - 1) / kChunk;
@@ -201,30 +391,19 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx) {
     int& k = count[size_t(ch)];
     for (int64_t n = ch * kChunk; n < std::min(nn, (ch + 1) * kChunk) && k < kCap; ++n)
       for (int32_t q = c->h_nb_ptr[size_t(n)]; q < c->h_nb_ptr[size_t(n) + 1] && k < kCap; ++q) {
-        const int64_t o = int64_t(nb_idx[size_t(q)]) - n;
+        const int64_t m = nb_idx[size_t(q)];
+        const int64_t o = perm ? int64_t((*perm)[size_t(m)]) - (*perm)[size_t(n)] : m - n;
         if (o <= 0) continue;
         if (std::find(f.begin(), f.begin() + k, int32_t(o)) == f.begin() + k) f[size_t(k++)] = int32_t(o);
       }
   });
-  std::vector<int32_t> all;
   for (int64_t ch = 0; ch < nchunk; ++ch) {
     for (int j = 0; j < count[size_t(ch)]; ++j)
       if (std::find(all.begin(), all.end(), found[size_t(ch)][size_t(j)]) == all.end())
         all.push_back(found[size_t(ch)][size_t(j)]);
-    if (int(all.size()) > ufc::kMaxDiags) return;
+    if (int(all.size()) > ufc::kMaxDiags) return false;
   }
-  if (all.empty()) return;
-  std::sort(all.begin(), all.end());
-  // tile-major entries with ceil(o_last / kDiagTile) zero tiles in front; the
-  // SpMV indexes them with 32-bit integers
-  const int64_t pad_tiles = (int64_t(all.back()) + ufc::kDiagTile - 1) / ufc::kDiagTile;
-  const int64_t tiles = (nn + ufc::kDiagTile - 1) / ufc::kDiagTile + pad_tiles;
-  const int64_t entries = tiles * int64_t(all.size()) * ufc::kDiagTile;
-  if (entries >= (int64_t(1) << 31)) return;
-  c->n_diag = int32_t(all.size());
-  for (int s = 0; s < ufc::kMaxDiags; ++s) c->h_diag_off[s] = s < c->n_diag ? all[size_t(s)] : 0;
-  c->diag_pad_tiles = int32_t(pad_tiles);
-  c->diag_entries = entries;
+  return true;
 }
 
 // Stencil-pattern dictionary of the 16-bit view (off16, CSR pair order):
@@ -775,7 +954,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       c->nb_off16.release();
     }
     build_stencil_patterns(c, off16);
-    build_sym_diagonals(c, nb_idx);
+    build_sym_diagonals(c, nb_idx, s_elem);
     c->fp32_view_request = -1;
     ufc::select_fp32_view(c);
     up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
@@ -1088,6 +1267,38 @@ int ufc_rt_residual(ufc_ctx* c, const double* u, double* r, double* norms, ufc_s
   });
 }
 
+int ufc_mesh_box_numbering(int32_t n_nodes, int32_t n_elems, int32_t index_base,
+                           const int32_t* conn, int32_t* perm, ufc_status* st) {
+  return guarded(st, "ufc_mesh_box_numbering", [&](std::string& msg) {
+    UFC_REQUIRE(conn && perm, UFC_STATUS_INVALID, "null argument");
+    UFC_REQUIRE(n_nodes > 0 && n_elems > 0, UFC_STATUS_INVALID, "empty mesh");
+    std::vector<int32_t> cv(static_cast<size_t>(n_elems) * 8);
+    std::vector<int32_t> cnt(static_cast<size_t>(n_nodes), 0);
+    for (size_t i = 0; i < cv.size(); ++i) {
+      const int32_t v = conn[i] - index_base;
+      UFC_REQUIRE(v >= 0 && v < n_nodes, UFC_STATUS_INVALID, "connectivity entry out of range");
+      cv[i] = v;
+      UFC_REQUIRE(++cnt[size_t(v)] <= ufc::kMaxNodeElems, UFC_STATUS_UNSUPPORTED,
+                  "not a structured box (a node in more than 8 elements)");
+    }
+    std::vector<int32_t> node_elem(static_cast<size_t>(n_nodes) * 8, -1);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int32_t e = 0; e < n_elems; ++e)
+      for (int a = 0; a < 8; ++a) {
+        const int32_t n = cv[8 * size_t(e) + a];
+        node_elem[8 * size_t(n) + size_t(cnt[size_t(n)]++)] = e;
+      }
+    const std::vector<int32_t> p = box_numbering(n_nodes, cv, node_elem);
+    if (p.empty()) {
+      msg = "not a structured box of hexes";
+      return UFC_STATUS_UNSUPPORTED;
+    }
+    std::copy(p.begin(), p.end(), perm);
+    msg = "box numbering found";
+    return UFC_STATUS_OK;
+  });
+}
+
 // ======================================================================= L2_NM
 int ufc_nm_spmv(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_status* st) {
   return guarded(st, "ufc_nm_spmv", [&](std::string&) {
@@ -1104,9 +1315,9 @@ int ufc_nm_spmv(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_s
       copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
     } else {
       UFC_REQUIRE(c->have32, UFC_STATUS_INVALID, "no FP32 values assembled");
-      ufc::cast_f64_to_f32(c, dx, c->s_x.get());
+      ufc::to_view32(c, dx, c->s_x.get());
       ufc::spmv32(c, c->s_x.get(), c->s_ap.get());
-      ufc::cast_f32_to_f64(c, c->s_ap.get(), c->w_t.get());
+      ufc::from_view32(c, c->s_ap.get(), c->w_t.get());
       copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
     }
     ufc::reset_spmv_events(c);

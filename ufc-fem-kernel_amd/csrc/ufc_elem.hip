@@ -625,22 +625,27 @@ __global__ __launch_bounds__(256) void k_offdiag_compact(int32_t n_nodes,
 // as float4 (0,0) (0,1) (0,2) (1,0), float4 (1,1) (1,2) (2,0) (2,1) and
 // float (2,2).  The pairs of the pattern are fixed, so the entries never
 // written stay zero from the allocation.
+// perm != null: the entries are in the box numbering (node n's rows go to
+// entry (s, perm[n]) for the neighbours m with perm[m] - perm[n] = o_s > 0).
 __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
                                                    const int32_t* __restrict__ nb_ptr,
                                                    const int32_t* __restrict__ nb_idx,
                                                    const float* __restrict__ val, DiagOffsets off,
                                                    int nd, int32_t pad_tiles,
+                                                   const int32_t* __restrict__ perm,
                                                    float4* __restrict__ da,
                                                    float4* __restrict__ db,
                                                    float* __restrict__ dc) {
   const int k = threadIdx.x & 31;
   const int64_t ng = int64_t(gridDim.x) * (blockDim.x / 32);
-  for (int64_t n = int64_t(blockIdx.x) * (blockDim.x / 32) + threadIdx.x / 32; n < n_nodes;
-       n += ng) {
-    const int64_t q0 = nb_ptr[n];
-    const int deg = int(nb_ptr[n + 1] - q0);
+  for (int64_t n0 = int64_t(blockIdx.x) * (blockDim.x / 32) + threadIdx.x / 32; n0 < n_nodes;
+       n0 += ng) {
+    const int64_t q0 = nb_ptr[n0];
+    const int deg = int(nb_ptr[n0 + 1] - q0);
     if (k >= deg) continue;
-    const int64_t o = int64_t(nb_idx[q0 + k]) - n;
+    const int64_t m0 = nb_idx[q0 + k];
+    const int64_t n = perm ? perm[n0] : n0;
+    const int64_t o = (perm ? perm[m0] : m0) - n;
     if (o <= 0) continue;
     int s = -1;
 #pragma unroll
@@ -663,18 +668,21 @@ __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
 // pen32 = the self-block diagonal of the difference-form values (the
 // penalty), pen_flag bit j of word w = node 32 w + j has a nonzero one: one
 // thread per node, the bits of a wave's 64 nodes formed by a ballot (waves
-// start at multiples of 64 nodes, so they fill two whole words)
+// start at multiples of 64 nodes, so they fill two whole words).  iperm !=
+// null: thread n is box number n and reads node iperm[n].
 __global__ __launch_bounds__(256) void k_offdiag_penalty(int32_t n_nodes,
                                                          const int32_t* __restrict__ diag_pos,
                                                          const float* __restrict__ val,
+                                                         const int32_t* __restrict__ iperm,
                                                          float* __restrict__ pen,
                                                          uint32_t* __restrict__ flag) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   bool any = false;
   if (n < n_nodes) {
+    const int64_t src = iperm ? iperm[n] : n;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float p = val[diag_pos[3 * n + d]];
+      const float p = val[diag_pos[3 * src + d]];
       pen[3 * n + d] = p;
       any = any || p != 0.f;
     }
@@ -684,6 +692,16 @@ __global__ __launch_bounds__(256) void k_offdiag_penalty(int32_t n_nodes,
   const int64_t w = (n - lane) / 32;  // first word of this wave
   if (lane == 0 && n < n_nodes) flag[w] = uint32_t(bits);
   if (lane == 32 && n < n_nodes) flag[w + 1] = uint32_t(bits >> 32);
+}
+
+// o[3 i + d] = a[3 iperm[i] + d]: a node-ordered FP32 vector in the box numbering
+__global__ void k_permute_f32(int32_t n_nodes, const int32_t* __restrict__ iperm,
+                              const float* __restrict__ a, float* __restrict__ o) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const int64_t n = iperm[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) o[3 * i + d] = a[3 * n + d];
 }
 
 // diag[eq[i]] += pen[i]
@@ -1263,14 +1281,23 @@ void launch_fp32_offdiag(ufc_ctx* c) {
     }
     DiagOffsets off;
     for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
+    const bool perm = fp32_permuted(c);
     hipLaunchKernelGGL(k_diag_fill, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), off, int(c->n_diag),
-                       c->diag_pad_tiles,
+                       c->diag_pad_tiles, perm ? c->sym_perm.get() : nullptr,
                        reinterpret_cast<float4*>(c->diag_a.get()),
                        reinterpret_cast<float4*>(c->diag_b.get()), c->diag_c.get());
+    if (perm) {
+      // the Jacobi diagonal of the FP32 PCG in the box numbering
+      c->diag32p.resize(size_t(c->n_dof));
+      hipLaunchKernelGGL(k_permute_f32, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0,
+                         c->stream, c->n_nodes, c->sym_iperm.get(), c->diag32.get(),
+                         c->diag32p.get());
+    }
   }
   hipLaunchKernelGGL(k_offdiag_penalty, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
-                     c->n_nodes, c->diag_pos.get(), c->val32.get(), c->pen32.get(),
+                     c->n_nodes, c->diag_pos.get(), c->val32.get(),
+                     fp32_permuted(c) ? c->sym_iperm.get() : nullptr, c->pen32.get(),
                      c->pen_flag.get());
   UFC_HIP_CHECK(hipGetLastError());
 }

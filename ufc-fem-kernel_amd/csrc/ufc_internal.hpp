@@ -208,6 +208,19 @@ struct ufc_ctx {
   ufc::DBuf<float> diag_a;      // (diag_entries, 4)
   ufc::DBuf<float> diag_b;      // (diag_entries, 4)
   ufc::DBuf<float> diag_c;      // (diag_entries)
+  // Box numbering of the symmetric view: when the mesh's own numbering has
+  // more than kMaxDiags offsets but the mesh is a structured box of hexes
+  // (any node numbering), the view runs in the box's lexicographic numbering
+  // instead.  sym_perm[n] = that number of node n, sym_iperm its inverse; the
+  // entries, pen32, pen_flag, diag32p and the FP32 PCG's vectors are then in
+  // the box numbering and converted at the solve's boundary.
+  bool sym_permuted = false;
+  std::vector<int32_t> h_sym_perm;
+  ufc::DBuf<int32_t> sym_perm;   // (n_nodes)
+  ufc::DBuf<int32_t> sym_iperm;  // (n_nodes)
+  ufc::DBuf<float> diag32p;      // diag32 in the box numbering
+  ufc::DBuf<double> w_perm;      // FP64 right-hand side in the box numbering
+  ufc::DBuf<double> w_perm2;     // FP64 iterate in the box numbering (restarted form)
   int32_t fp32_view = -1;       // view of the FP32 SpMV (kFp32View*); -1 before the pattern
   int32_t fp32_view_request = -1;  // -1: the fastest available
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)
@@ -305,6 +318,11 @@ struct ufc_ctx {
 
 namespace ufc {
 
+// the FP32 view runs in the box numbering (sym_perm)
+inline bool fp32_permuted(const ufc_ctx* c) {
+  return c->sym_permuted && c->fp32_view == kFp32ViewSymDiag;
+}
+
 // ---------------------------------------------------------------- launchers
 // device address of the allocation-free entry point's scratch array on the
 // current device, resolved once per device (ufc_noalloc.hip)
@@ -354,6 +372,10 @@ int pcg32_segment(ufc_ctx* c, const double* b64, float* d, bool resume, int max_
 void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u);
 void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o);
 void cast_f32_to_f64(ufc_ctx* c, const float* a, double* o);
+// FP64 vector in node order <-> FP32 vector in the FP32 view's numbering
+// (the same numbering unless fp32_permuted)
+void to_view32(ufc_ctx* c, const double* a, float* o);
+void from_view32(ufc_ctx* c, const float* a, double* o);
 void build_col_idx(ufc_ctx* c);
 double spmv_event_avg_ms(ufc_ctx* c, int* calls);
 void reset_spmv_events(ufc_ctx* c);

@@ -1025,6 +1025,38 @@ __global__ void k_cast_f64(int64_t n, const float* __restrict__ a, double* __res
   if (i < n) o[i] = double(a[i]);
 }
 
+// Conversions between node order and the box numbering of the symmetric
+// FP32 view (perm[n] = box number of node n, iperm its inverse); one thread
+// per box number i, which writes its 3 contiguous entries and reads node
+// iperm[i]'s 3 contiguous ones.
+template <typename A, typename B>
+__global__ void k_to_box(int32_t n_nodes, const int32_t* __restrict__ iperm,
+                         const A* __restrict__ a, B* __restrict__ o) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const int64_t n = iperm[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) o[3 * i + d] = B(a[3 * n + d]);
+}
+template <typename A, typename B>
+__global__ void k_from_box(int32_t n_nodes, const int32_t* __restrict__ iperm,
+                           const A* __restrict__ a, B* __restrict__ o) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const int64_t n = iperm[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) o[3 * n + d] = B(a[3 * i + d]);
+}
+// u (node order) += d (box numbering)
+__global__ void k_axpy_f32_box(int32_t n_nodes, const int32_t* __restrict__ iperm,
+                               const float* __restrict__ d, double* __restrict__ u) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const int64_t n = iperm[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u[3 * n + k] += double(d[3 * i + k]);
+}
+
 // col_idx of node n's three rows from its neighbour list (one wave per node)
 __global__ __launch_bounds__(kSolveThreads) void k_expand_cols(int32_t n_nodes,
                                                               const int32_t* __restrict__ nb_ptr,
@@ -1324,22 +1356,68 @@ int pcg64(ufc_ctx* c, const double* b, double* x, bool x0_zero, int max_iter, do
                                 niter, rnorm);
 }
 
+// In the box numbering (fp32_permuted) the right-hand side is carried over
+// into w_perm and the Jacobi diagonal is diag32p; the FP32 vectors never leave
+// that numbering between segments, so a resumed CG keeps its direction.
+static const double* rhs_view(ufc_ctx* c, const double* b64) {
+  if (!fp32_permuted(c)) return b64;
+  c->w_perm.resize(size_t(c->n_dof));
+  hipLaunchKernelGGL((k_to_box<double, double>), dim3(blocks_for(c->n_nodes, 256)), dim3(256),
+                     0, c->stream, c->n_nodes, c->sym_iperm.get(), b64, c->w_perm.get());
+  UFC_HIP_CHECK(hipGetLastError());
+  return c->w_perm.get();
+}
+
+static const float* diag_view(ufc_ctx* c) {
+  return fp32_permuted(c) ? c->diag32p.get() : c->diag32.get();
+}
+
 int pcg32(ufc_ctx* c, const double* b64, double* x, bool zero_x, bool resume, int max_iter,
           double rtol, int check_every, int* niter, double* rnorm) {
   // the iteration starts from 0; without zero_x the correction accumulates
   // into the caller's x (the refinement's u += d)
   if (zero_x) UFC_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * size_t(c->n_dof), c->stream));
-  return pcg_run<float, true, double>(c, b64, x, nullptr, c->s_r.get(), c->s_p.get(),
-                                      c->s_ap.get(), c->diag32.get(), c->val32.get(), max_iter,
-                                      rtol, check_every, niter, rnorm, resume);
+  const double* b = rhs_view(c, b64);
+  double* xv = x;
+  const unsigned g = blocks_for(c->n_nodes, 256);
+  if (fp32_permuted(c)) {
+    c->w_perm2.resize(size_t(c->n_dof));
+    xv = c->w_perm2.get();
+    hipLaunchKernelGGL((k_to_box<double, double>), dim3(g), dim3(256), 0, c->stream, c->n_nodes,
+                       c->sym_iperm.get(), x, xv);
+  }
+  const int code = pcg_run<float, true, double>(c, b, xv, nullptr, c->s_r.get(), c->s_p.get(),
+                                                c->s_ap.get(), diag_view(c), c->val32.get(),
+                                                max_iter, rtol, check_every, niter, rnorm,
+                                                resume);
+  if (xv != x)
+    hipLaunchKernelGGL((k_from_box<double, double>), dim3(g), dim3(256), 0, c->stream,
+                       c->n_nodes, c->sym_iperm.get(), xv, x);
+  UFC_HIP_CHECK(hipGetLastError());
+  return code;
 }
 
 int pcg32_segment(ufc_ctx* c, const double* b64, float* d, bool resume, int max_iter,
                   double rtol, int check_every, int* niter, double* rnorm) {
   UFC_HIP_CHECK(hipMemsetAsync(d, 0, sizeof(float) * size_t(c->n_dof), c->stream));
-  return pcg_run<float, true, float>(c, b64, d, nullptr, c->s_r.get(), c->s_p.get(),
-                                     c->s_ap.get(), c->diag32.get(), c->val32.get(), max_iter,
+  const double* b = rhs_view(c, b64);
+  return pcg_run<float, true, float>(c, b, d, nullptr, c->s_r.get(), c->s_p.get(),
+                                     c->s_ap.get(), diag_view(c), c->val32.get(), max_iter,
                                      rtol, check_every, niter, rnorm, resume);
+}
+
+void to_view32(ufc_ctx* c, const double* a, float* o) {
+  if (!fp32_permuted(c)) return cast_f64_to_f32(c, a, o);
+  hipLaunchKernelGGL((k_to_box<double, float>), dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0,
+                     c->stream, c->n_nodes, c->sym_iperm.get(), a, o);
+  UFC_HIP_CHECK(hipGetLastError());
+}
+
+void from_view32(ufc_ctx* c, const float* a, double* o) {
+  if (!fp32_permuted(c)) return cast_f32_to_f64(c, a, o);
+  hipLaunchKernelGGL((k_from_box<float, double>), dim3(blocks_for(c->n_nodes, 256)), dim3(256),
+                     0, c->stream, c->n_nodes, c->sym_iperm.get(), a, o);
+  UFC_HIP_CHECK(hipGetLastError());
 }
 
 void cast_f64_to_f32(ufc_ctx* c, const double* a, float* o) {
@@ -1376,8 +1454,12 @@ void reset_spmv_events(ufc_ctx* c) {
 }
 
 void axpy_f32_to_f64(ufc_ctx* c, const float* d, double* u) {
-  hipLaunchKernelGGL(k_axpy_f32, dim3(blocks_for(c->n_dof, 256)), dim3(256), 0, c->stream,
-                     c->n_dof, d, u);
+  if (fp32_permuted(c))
+    hipLaunchKernelGGL(k_axpy_f32_box, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0,
+                       c->stream, c->n_nodes, c->sym_iperm.get(), d, u);
+  else
+    hipLaunchKernelGGL(k_axpy_f32, dim3(blocks_for(c->n_dof, 256)), dim3(256), 0, c->stream,
+                       c->n_dof, d, u);
   UFC_HIP_CHECK(hipGetLastError());
 }
 

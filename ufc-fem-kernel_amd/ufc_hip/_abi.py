@@ -185,6 +185,8 @@ _SIGS = {
                      C.c_double, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_field": [C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_void_p,
                   P(Status)],
+    "ufc_mesh_box_numbering": [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                               P(Status)],
     "ufc_rt_restart_save": [C.c_void_p, C.c_char_p, P(RestartData), C.c_int32, P(Status)],
     "ufc_rt_restart_load": [C.c_void_p, C.c_char_p, P(RestartData), C.c_int32, P(Status)],
     "ufc_ap_read_inp": [C.c_char_p, P(ApModel), P(Status)],

@@ -101,6 +101,23 @@ def mesh_box(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0, index_base=
     return coords, conn
 
 
+def box_numbering(conn, n_nodes, index_base=0):
+    """ufc_mesh_box_numbering: perm (n_nodes,) with perm[n] the lexicographic
+    number of node n when the hex mesh is a structured box under some node
+    numbering, else None."""
+    L = A.lib()
+    conn = np.ascontiguousarray(conn, np.int32)
+    perm = np.empty(n_nodes, np.int32)
+    st = A.Status()
+    code = L.ufc_mesh_box_numbering(n_nodes, conn.shape[0], index_base,
+                                    conn.ctypes.data_as(C.c_void_p),
+                                    perm.ctypes.data_as(C.c_void_p), C.byref(st))
+    if code == A.STATUS_UNSUPPORTED:
+        return None
+    A.check(code, st)
+    return perm
+
+
 def field(kind: str, n: int, seed: int, a: float, b: float) -> np.ndarray:
     """kind 'lognormal': exp(a + b z); kind 'uniform': U[a, b]; keyed by (seed, element)."""
     L = A.lib()
