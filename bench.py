@@ -311,6 +311,7 @@ def renumbered_run(args, torch, dev, ufc_hip, P, n, sols):
     t0 = time.perf_counter()
     cr = ufc_hip.Context(0)
     P.setup_context(cr, pr, ufc_hip.PREC_MIXED)
+    cr.enable_timing(SPMV_EVENT_EVERY)
     t1 = time.perf_counter()
     pr.young_dev = torch.as_tensor(pr.young, device=dev)
     x = torch.zeros(pr.n_dof, dtype=torch.float64, device=dev)

@@ -153,3 +153,64 @@ def test_j2_cube16_newton_vs_oracle(ctx, orc):
     assert np.abs(sg[:, :6] - so[:, :6]).max() < 1e-6 * np.abs(so[:, :6]).max()
     assert np.abs(sg[:, 6] - so[:, 6]).max() < 1e-6 * so[:, 6].max()
     assert np.abs(sg[:, 7:] - so[:, 7:]).max() < 1e-6 * np.abs(so[:, 7:]).max()
+
+
+def test_nnz_beyond_int32():
+    """A 208 x 206 x 206 box (8.96M nodes, 26.9M DOF, nnz 2,155,283,649 >
+    2^31): value positions are 64-bit, so assembly, both SpMVs and both
+    solves run; the CSR export, whose row_ptr is the reference's int32
+    rowPtr, refuses (UFC_STATUS_UNSUPPORTED).  Size-independent checks: the
+    FP64 K annihilates rigid translations and an infinitesimal rotation, the
+    FP32 symmetric view applies K to FP32 rounding, the mixed and FP64
+    solutions agree to 1e-8, and the element-by-element FP64 residual of the
+    mixed solution (no CSR involved) is below 2e-10."""
+    import ufc_hip
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    nx, ny, nz = 208, 206, 206
+    coords, conn = ufc_hip.mesh_box(nx, ny, nz, 1.0, 1.0, 1.0)
+    young = ufc_hip.field("lognormal", conn.shape[0], P.SEED, np.log(1e9), 0.5)
+    z0 = np.nonzero(coords[:, 2] == 0.0)[0]
+    clamp = (3 * z0[:, None] + np.arange(3)).ravel().astype(np.int32)
+    top = np.arange(nx * ny * nz, dtype=np.int32).reshape(nz, ny, nx)[nz - 1].ravel()
+    with ufc_hip.Context(0) as ctx:
+        say("nnz > 2^31: pattern")
+        info = ctx.build_pattern(coords, conn)
+        assert info.nnz == 9 * (3 * nx + 1) * (3 * ny + 1) * (3 * nz + 1) > 2 ** 31 - 1
+        assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG
+        with pytest.raises(A.UfcError) as e:
+            ctx.get_pattern()
+        assert e.value.code == A.STATUS_UNSUPPORTED
+        ctx.assemble_k(young, 0.3, precision=0)
+        n = ctx.n_dof
+        rnd = np.random.default_rng(5).normal(size=n)
+        y_r = ctx.spmv(rnd, precision=0)
+        for d in range(3):
+            t = np.zeros(n)
+            t[d::3] = 1.0
+            y_t = ctx.spmv(t, precision=0)
+            say("translation", d, np.linalg.norm(y_t) / np.linalg.norm(y_r))
+            assert np.linalg.norm(y_t) <= 1e-12 * np.linalg.norm(y_r) * np.sqrt(n)
+        rot = np.zeros((coords.shape[0], 3))
+        rot[:, 0], rot[:, 1] = -coords[:, 1], coords[:, 0]  # rotation about z
+        y_rot = ctx.spmv(rot.ravel(), precision=0)
+        assert np.linalg.norm(y_rot) <= 1e-10 * np.linalg.norm(y_r)
+        ctx.load_pressure(top, np.full(top.size, 1, np.int32), 1.0e6)
+        ctx.apply_bc(clamp, None, P.PENALTY)
+        ctx.assemble_k(young, 0.3, precision=0)
+        y64 = ctx.spmv(rnd, precision=0)
+        say("nnz > 2^31: FP64 solve")
+        u64, i64 = ctx.pcg(precision=0, rtol=1e-10)
+        assert i64.converged
+        ctx.assemble_k(young, 0.3, precision=2)
+        y32 = ctx.spmv(rnd, precision=1)
+        assert np.abs(y32 - y64).max() <= 1e-5 * np.abs(y64).max()
+        say("nnz > 2^31: mixed solve")
+        um, im = ctx.pcg(precision=2, rtol=1e-10, replace_rtol=0.1)
+        assert im.converged
+        d = np.linalg.norm(um - u64) / np.linalg.norm(u64)
+        r, norms = ctx.residual(um)
+        F = ctx.get_rhs()
+        rel = norms[0] / np.linalg.norm(F)
+        say("nnz > 2^31: iterations", i64.niter, im.niter, "mixed vs FP64", d, "residual", rel)
+        assert d < 1e-8 and rel < 2e-10

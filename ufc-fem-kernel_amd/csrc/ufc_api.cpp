@@ -862,21 +862,27 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     });
     UFC_REQUIRE(bad[0] < 0, UFC_STATUS_UNSUPPORTED,
                 "node " + std::to_string(bad[0] + base) + " has more than 32 neighbours");
-    c->h_nb_ptr.assign(size_t(nn) + 1, 0);
+    // node-pair pointers are int32 (nnz / 9 < 2^31); value positions are
+    // int64, so nnz itself may pass 2^31 (the CSR export, whose row_ptr is
+    // the reference's int32 rowPtr, then refuses)
+    int64_t npairs = 0;
     int32_t maxdeg = 0;
     for (int32_t n = 0; n < nn; ++n) {
       UFC_REQUIRE(deg[size_t(n)] > 0, UFC_STATUS_INVALID,
                   "node " + std::to_string(n + base) + " belongs to no element");
-      c->h_nb_ptr[size_t(n) + 1] = c->h_nb_ptr[size_t(n)] + deg[size_t(n)];
+      npairs += deg[size_t(n)];
       maxdeg = std::max(maxdeg, deg[size_t(n)]);
     }
-    const int64_t npairs = c->h_nb_ptr[size_t(nn)];
+    UFC_REQUIRE(npairs < INT32_MAX, UFC_STATUS_UNSUPPORTED,
+                "more than 2^31 - 1 node pairs (nnz / 9)");
+    c->h_nb_ptr.assign(size_t(nn) + 1, 0);
+    for (int32_t n = 0; n < nn; ++n)
+      c->h_nb_ptr[size_t(n) + 1] = c->h_nb_ptr[size_t(n)] + deg[size_t(n)];
     const int64_t nnz = 9 * npairs;
-    UFC_REQUIRE(nnz < INT32_MAX, UFC_STATUS_UNSUPPORTED,
-                "nnz exceeds the int32 CSR of the reference (NM_Assem_Sparse.f90:45-56)");
+    const bool csr32 = nnz < INT32_MAX;
     std::vector<int32_t> nb_idx(static_cast<size_t>(npairs));
-    std::vector<int32_t> row_ptr(static_cast<size_t>(nn) * 3 + 1);
-    std::vector<int32_t> diag_pos(static_cast<size_t>(nn) * 3);
+    std::vector<int32_t> row_ptr(csr32 ? static_cast<size_t>(nn) * 3 + 1 : 0);
+    std::vector<int64_t> diag_pos(static_cast<size_t>(nn) * 3);
     std::vector<uint64_t> s_code(static_cast<size_t>(nn) * 8, 0);
     parallel_for(nn, [&](int64_t n) {
       const int32_t q0 = c->h_nb_ptr[size_t(n)];
@@ -885,8 +891,8 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       std::copy(L, L + dg, nb_idx.begin() + q0);
       const int kself = int(std::lower_bound(L, L + dg, int32_t(n)) - L);
       for (int d = 0; d < 3; ++d) {
-        row_ptr[3 * size_t(n) + d] = 9 * q0 + d * 3 * dg;
-        diag_pos[3 * size_t(n) + d] = 9 * q0 + d * 3 * dg + 3 * kself + d;
+        if (csr32) row_ptr[3 * size_t(n) + d] = int32_t(9 * int64_t(q0) + d * 3 * dg);
+        diag_pos[3 * size_t(n) + d] = 9 * int64_t(q0) + d * 3 * dg + 3 * kself + d;
       }
       for (int s = 0; s < 8; ++s) {
         const int32_t e = s_elem[8 * size_t(n) + s];
@@ -900,7 +906,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
         s_code[8 * size_t(n) + s] = code;
       }
     });
-    row_ptr[size_t(nn) * 3] = int32_t(nnz);
+    if (csr32) row_ptr[size_t(nn) * 3] = int32_t(nnz);
     // 16-bit neighbour offsets when the numbering's bandwidth allows
     int64_t maxoff = 0;
     for (int64_t n = 0; n < nn; ++n)
@@ -922,7 +928,10 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     c->conn.resize(size_t(ne) * 8);
     c->nb_ptr.resize(size_t(nn) + 1 + ufc::kNbPtrPad);
     c->nb_idx.resize(size_t(npairs));
-    c->row_ptr.resize(size_t(nn) * 3 + 1);
+    if (csr32)
+      c->row_ptr.resize(size_t(nn) * 3 + 1);
+    else
+      c->row_ptr.release();
     c->diag_pos.resize(size_t(nn) * 3);
     c->slot_elem.resize(size_t(nn) * 8);
     c->slot_code.resize(size_t(nn) * 8);
@@ -957,7 +966,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     build_sym_diagonals(c, nb_idx, s_elem);
     c->fp32_view_request = -1;
     ufc::select_fp32_view(c);
-    up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
+    if (csr32) up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());
     up(c->diag_pos.get(), diag_pos.data(), c->diag_pos.bytes());
     up(c->slot_elem.get(), s_elem.data(), c->slot_elem.bytes());
     up(c->slot_code.get(), s_code.data(), c->slot_code.bytes());
@@ -992,6 +1001,9 @@ int ufc_rt_asm_get_pattern(ufc_ctx* c, int32_t index_base, int32_t* row_ptr, int
     check_mesh(c);
     UFC_REQUIRE(row_ptr || col_idx, UFC_STATUS_INVALID, "null argument");
     UFC_REQUIRE(index_base == 0 || index_base == 1, UFC_STATUS_INVALID, "index_base must be 0 or 1");
+    UFC_REQUIRE(c->nnz + index_base <= INT32_MAX, UFC_STATUS_UNSUPPORTED,
+                "nnz = " + std::to_string(c->nnz) + " does not fit the int32 rowPtr of "
+                "RT_CSRMatrix (NM_Assem_Sparse.f90:45-56)");
     UFC_HIP_CHECK(hipSetDevice(c->device));
     if (!c->col_idx.get()) ufc::build_col_idx(c);
     if (row_ptr) {

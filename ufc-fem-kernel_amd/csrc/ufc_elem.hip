@@ -569,7 +569,7 @@ __global__ __launch_bounds__(64) void k_assemble(
 }
 
 template <typename T>
-__global__ void k_diag(int64_t n, const int32_t* __restrict__ pos, const T* __restrict__ val,
+__global__ void k_diag(int64_t n, const int64_t* __restrict__ pos, const T* __restrict__ val,
                        T* __restrict__ diag) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) diag[i] = val[pos[i]];
@@ -579,7 +579,7 @@ __global__ void k_diag(int64_t n, const int32_t* __restrict__ pos, const T* __re
 // applies sum_{m != n} K_nm (x_m - x_n) plus the penalty diagonal stored in
 // the self block)
 __global__ void k_zero_self(int32_t n_nodes, const int32_t* __restrict__ nb_ptr,
-                            const int32_t* __restrict__ diag_pos, float* __restrict__ val) {
+                            const int64_t* __restrict__ diag_pos, float* __restrict__ val) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   const int d3 = 3 * (nb_ptr[n + 1] - nb_ptr[n]);
@@ -597,7 +597,7 @@ __global__ void k_zero_self(int32_t n_nodes, const int32_t* __restrict__ nb_ptr,
 // per row.  Both sides are contiguous across the lanes.
 __global__ __launch_bounds__(256) void k_offdiag_compact(int32_t n_nodes,
                                                          const int32_t* __restrict__ nb_ptr,
-                                                         const int32_t* __restrict__ diag_pos,
+                                                         const int64_t* __restrict__ diag_pos,
                                                          const float* __restrict__ val,
                                                          float* __restrict__ valm) {
   const int k = threadIdx.x & 31;
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
 // start at multiples of 64 nodes, so they fill two whole words).  iperm !=
 // null: thread n is box number n and reads node iperm[n].
 __global__ __launch_bounds__(256) void k_offdiag_penalty(int32_t n_nodes,
-                                                         const int32_t* __restrict__ diag_pos,
+                                                         const int64_t* __restrict__ diag_pos,
                                                          const float* __restrict__ val,
                                                          const int32_t* __restrict__ iperm,
                                                          float* __restrict__ pen,
@@ -713,7 +713,7 @@ __global__ void k_penalty_vec(int32_t n_bc, const int32_t* __restrict__ eq,
 
 template <typename T>
 __global__ void k_penalty(int32_t n_bc, const int32_t* __restrict__ eq,
-                          const double* __restrict__ pen, const int32_t* __restrict__ pos,
+                          const double* __restrict__ pen, const int64_t* __restrict__ pos,
                           T* __restrict__ val, double sign) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_bc) {

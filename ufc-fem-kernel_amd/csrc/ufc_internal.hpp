@@ -223,9 +223,9 @@ struct ufc_ctx {
   ufc::DBuf<double> w_perm2;     // FP64 iterate in the box numbering (restarted form)
   int32_t fp32_view = -1;       // view of the FP32 SpMV (kFp32View*); -1 before the pattern
   int32_t fp32_view_request = -1;  // -1: the fastest available
-  ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1)
+  ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1) when nnz < 2^31 (the exported CSR)
   ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
-  ufc::DBuf<int32_t> diag_pos;    // (n_dof) position of K(i,i) in values
+  ufc::DBuf<int64_t> diag_pos;    // (n_dof) position of K(i,i) in values
   // element-to-slot map, node major: for node n and slot s < 8, the element
   // (ascending, -1 if none) and a code: bits 5b..5b+4 = position in n's
   // neighbour list of the element's local node b, bits 40..42 = n's local index
