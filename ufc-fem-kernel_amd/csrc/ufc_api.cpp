@@ -340,6 +340,7 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
   c->diag_a.release();
   c->diag_b.release();
   c->diag_c.release();
+  c->diag_from_assembly = false;
   const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
   std::vector<int32_t> all;
   if (!collect_sym_offsets(c, nb_idx, nullptr, all)) {
@@ -501,6 +502,7 @@ void select_fp32_view(ufc_ctx* c) {
     c->diag_a.release();
     c->diag_b.release();
     c->diag_c.release();
+    c->diag_from_assembly = false;
   }
 }
 

@@ -491,12 +491,29 @@ __global__ __launch_bounds__(64) void k_eval_ke(int32_t n_elems, const double* _
 // exactly once, without atomics.  The 8 nodes' segments are contiguous in the
 // values array (node n's 3 rows start at 9*nb_ptr[n]), so the write-out is a
 // single coalesced stream.
+// Block diagonals of the symmetric FP32 view written by the FP32 assembly
+// itself (nd = 0: none): for node n and diagonal s, the block K(n, m) with m
+// the node at box number perm[n] + o_s, found in n's sorted neighbour list,
+// goes to entry (s, perm[n]) (perm = null: the node numbering is the box's).
+struct DiagWrite {
+  DiagOffsets off;
+  int nd;
+  int32_t pad_tiles;
+  const int32_t* perm;
+  const int32_t* iperm;
+  const int32_t* nb_idx;
+  float4* da;
+  float4* db;
+  float* dc;
+};
+
 template <typename T, int MAT, int VAR>
 __global__ __launch_bounds__(64) void k_assemble(
     int32_t n_nodes, const double* __restrict__ coords, const int32_t* __restrict__ conn,
     const int32_t* __restrict__ slot_elem, const uint64_t* __restrict__ slot_code,
     const int32_t* __restrict__ nb_ptr, const double* __restrict__ young, double nu,
-    const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val) {
+    const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val,
+    DiagWrite dw) {
   __shared__ T acc[8 * 9 * kMaxNodeNbrs];
   const int lane = threadIdx.x;
   const int nl = lane >> 3, s = lane & 7;
@@ -566,6 +583,43 @@ __global__ __launch_bounds__(64) void k_assemble(
   }
   T* out = val + 9 * q0;
   for (int i = lane; i < total; i += 64) out[i] = acc[i];
+  if constexpr (sizeof(T) == 4) {
+    if (dw.nd > 0) {
+      // lane = (diagonal s, node): eight consecutive nodes of one diagonal
+      // write consecutive entries
+      for (int pr = lane; pr < 8 * kMaxDiags; pr += 64) {
+        const int ln = pr & 7, sd = pr >> 3;
+        const int64_t nn0 = node0 + ln;
+        if (sd >= dw.nd || nn0 >= nodeN) continue;
+        const int64_t bn = dw.perm ? dw.perm[nn0] : nn0;
+        const int64_t bt = bn + dw.off.o[sd];
+        if (bt >= n_nodes) continue;
+        const int32_t m = dw.iperm ? dw.iperm[bt] : int32_t(bt);
+        const int64_t p0 = nb_ptr[nn0];
+        const int dg = int(nb_ptr[nn0 + 1] - p0);
+        int lo = 0, hi = dg - 1, q = -1;
+        while (lo <= hi) {
+          const int mid = (lo + hi) >> 1;
+          const int32_t v = dw.nb_idx[p0 + mid];
+          if (v == m) {
+            q = mid;
+            break;
+          }
+          if (v < m)
+            lo = mid + 1;
+          else
+            hi = mid - 1;
+        }
+        if (q < 0) continue;  // no neighbour at this offset: the entry stays zero
+        const T* src = acc + 9 * (p0 - q0) + 3 * q;
+        const int d3 = 3 * dg;
+        const int64_t e = ((bn / kDiagTile + dw.pad_tiles) * dw.nd + sd) * kDiagTile + bn % kDiagTile;
+        dw.da[e] = make_float4(src[0], src[1], src[2], src[d3]);
+        dw.db[e] = make_float4(src[d3 + 1], src[d3 + 2], src[2 * d3], src[2 * d3 + 1]);
+        dw.dc[e] = src[2 * d3 + 2];
+      }
+    }
+  }
 }
 
 template <typename T>
@@ -1118,31 +1172,61 @@ void eval_ke_var(int32_t n_elems, const double* coords, const double* young, dou
 }
 
 template <typename T, int VAR>
-void assemble_elastic(ufc_ctx* c, T* val) {
+void assemble_elastic(ufc_ctx* c, T* val, const DiagWrite& dw) {
   hipLaunchKernelGGL((k_assemble<T, UFC_MAT_ELASTIC, VAR>), dim3(blocks_for(c->n_nodes, 8)),
                      dim3(64), 0, c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
                      c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), c->young.get(),
-                     c->nu, nullptr, c->inc_c.get(), val);
+                     c->nu, nullptr, c->inc_c.get(), val, dw);
 }
 
 template <typename T>
-void assemble_variant(ufc_ctx* c, T* val) {
+void assemble_variant(ufc_ctx* c, T* val, const DiagWrite& dw) {
   switch (c->variant) {
     case kVarReduced:
-      assemble_elastic<T, kVarReduced>(c, val);
+      assemble_elastic<T, kVarReduced>(c, val, dw);
       break;
     case kVarSelective:
-      assemble_elastic<T, kVarSelective>(c, val);
+      assemble_elastic<T, kVarSelective>(c, val, dw);
       break;
     case kVarIncompat:
-      assemble_elastic<T, kVarIncompat>(c, val);
+      assemble_elastic<T, kVarIncompat>(c, val, dw);
       break;
     case kVarGauss27:
-      assemble_elastic<T, kVarGauss27>(c, val);
+      assemble_elastic<T, kVarGauss27>(c, val, dw);
       break;
     default:
-      assemble_elastic<T, kVarFull>(c, val);
+      assemble_elastic<T, kVarFull>(c, val, dw);
   }
+}
+
+// the symmetric view's arrays, zeroed once: the pattern fixes which entries
+// are ever written, so the others stay zero
+void ensure_diag_arrays(ufc_ctx* c) {
+  if (c->diag_c.get()) return;
+  const size_t entries = size_t(c->diag_entries);
+  c->diag_a.resize(4 * entries);
+  c->diag_b.resize(4 * entries);
+  c->diag_c.resize(entries);
+  UFC_HIP_CHECK(hipMemsetAsync(c->diag_a.get(), 0, c->diag_a.bytes(), c->stream));
+  UFC_HIP_CHECK(hipMemsetAsync(c->diag_b.get(), 0, c->diag_b.bytes(), c->stream));
+  UFC_HIP_CHECK(hipMemsetAsync(c->diag_c.get(), 0, c->diag_c.bytes(), c->stream));
+}
+
+DiagWrite diag_write(ufc_ctx* c, bool fp32) {
+  DiagWrite dw{};
+  dw.nd = 0;
+  if (!fp32 || c->fp32_view != kFp32ViewSymDiag || c->n_diag == 0) return dw;
+  ensure_diag_arrays(c);
+  for (int s = 0; s < kMaxDiags; ++s) dw.off.o[s] = c->h_diag_off[s];
+  dw.nd = c->n_diag;
+  dw.pad_tiles = c->diag_pad_tiles;
+  dw.perm = fp32_permuted(c) ? c->sym_perm.get() : nullptr;
+  dw.iperm = fp32_permuted(c) ? c->sym_iperm.get() : nullptr;
+  dw.nb_idx = c->nb_idx.get();
+  dw.da = reinterpret_cast<float4*>(c->diag_a.get());
+  dw.db = reinterpret_cast<float4*>(c->diag_b.get());
+  dw.dc = c->diag_c.get();
+  return dw;
 }
 
 template <int VAR, bool SENS>
@@ -1214,22 +1298,24 @@ void launch_assemble(ufc_ctx* c, int precision, int material) {
   if (material == UFC_MAT_ELASTIC && c->variant == kVarIncompat)
     launch_incompat_c(c->n_elems, c->coords.get(), c->conn.get(), c->young.get(), c->nu,
                       c->inc_c.get(), c->stream);
+  const DiagWrite dw = diag_write(c, precision != UFC_PREC_FP64);
+  if (precision != UFC_PREC_FP64) c->diag_from_assembly = dw.nd > 0;
   if (precision == UFC_PREC_FP64) {
     if (material == UFC_MAT_ELASTIC)
-      assemble_variant<double>(c, c->val64.get());
+      assemble_variant<double>(c, c->val64.get(), dw);
     else
       hipLaunchKernelGGL((k_assemble<double, UFC_MAT_J2, kVarFull>), dim3(grid), dim3(64), 0,
                          c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
                          c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), nullptr, c->nu,
-                         dgp, nullptr, c->val64.get());
+                         dgp, nullptr, c->val64.get(), dw);
   } else {
     if (material == UFC_MAT_ELASTIC)
-      assemble_variant<float>(c, c->val32.get());
+      assemble_variant<float>(c, c->val32.get(), dw);
     else
       hipLaunchKernelGGL((k_assemble<float, UFC_MAT_J2, kVarFull>), dim3(grid), dim3(64), 0,
                          c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
                          c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), nullptr, c->nu,
-                         dgp, nullptr, c->val32.get());
+                         dgp, nullptr, c->val32.get(), dw);
   }
   UFC_HIP_CHECK(hipGetLastError());
 }
@@ -1270,23 +1356,19 @@ void launch_fp32_offdiag(ufc_ctx* c) {
     hipLaunchKernelGGL(k_offdiag_compact, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->diag_pos.get(), c->val32.get(), c->val32m.get());
   } else {
-    const size_t entries = size_t(c->diag_entries);
-    if (!c->diag_c.get()) {
-      c->diag_a.resize(4 * entries);
-      c->diag_b.resize(4 * entries);
-      c->diag_c.resize(entries);
-      UFC_HIP_CHECK(hipMemsetAsync(c->diag_a.get(), 0, c->diag_a.bytes(), c->stream));
-      UFC_HIP_CHECK(hipMemsetAsync(c->diag_b.get(), 0, c->diag_b.bytes(), c->stream));
-      UFC_HIP_CHECK(hipMemsetAsync(c->diag_c.get(), 0, c->diag_c.bytes(), c->stream));
-    }
-    DiagOffsets off;
-    for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
     const bool perm = fp32_permuted(c);
-    hipLaunchKernelGGL(k_diag_fill, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
-                       c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), off, int(c->n_diag),
-                       c->diag_pad_tiles, perm ? c->sym_perm.get() : nullptr,
-                       reinterpret_cast<float4*>(c->diag_a.get()),
-                       reinterpret_cast<float4*>(c->diag_b.get()), c->diag_c.get());
+    if (!c->diag_from_assembly) {
+      // the view was chosen after the FP32 assembly: fill it from val32
+      ensure_diag_arrays(c);
+      DiagOffsets off;
+      for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
+      hipLaunchKernelGGL(k_diag_fill, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
+                         c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), off, int(c->n_diag),
+                         c->diag_pad_tiles, perm ? c->sym_perm.get() : nullptr,
+                         reinterpret_cast<float4*>(c->diag_a.get()),
+                         reinterpret_cast<float4*>(c->diag_b.get()), c->diag_c.get());
+      c->diag_from_assembly = true;  // filled now: a later penalty change keeps them
+    }
     if (perm) {
       // the Jacobi diagonal of the FP32 PCG in the box numbering
       c->diag32p.resize(size_t(c->n_dof));

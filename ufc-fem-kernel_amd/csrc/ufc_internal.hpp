@@ -215,6 +215,9 @@ struct ufc_ctx {
   // entries, pen32, pen_flag, diag32p and the FP32 PCG's vectors are then in
   // the box numbering and converted at the solve's boundary.
   bool sym_permuted = false;
+  // the block diagonals hold the current FP32 values (written by the FP32
+  // assembly, or filled from val32 when the view was selected afterwards)
+  bool diag_from_assembly = false;
   std::vector<int32_t> h_sym_perm;
   ufc::DBuf<int32_t> sym_perm;   // (n_nodes)
   ufc::DBuf<int32_t> sym_iperm;  // (n_nodes)
