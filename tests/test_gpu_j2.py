@@ -215,3 +215,29 @@ def test_j2_newton_failure_cuts_back_and_stops(ctx, tmp_path):
     assert rs.lambda_ == res.increments[-1].load_factor
     assert np.array_equal(u_ck, res.u)
     assert np.array_equal(ctx.j2_get_state(), s_fail)
+
+
+@pytest.mark.parametrize("law", ["linear", "swift", "voce"])
+@pytest.mark.parametrize("precision", [0, 2])
+def test_j2_uniaxial_stress_closed_form(ctx, law, precision):
+    """A boundary-value problem with a closed form through the GPU Newton
+    loop: a 4x4x4 unit cube on rollers, u_z on top driven to 1% strain in ten
+    increments.  The FE solution is homogeneous uniaxial stress, so at 0.5%
+    and 1% every Gauss point's stress and plastic strain, and the lateral
+    face, equal the law's closed form (bisection on sigma = sigma_y(peeq),
+    not the radial return).  Newton stops at 1e-7 of the internal force (the
+    penalty rows resolve 1e12 ulp(u))."""
+    from ufc_hip import static as S
+    from ufc_hip.nonlinear import j2_static
+    from j2_cases import J2_LAWS_MPA, check_uniaxial_state, uniaxial_bvp
+    coords, conn = S.mesh_box(4, 4, 4)
+    eq, top, x1 = uniaxial_bvp(coords)
+    props = J2_LAWS_MPA[law]
+    print(f"[gpu-test] j2 uniaxial {law} precision {precision}", flush=True)
+    for stop, e33 in ((5, 0.005), (None, 0.01)):
+        ctx.build_pattern(coords, conn)
+        res = j2_static(ctx, eq, 0.01 * top, props, n_increments=10, precision=precision,
+                        tol_r=1e-7, lin_rtol=1e-12, stop_after=stop)
+        assert res.converged and res.cutbacks == 0
+        assert abs(res.increments[-1].load_factor - e33 / 0.01) < 1e-12
+        check_uniaxial_state(props, e33, res.u, ctx.j2_get_state(), x1)

@@ -11,6 +11,9 @@ import os
 import numpy as np
 import pytest
 
+from j2_cases import (J2_LAWS_MPA, UNIAXIAL_PATH, check_uniaxial_state, sigma_y as _sigma_y,
+                      uniaxial_bvp, uniaxial_closed_form as _uniaxial_closed_form)
+
 UNIT = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                  [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
 GP3 = [(-math.sqrt(0.6), 5 / 9), (0.0, 8 / 9), (math.sqrt(0.6), 5 / 9)]
@@ -429,34 +432,6 @@ J2_LAWS = {
 }
 
 
-def _sigma_y(props, e):
-    E, nu, sy0, H, t, _, K, n, e0, Q, d = props
-    if t == 2:
-        return K * (e0 + e) ** n
-    if t == 3:
-        return sy0 + Q * (1.0 - math.exp(-d * e))
-    return sy0 + H * e
-
-
-def _uniaxial_closed_form(props, e11):
-    """Monotonic uniaxial stress: sigma = sigma_y(peeq) with
-    e11 = sigma / E + peeq beyond yield, found by bisection on
-    g(sigma) = sigma - sigma_y(e11 - sigma / E), which increases with sigma;
-    sigma = E e11 below yield."""
-    E = props[0]
-    if E * e11 <= _sigma_y(props, 0.0):
-        return E * e11, 0.0
-    lo, hi = 0.0, E * e11
-    for _ in range(200):
-        mid = 0.5 * (lo + hi)
-        if mid - _sigma_y(props, e11 - mid / E) > 0.0:
-            hi = mid
-        else:
-            lo = mid
-    sig = 0.5 * (lo + hi)
-    return sig, e11 - sig / E
-
-
 @pytest.mark.parametrize("law", ["linear", "swift", "voce"])
 def test_j2_hardening_uniaxial_closed_form(orc, law):
     """Each hardening law under uniaxial stress against the closed form of
@@ -553,6 +528,28 @@ def test_j2_newton_converges_quadratically(orc):
     assert max(its) <= 6
     peeq = state.reshape(-1, 13)[:, 6]
     assert peeq.max() > 0  # yielded by 0.3% strain (yield strain 0.125%)
+
+
+@pytest.mark.parametrize("law", ["linear", "swift", "voce"])
+def test_j2_newton_uniaxial_stress_closed_form(orc, law):
+    """orc_j2_newton_step on a boundary-value problem with a closed form: a
+    2x2x2 unit cube on rollers, u_z on top driven through 0.1-1% strain
+    (E2E_C2_02 loading).  The FE solution is homogeneous uniaxial stress, so
+    each increment's Gauss-point stresses, plastic strains and lateral
+    displacement equal the law's own closed form.  Newton stops at 1e-7 of
+    the internal force: the penalty rows resolve P ulp(u) = 1e12 x 1.7e-18,
+    about 1e-8 of it here."""
+    props = J2_LAWS_MPA[law]
+    coords, conn = orc.mesh_box(2, 2, 2)
+    eq, top, x1 = uniaxial_bvp(coords)
+    state = np.zeros(8 * 8 * 13)
+    u = np.zeros(3 * coords.shape[0])
+    for e33 in UNIAXIAL_PATH:
+        st, u, state, nn, rn = orc.j2_newton_step(coords.shape[0], conn, coords, props, eq,
+                                                  e33 * top, 1e12, state, u, tol_r=1e-7)
+        assert st == 0, (law, e33)
+        check_uniaxial_state(props, e33, u, state, x1)
+    assert state.reshape(-1, 13)[:, 6].min() > 0.0  # every point yielded
 
 
 def test_spmv_transpose_against_dense(orc):
