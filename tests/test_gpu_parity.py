@@ -340,10 +340,15 @@ def test_pattern_rejects_invalid_meshes(ctx):
     with pytest.raises(A.UfcError) as e:
         ctx.build_pattern(coords, bad)
     assert e.value.code == A.STATUS_INVALID
-    many = np.concatenate([conn] + [conn[:1]] * 8)   # node 0 in 9 elements
+    # node 0 in 40 more elements of 7 fresh nodes each: 1 + 27 + 280 neighbours,
+    # beyond the general nodes' 256
+    nn = coords.shape[0]
+    fresh = nn + np.arange(40 * 7).reshape(40, 7)
+    many = np.concatenate([conn, np.column_stack([np.zeros(40, np.int32), fresh])]).astype(np.int32)
+    far = np.random.default_rng(0).uniform(3, 4, (fresh.size, 3))
     with pytest.raises(A.UfcError) as e:
-        ctx.build_pattern(coords, many)
-    assert e.value.code == A.STATUS_UNSUPPORTED
+        ctx.build_pattern(np.concatenate([coords, far]), many)
+    assert e.value.code == A.STATUS_UNSUPPORTED and "256 neighbours" in e.value.message
     iso = np.concatenate([coords, [[9.0, 9.0, 9.0]]])   # a node in no element
     with pytest.raises(A.UfcError) as e:
         ctx.build_pattern(iso, conn)

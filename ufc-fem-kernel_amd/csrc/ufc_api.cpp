@@ -329,8 +329,7 @@ std::vector<int32_t> box_numbering(int32_t nn, const std::vector<int32_t>& conn,
 bool collect_sym_offsets(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
                          const std::vector<int32_t>* perm, std::vector<int32_t>& all);
 
-void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
-                         const std::vector<int32_t>& node_elem) {
+void reset_sym_diagonals(ufc_ctx* c) {
   c->n_diag = 0;
   c->sym_permuted = false;
   c->h_sym_perm.clear();
@@ -341,6 +340,11 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
   c->diag_b.release();
   c->diag_c.release();
   c->diag_from_assembly = false;
+}
+
+void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
+                         const std::vector<int32_t>& node_elem) {
+  reset_sym_diagonals(c);
   const int64_t nn = int64_t(c->h_nb_ptr.size()) - 1;
   std::vector<int32_t> all;
   if (!collect_sym_offsets(c, nb_idx, nullptr, all)) {
@@ -822,56 +826,66 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     // node -> (element, local index), elements ascending
     std::vector<int32_t> cnt(static_cast<size_t>(nn), 0);
     for (int32_t v : c->h_conn) cnt[size_t(v)]++;
+    int32_t maxcnt = 0;
     for (int32_t n = 0; n < nn; ++n) {
-      UFC_REQUIRE(cnt[size_t(n)] <= ufc::kMaxNodeElems, UFC_STATUS_UNSUPPORTED,
-                  "node " + std::to_string(n + base) + " belongs to more than 8 elements");
       // a node in no element has an empty CSR row and no diagonal: its
       // equations could be neither solved nor constrained
       UFC_REQUIRE(cnt[size_t(n)] > 0, UFC_STATUS_INVALID,
                   "node " + std::to_string(n + base) + " belongs to no element");
+      maxcnt = std::max(maxcnt, cnt[size_t(n)]);
     }
+    std::vector<int64_t> ne_ptr(size_t(nn) + 1, 0);
+    for (int32_t n = 0; n < nn; ++n) ne_ptr[size_t(n) + 1] = ne_ptr[size_t(n)] + cnt[size_t(n)];
+    std::vector<int32_t> ne_elem(size_t(ne) * 8);
+    std::vector<uint8_t> ne_loc(size_t(ne) * 8);
+    {
+      std::vector<int64_t> at(ne_ptr.begin(), ne_ptr.end() - 1);
+      for (int32_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a) {
+          const int32_t n = c->h_conn[8 * size_t(e) + a];
+          const int64_t k = at[size_t(n)]++;
+          ne_elem[size_t(k)] = e;
+          ne_loc[size_t(k)] = uint8_t(a);
+        }
+    }
+    // the slot map: the first kMaxNodeElems elements of every node
     std::vector<int32_t> s_elem(static_cast<size_t>(nn) * 8, -1);
     std::vector<uint8_t> s_loc(static_cast<size_t>(nn) * 8, 0);
-    std::fill(cnt.begin(), cnt.end(), 0);
-    for (int32_t e = 0; e < ne; ++e)
-      for (int a = 0; a < 8; ++a) {
-        const int32_t n = c->h_conn[8 * size_t(e) + a];
-        const int s = cnt[size_t(n)]++;
-        s_elem[8 * size_t(n) + s] = e;
-        s_loc[8 * size_t(n) + s] = uint8_t(a);
+    parallel_for(nn, [&](int64_t n) {
+      const int64_t q0 = ne_ptr[size_t(n)];
+      const int k = int(std::min<int64_t>(ufc::kMaxNodeElems, ne_ptr[size_t(n) + 1] - q0));
+      for (int s = 0; s < k; ++s) {
+        s_elem[8 * size_t(n) + s] = ne_elem[size_t(q0 + s)];
+        s_loc[8 * size_t(n) + s] = ne_loc[size_t(q0 + s)];
       }
-    // neighbour lists (sorted, unique)
+    });
+    // neighbour lists (sorted, unique): the union of the node's elements
+    auto node_list = [&](int64_t n, std::vector<int32_t>& buf) {
+      buf.clear();
+      for (int64_t q = ne_ptr[size_t(n)]; q < ne_ptr[size_t(n) + 1]; ++q) {
+        const int32_t* en = c->h_conn.data() + 8 * size_t(ne_elem[size_t(q)]);
+        buf.insert(buf.end(), en, en + 8);
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+    };
     std::vector<int32_t> deg(static_cast<size_t>(nn));
-    std::vector<int32_t> nbtmp(static_cast<size_t>(nn) * ufc::kMaxNodeNbrs);
     std::vector<int32_t> bad(1, -1);
     parallel_for(nn, [&](int64_t n) {
-      int32_t* L = nbtmp.data() + n * ufc::kMaxNodeNbrs;
-      int32_t buf[64];
-      int k = 0;
-      for (int s = 0; s < 8; ++s) {
-        const int32_t e = s_elem[8 * size_t(n) + s];
-        if (e < 0) break;
-        for (int a = 0; a < 8; ++a) buf[k++] = c->h_conn[8 * size_t(e) + a];
-      }
-      std::sort(buf, buf + k);
-      const int u = int(std::unique(buf, buf + k) - buf);
-      deg[size_t(n)] = u;
-      if (u > ufc::kMaxNodeNbrs) {
-        bad[0] = int32_t(n);
-        return;
-      }
-      std::copy(buf, buf + u, L);
+      thread_local std::vector<int32_t> buf;
+      node_list(n, buf);
+      deg[size_t(n)] = int32_t(buf.size());
+      if (deg[size_t(n)] > ufc::kMaxGenNbrs) bad[0] = int32_t(n);
     });
     UFC_REQUIRE(bad[0] < 0, UFC_STATUS_UNSUPPORTED,
-                "node " + std::to_string(bad[0] + base) + " has more than 32 neighbours");
+                "node " + std::to_string(bad[0] + base) + " has more than " +
+                    std::to_string(ufc::kMaxGenNbrs) + " neighbours");
     // node-pair pointers are int32 (nnz / 9 < 2^31); value positions are
     // int64, so nnz itself may pass 2^31 (the CSR export, whose row_ptr is
     // the reference's int32 rowPtr, then refuses)
     int64_t npairs = 0;
     int32_t maxdeg = 0;
     for (int32_t n = 0; n < nn; ++n) {
-      UFC_REQUIRE(deg[size_t(n)] > 0, UFC_STATUS_INVALID,
-                  "node " + std::to_string(n + base) + " belongs to no element");
       npairs += deg[size_t(n)];
       maxdeg = std::max(maxdeg, deg[size_t(n)]);
     }
@@ -880,6 +894,12 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     c->h_nb_ptr.assign(size_t(nn) + 1, 0);
     for (int32_t n = 0; n < nn; ++n)
       c->h_nb_ptr[size_t(n) + 1] = c->h_nb_ptr[size_t(n)] + deg[size_t(n)];
+    // general nodes: blocks of 8 nodes with one outside the slot map
+    const int64_t nblk = (int64_t(nn) + 7) / 8;
+    std::vector<uint8_t> gflag(static_cast<size_t>(nblk), 0);
+    for (int32_t n = 0; n < nn; ++n)
+      if (cnt[size_t(n)] > ufc::kMaxNodeElems || deg[size_t(n)] > ufc::kMaxNodeNbrs)
+        gflag[size_t(n / 8)] = 1;
     const int64_t nnz = 9 * npairs;
     const bool csr32 = nnz < INT32_MAX;
     std::vector<int32_t> nb_idx(static_cast<size_t>(npairs));
@@ -887,15 +907,18 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
     std::vector<int64_t> diag_pos(static_cast<size_t>(nn) * 3);
     std::vector<uint64_t> s_code(static_cast<size_t>(nn) * 8, 0);
     parallel_for(nn, [&](int64_t n) {
+      thread_local std::vector<int32_t> buf;
+      node_list(n, buf);
       const int32_t q0 = c->h_nb_ptr[size_t(n)];
       const int dg = deg[size_t(n)];
-      const int32_t* L = nbtmp.data() + n * ufc::kMaxNodeNbrs;
+      const int32_t* L = buf.data();
       std::copy(L, L + dg, nb_idx.begin() + q0);
       const int kself = int(std::lower_bound(L, L + dg, int32_t(n)) - L);
       for (int d = 0; d < 3; ++d) {
         if (csr32) row_ptr[3 * size_t(n) + d] = int32_t(9 * int64_t(q0) + d * 3 * dg);
         diag_pos[3 * size_t(n) + d] = 9 * int64_t(q0) + d * 3 * dg + 3 * kself + d;
       }
+      if (gflag[size_t(n / 8)]) return;  // the general kernels find positions themselves
       for (int s = 0; s < 8; ++s) {
         const int32_t e = s_elem[8 * size_t(n) + s];
         if (e < 0) break;
@@ -909,12 +932,14 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       }
     });
     if (csr32) row_ptr[size_t(nn) * 3] = int32_t(nnz);
-    // 16-bit neighbour offsets when the numbering's bandwidth allows
+    // 16-bit neighbour offsets when the numbering's bandwidth allows (and
+    // every node fits a 32-lane group)
     int64_t maxoff = 0;
     for (int64_t n = 0; n < nn; ++n)
       for (int32_t q = c->h_nb_ptr[size_t(n)]; q < c->h_nb_ptr[size_t(n) + 1]; ++q)
         maxoff = std::max<int64_t>(maxoff, std::abs(int64_t(nb_idx[size_t(q)]) - n));
-    c->have_off16 = maxoff <= 32767;
+    c->wide = maxdeg > ufc::kMaxNodeNbrs;
+    c->have_off16 = maxoff <= 32767 && !c->wide;
 
     c->n_nodes = nn;
     c->n_elems = ne;
@@ -965,7 +990,42 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       c->nb_off16.release();
     }
     build_stencil_patterns(c, off16);
-    build_sym_diagonals(c, nb_idx, s_elem);
+    // general nodes: their lists of 8 e + a, and the block flags
+    std::vector<int32_t> g_node, g_ptr(1, 0), g_ne;
+    for (int64_t b = 0; b < nblk; ++b) {
+      if (!gflag[size_t(b)]) continue;
+      for (int64_t n = 8 * b; n < std::min<int64_t>(nn, 8 * b + 8); ++n) {
+        g_node.push_back(int32_t(n));
+        for (int64_t q = ne_ptr[size_t(n)]; q < ne_ptr[size_t(n) + 1]; ++q)
+          g_ne.push_back(8 * ne_elem[size_t(q)] + ne_loc[size_t(q)]);
+        g_ptr.push_back(int32_t(g_ne.size()));
+      }
+    }
+    c->n_gen = int32_t(g_node.size());
+    if (c->n_gen) {
+      UFC_REQUIRE(int64_t(ne) * 8 < INT32_MAX, UFC_STATUS_UNSUPPORTED,
+                  "general nodes need 8 n_elems < 2^31");
+      c->gen_flag.resize(size_t(nblk));
+      c->gen_node.resize(g_node.size());
+      c->gen_ne_ptr.resize(g_ptr.size());
+      c->gen_ne.resize(g_ne.size());
+      up(c->gen_flag.get(), gflag.data(), c->gen_flag.bytes());
+      up(c->gen_node.get(), g_node.data(), c->gen_node.bytes());
+      up(c->gen_ne_ptr.get(), g_ptr.data(), c->gen_ne_ptr.bytes());
+      up(c->gen_ne.get(), g_ne.data(), c->gen_ne.bytes());
+      UFC_HIP_CHECK(hipStreamSynchronize(c->stream));
+    } else {
+      c->gen_flag.release();
+      c->gen_node.release();
+      c->gen_ne_ptr.release();
+      c->gen_ne.release();
+    }
+    // the symmetric view needs a structured box, whose nodes all fit the slot
+    // map (the FP32 assembly writes its diagonals)
+    if (c->n_gen == 0)
+      build_sym_diagonals(c, nb_idx, s_elem);
+    else
+      reset_sym_diagonals(c);
     c->fp32_view_request = -1;
     ufc::select_fp32_view(c);
     if (csr32) up(c->row_ptr.get(), row_ptr.data(), c->row_ptr.bytes());

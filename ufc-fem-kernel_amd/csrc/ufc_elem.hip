@@ -507,14 +507,49 @@ struct DiagWrite {
   float* dc;
 };
 
+// The 8 row blocks K_e(a, b) of element elem for its local node a: elastic
+// (variant VAR, E_e = young[elem]) or the J2 tangent (MAT = UFC_MAT_J2, the
+// Gauss-point tangents dgp).
+template <typename T, int MAT, int VAR>
+__device__ __forceinline__ void elem_node_rows(const double* __restrict__ coords,
+                                               const int32_t* __restrict__ conn, int32_t elem,
+                                               int a, const double* __restrict__ young,
+                                               double nu, const double* __restrict__ dgp,
+                                               const double* __restrict__ inc_c,
+                                               T (&kb)[8][3][3]) {
+  T x[8][3];
+  gather_coords(coords, conn, elem, x);
+  if constexpr (MAT == UFC_MAT_ELASTIC) {
+    const double E = young[elem];
+    const T lam = T(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    const T mu = T(E / (2.0 * (1.0 + nu)));
+    elem_rows<T, VAR>(x, a, lam, mu,
+                      VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * elem : nullptr, kb);
+  } else {
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      T g[8][3], dj;
+      if (!c3d8_dndx(x, gp, g, dj)) continue;
+      T ga[3];
+      UFC_SELECT_ROW(g, a, ga);
+      T bd[3][6];
+      bt_d(ga, dgp + 36 * (8 * int64_t(elem) + gp), bd);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) btdb_general(bd, g[b], dj, kb[b]);
+    }
+  }
+}
+
+// gen_flag != null: blocks of general nodes (k_assemble_gen) are skipped
 template <typename T, int MAT, int VAR>
 __global__ __launch_bounds__(64) void k_assemble(
     int32_t n_nodes, const double* __restrict__ coords, const int32_t* __restrict__ conn,
     const int32_t* __restrict__ slot_elem, const uint64_t* __restrict__ slot_code,
     const int32_t* __restrict__ nb_ptr, const double* __restrict__ young, double nu,
     const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val,
-    DiagWrite dw) {
+    DiagWrite dw, const uint8_t* __restrict__ gen_flag) {
   __shared__ T acc[8 * 9 * kMaxNodeNbrs];
+  if (gen_flag && gen_flag[blockIdx.x]) return;
   const int lane = threadIdx.x;
   const int nl = lane >> 3, s = lane & 7;
   const int64_t node0 = int64_t(blockIdx.x) * 8;
@@ -541,30 +576,9 @@ __global__ __launch_bounds__(64) void k_assemble(
 #pragma unroll
       for (int j = 0; j < 3; ++j) kb[b][i][j] = T(0);
 
-  if (elem >= 0) {
-    const int a = int((code >> 40) & 7);
-    T x[8][3];
-    gather_coords(coords, conn, elem, x);
-    if constexpr (MAT == UFC_MAT_ELASTIC) {
-      const double E = young[elem];
-      const T lam = T(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
-      const T mu = T(E / (2.0 * (1.0 + nu)));
-      elem_rows<T, VAR>(x, a, lam, mu,
-                        VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * elem : nullptr, kb);
-    } else {
-#pragma unroll
-      for (int gp = 0; gp < 8; ++gp) {
-        T g[8][3], dj;
-        if (!c3d8_dndx(x, gp, g, dj)) continue;
-        T ga[3];
-        UFC_SELECT_ROW(g, a, ga);
-        T bd[3][6];
-        bt_d(ga, dgp + 36 * (8 * int64_t(elem) + gp), bd);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) btdb_general(bd, g[b], dj, kb[b]);
-      }
-    }
-  }
+  if (elem >= 0)
+    elem_node_rows<T, MAT, VAR>(coords, conn, elem, int((code >> 40) & 7), young, nu, dgp, inc_c,
+                                kb);
   __syncthreads();
   const int d3 = 3 * deg;
   for (int t = 0; t < 8; ++t) {
@@ -620,6 +634,79 @@ __global__ __launch_bounds__(64) void k_assemble(
       }
     }
   }
+}
+
+// position of node m in the sorted list L[0 .. deg) (m is in it)
+__device__ __forceinline__ int list_pos(const int32_t* __restrict__ L, int deg, int32_t m) {
+  int lo = 0, hi = deg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (L[mid] < m)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// The node rows of the general nodes (outside the slot map): one workgroup
+// per node gen_node[blockIdx.x]; lane i takes the node's element i of a
+// chunk of 64 (gen_ne = 8 e + a, elements ascending), computes its 8 row
+// blocks and finds the columns of the element's nodes in the node's
+// neighbour list; the lanes then add their blocks to the LDS row segment one
+// after another, in element order (the order of the slot-map kernel), and the
+// segment is written once.
+template <typename T, int MAT, int VAR>
+__global__ __launch_bounds__(64) void k_assemble_gen(
+    const int32_t* __restrict__ gen_node, const int32_t* __restrict__ gen_ne_ptr,
+    const int32_t* __restrict__ gen_ne, const double* __restrict__ coords,
+    const int32_t* __restrict__ conn, const int32_t* __restrict__ nb_ptr,
+    const int32_t* __restrict__ nb_idx, const double* __restrict__ young, double nu,
+    const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val) {
+  __shared__ T acc[9 * kMaxGenNbrs];
+  const int lane = threadIdx.x;
+  const int64_t n = gen_node[blockIdx.x];
+  const int64_t q0 = nb_ptr[n];
+  const int deg = int(nb_ptr[n + 1] - q0);
+  const int total = 9 * deg, d3 = 3 * deg;
+  for (int i = lane; i < total; i += 64) acc[i] = T(0);
+  const int32_t p0 = gen_ne_ptr[blockIdx.x];
+  const int ne = gen_ne_ptr[blockIdx.x + 1] - p0;
+  __syncthreads();
+  for (int c0 = 0; c0 < ne; c0 += 64) {
+    const int i = c0 + lane;
+    T kb[8][3][3];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) kb[b][r][j] = T(0);
+    int pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < ne) {
+      const int32_t pk = gen_ne[p0 + i];
+      const int32_t elem = pk >> 3;
+      elem_node_rows<T, MAT, VAR>(coords, conn, elem, pk & 7, young, nu, dgp, inc_c, kb);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) pos[b] = list_pos(nb_idx + q0, deg, conn[8 * int64_t(elem) + b]);
+    }
+    const int cnt = ne - c0 < 64 ? ne - c0 : 64;
+    for (int t = 0; t < cnt; ++t) {
+      if (lane == t) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          T* dst = acc + 3 * pos[b];
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dst[r * d3 + j] += kb[b][r][j];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  T* out = val + 9 * q0;
+  for (int i = lane; i < total; i += 64) out[i] = acc[i];
 }
 
 template <typename T>
@@ -867,15 +954,17 @@ __global__ __launch_bounds__(128) void k_elem_force(int32_t n_elems,
 // slot_loc word), so the slot reads of a wave are one contiguous 256-byte run;
 // the eight contributions are added by a fixed xor tree ((s0+s1)+(s2+s3)) +
 // ((s4+s5)+(s6+s7)) and lanes s = 0..2 write component s.
+// gen_flag != null: the nodes of general blocks are left to k_node_gather_gen.
 __global__ __launch_bounds__(256) void k_node_gather(int32_t n_nodes,
                                                      const int32_t* __restrict__ slot_elem,
                                                      const uint32_t* __restrict__ slot_loc,
+                                                     const uint8_t* __restrict__ gen_flag,
                                                      const double* __restrict__ w,
                                                      double* __restrict__ y) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t n = t >> 3;
   const int s = int(t & 7);
-  const bool live = n < n_nodes;
+  const bool live = n < n_nodes && !(gen_flag && gen_flag[n >> 3]);
   const int64_t nn = live ? n : 0;
   const int32_t e = slot_elem[8 * nn + s];
   const int a = int((slot_loc[nn] >> (4 * s)) & 7);
@@ -890,6 +979,28 @@ __global__ __launch_bounds__(256) void k_node_gather(int32_t n_nodes,
 #pragma unroll
     for (int d = 0; d < 3; ++d) f[d] += __shfl_xor(f[d], o, 64);
   if (live && s < 3) y[3 * n + s] = (s == 0) ? f[0] : ((s == 1) ? f[1] : f[2]);
+}
+
+// y[3n+d] for the general nodes: one thread per node over its elements in
+// ascending order
+__global__ __launch_bounds__(256) void k_node_gather_gen(int32_t n_gen,
+                                                         const int32_t* __restrict__ gen_node,
+                                                         const int32_t* __restrict__ gen_ne_ptr,
+                                                         const int32_t* __restrict__ gen_ne,
+                                                         const double* __restrict__ w,
+                                                         double* __restrict__ y) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_gen) return;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int32_t q = gen_ne_ptr[i]; q < gen_ne_ptr[i + 1]; ++q) {
+    const int32_t pk = gen_ne[q];
+    const double* p = w + 24 * int64_t(pk >> 3) + 3 * (pk & 7);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] += p[d];
+  }
+  const int64_t n = gen_node[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) y[3 * n + d] = f[d];
 }
 
 // ---------------------------------------------------------------- variant apply
@@ -1171,12 +1282,24 @@ void eval_ke_var(int32_t n_elems, const double* coords, const double* young, dou
                        young, nu, inc_c, static_cast<double*>(ke));
 }
 
+// the general nodes' rows, after the slot-map kernel
+template <typename T, int MAT, int VAR>
+void assemble_general(ufc_ctx* c, const double* dgp, T* val) {
+  if (c->n_gen == 0) return;
+  hipLaunchKernelGGL((k_assemble_gen<T, MAT, VAR>), dim3(unsigned(c->n_gen)), dim3(64), 0,
+                     c->stream, c->gen_node.get(), c->gen_ne_ptr.get(), c->gen_ne.get(),
+                     c->coords.get(), c->conn.get(), c->nb_ptr.get(), c->nb_idx.get(),
+                     MAT == UFC_MAT_ELASTIC ? c->young.get() : nullptr, c->nu, dgp,
+                     c->inc_c.get(), val);
+}
+
 template <typename T, int VAR>
 void assemble_elastic(ufc_ctx* c, T* val, const DiagWrite& dw) {
   hipLaunchKernelGGL((k_assemble<T, UFC_MAT_ELASTIC, VAR>), dim3(blocks_for(c->n_nodes, 8)),
                      dim3(64), 0, c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
                      c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), c->young.get(),
-                     c->nu, nullptr, c->inc_c.get(), val, dw);
+                     c->nu, nullptr, c->inc_c.get(), val, dw, c->gen_flag.get());
+  assemble_general<T, UFC_MAT_ELASTIC, VAR>(c, nullptr, val);
 }
 
 template <typename T>
@@ -1301,21 +1424,25 @@ void launch_assemble(ufc_ctx* c, int precision, int material) {
   const DiagWrite dw = diag_write(c, precision != UFC_PREC_FP64);
   if (precision != UFC_PREC_FP64) c->diag_from_assembly = dw.nd > 0;
   if (precision == UFC_PREC_FP64) {
-    if (material == UFC_MAT_ELASTIC)
+    if (material == UFC_MAT_ELASTIC) {
       assemble_variant<double>(c, c->val64.get(), dw);
-    else
+    } else {
       hipLaunchKernelGGL((k_assemble<double, UFC_MAT_J2, kVarFull>), dim3(grid), dim3(64), 0,
                          c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
                          c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), nullptr, c->nu,
-                         dgp, nullptr, c->val64.get(), dw);
+                         dgp, nullptr, c->val64.get(), dw, c->gen_flag.get());
+      assemble_general<double, UFC_MAT_J2, kVarFull>(c, dgp, c->val64.get());
+    }
   } else {
-    if (material == UFC_MAT_ELASTIC)
+    if (material == UFC_MAT_ELASTIC) {
       assemble_variant<float>(c, c->val32.get(), dw);
-    else
+    } else {
       hipLaunchKernelGGL((k_assemble<float, UFC_MAT_J2, kVarFull>), dim3(grid), dim3(64), 0,
                          c->stream, c->n_nodes, c->coords.get(), c->conn.get(),
                          c->slot_elem.get(), c->slot_code.get(), c->nb_ptr.get(), nullptr, c->nu,
-                         dgp, nullptr, c->val32.get(), dw);
+                         dgp, nullptr, c->val32.get(), dw, c->gen_flag.get());
+      assemble_general<float, UFC_MAT_J2, kVarFull>(c, dgp, c->val32.get());
+    }
   }
   UFC_HIP_CHECK(hipGetLastError());
 }
@@ -1427,7 +1554,11 @@ void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y) {
   UFC_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_node_gather, dim3(blocks_for(int64_t(c->n_nodes) * 8, 256)), dim3(256), 0,
                      c->stream, c->n_nodes, c->slot_elem.get(), c->slot_loc.get(),
-                     c->elem_w.get(), y);
+                     c->gen_flag.get(), c->elem_w.get(), y);
+  if (c->n_gen)
+    hipLaunchKernelGGL(k_node_gather_gen, dim3(blocks_for(c->n_gen, 256)), dim3(256), 0,
+                       c->stream, c->n_gen, c->gen_node.get(), c->gen_ne_ptr.get(),
+                       c->gen_ne.get(), c->elem_w.get(), y);
   UFC_HIP_CHECK(hipGetLastError());
 }
 

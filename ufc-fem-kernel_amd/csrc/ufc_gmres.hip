@@ -59,18 +59,20 @@ __global__ __launch_bounds__(kTThreads) void k_spmv_t(int32_t n_nodes,
     const int32_t q0 = nb_ptr[n];
     const int deg = nb_ptr[n + 1] - q0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    if (k < deg) {
-      const int32_t m = nb_idx[q0 + k];
-      const int32_t p = mirror[q0 + k];
+    // lane k on the neighbours k, k + 32, ... (one pass unless the list is
+    // longer than the group)
+    for (int kk = k; kk < deg; kk += kGroup) {
+      const int32_t m = nb_idx[q0 + kk];
+      const int32_t p = mirror[q0 + kk];
       if (p >= 0) {
         const int32_t mq = nb_ptr[m];
         const int64_t d3 = 3 * int64_t(nb_ptr[m + 1] - mq);
         const double* blk = val + 9 * int64_t(mq) + 3 * int64_t(p);
         const double x0 = x[3 * int64_t(m)], x1 = x[3 * int64_t(m) + 1], x2 = x[3 * int64_t(m) + 2];
         // row i of block (m, n) times x_m[i], accumulated into column j
-        a0 = blk[0] * x0 + blk[d3] * x1 + blk[2 * d3] * x2;
-        a1 = blk[1] * x0 + blk[d3 + 1] * x1 + blk[2 * d3 + 1] * x2;
-        a2 = blk[2] * x0 + blk[d3 + 2] * x1 + blk[2 * d3 + 2] * x2;
+        a0 += blk[0] * x0 + blk[d3] * x1 + blk[2 * d3] * x2;
+        a1 += blk[1] * x0 + blk[d3 + 1] * x1 + blk[2 * d3 + 1] * x2;
+        a2 += blk[2] * x0 + blk[d3 + 2] * x1 + blk[2 * d3 + 2] * x2;
       }
     }
 #pragma unroll

@@ -85,8 +85,11 @@ constexpr int kSpmvBlocks64 = 32768;  // FP64 CSR SpMV grid (measured best on gf
 constexpr int kSpmvBlocks32 = 32768;  // FP32 SpMV grid (measured best on gfx950: 0.415 ms at 16384, 0.405 at 32768, 0.415 at 65536 for 128^3)
 constexpr int kNbPtrPad = 3;  // device nb_ptr entries past n_nodes, equal to nb_ptr[n_nodes]
 constexpr int kMaxPartials = kSolveBlocks;  // per-workgroup partial sums (vector kernels)
-constexpr int kMaxNodeElems = 8;    // elements per node (slot map width)
-constexpr int kMaxNodeNbrs = 32;    // neighbours per node (5-bit slot codes)
+constexpr int kMaxNodeElems = 8;    // elements per node of the slot map
+constexpr int kMaxNodeNbrs = 32;    // neighbours per node of the slot map (5-bit codes) and the 32-lane views
+// nodes beyond the slot map (general nodes): neighbours per node (the LDS
+// row segment of the general assembly); their element count is unbounded
+constexpr int kMaxGenNbrs = 256;
 constexpr int kMaxPatterns = 256;   // stencil patterns of the FP32 view (one-byte ids)
 constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP32 view
 constexpr int kDiagThreads = 256;   // symmetric FP32 SpMV: one node per thread
@@ -237,6 +240,17 @@ struct ufc_ctx {
   // n's local index in the element of slot s, bits 4s..4s+2 (slot_code bits
   // 40..42 of all eight slots in one word, for the node gather)
   ufc::DBuf<uint32_t> slot_loc;   // (n_nodes)
+  // General nodes: a node in more than kMaxNodeElems elements or with more
+  // than kMaxNodeNbrs neighbours does not fit the slot map; every node of its
+  // 8-node assembly block (nodes 8b .. 8b + 7) is then assembled and gathered
+  // by the general kernels (one workgroup per node over its element list),
+  // and the slot-map kernels skip the block.
+  int32_t n_gen = 0;               // general nodes (whole blocks)
+  ufc::DBuf<uint8_t> gen_flag;     // (ceil(n_nodes / 8)) 1: block of general nodes; empty if none
+  ufc::DBuf<int32_t> gen_node;     // (n_gen) ascending
+  ufc::DBuf<int32_t> gen_ne_ptr;   // (n_gen + 1)
+  ufc::DBuf<int32_t> gen_ne;       // 8 e + a per element e of the node (local index a), e ascending
+  bool wide = false;               // some node has more than kMaxNodeNbrs neighbours
 
   // values
   ufc::DBuf<double> val64;

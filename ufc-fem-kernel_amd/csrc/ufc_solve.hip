@@ -58,6 +58,9 @@ constexpr int GROUP = 32;  // lanes per node
 constexpr int kViewId32 = 0;     // 4-byte neighbour ids
 constexpr int kViewOff16 = 1;    // 16-bit offsets m - n per neighbour
 constexpr int kViewPattern = 2;  // one stencil-pattern byte per node
+// 4-byte ids, neighbour lists longer than a 32-lane group (FP32 view 0 and
+// the FP64 CSR on meshes with nodes of more than kMaxNodeNbrs neighbours)
+constexpr int kViewWide = 3;
 constexpr int WAVES_PER_BLOCK = kSolveThreads / WAVE;
 constexpr int GROUPS_PER_BLOCK = kSpmvThreads / GROUP;
 
@@ -162,8 +165,9 @@ __device__ __forceinline__ void sum_partials(double (&out)[NV], const double* pa
 
 // ---------------------------------------------------------------- SpMV body
 // Plain CSR rows of node n in a[0..2] (valid in every lane of the 32-lane
-// group): every value's column comes from col_idx.
-template <typename T>
+// group): every value's column comes from col_idx.  WIDE: lane k also takes
+// the neighbours k + 32, k + 64, ... (lists longer than the group).
+template <typename T, bool WIDE = false>
 __device__ __forceinline__ void node_rows_csr(int64_t n, int k, const int32_t* __restrict__ nb_ptr,
                                               const int32_t* __restrict__ col_idx,
                                               const T* __restrict__ val, const T* __restrict__ x,
@@ -171,15 +175,16 @@ __device__ __forceinline__ void node_rows_csr(int64_t n, int k, const int32_t* _
   const int32_t q0 = nb_ptr[n];
   const int deg = nb_ptr[n + 1] - q0;
   a[0] = a[1] = a[2] = T(0);
-  if (k < deg) {
-    const int64_t base = 9 * int64_t(q0) + 3 * k;
+  for (int kk = k; kk < deg; kk += GROUP) {
+    const int64_t base = 9 * int64_t(q0) + 3 * kk;
     const int d3 = 3 * deg;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T* v = val + base + d * d3;
       const int32_t* c = col_idx + base + d * d3;
-      a[d] = v[0] * x[c[0]] + v[1] * x[c[1]] + v[2] * x[c[2]];
+      a[d] += v[0] * x[c[0]] + v[1] * x[c[1]] + v[2] * x[c[2]];
     }
+    if constexpr (!WIDE) break;
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) a[d] = sum_group32(a[d]);
@@ -270,6 +275,50 @@ __device__ __forceinline__ void node_pair_rows32(int64_t n0, int32_t n_nodes, in
                                                      false, false);
     a[d] = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
   }
+}
+
+// FP32 node-block view with 4-byte ids for neighbour lists of any length:
+// the node pair n0, n0 + 1 of node_pair_rows32, lane k of a group on the
+// neighbours k, k + 32, ...; x_n is loaded directly, the self block (the
+// penalty diagonal in difference form) multiplies x_n itself, and the group
+// sums its row partials with xor shuffles inside the 32 lanes.
+__device__ __forceinline__ void node_pair_rows32_wide(int64_t n0, int32_t n_nodes, int lane,
+                                                      const int32_t* __restrict__ nb_ptr,
+                                                      const int32_t* __restrict__ nb_idx,
+                                                      const float* __restrict__ val,
+                                                      const float* __restrict__ x, float (&a)[3],
+                                                      float& xk) {
+  const bool hi = lane >= GROUP;
+  const int k = lane & (GROUP - 1);
+  const int64_t n = n0 + (hi ? 1 : 0);
+  float xn[3] = {0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f};
+  if (n < n_nodes) {
+    const int32_t q0 = nb_ptr[n];
+    const int deg = nb_ptr[n + 1] - q0;
+    const int d3 = 3 * deg;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xn[c] = x[3 * n + c];
+    for (int kk = k; kk < deg; kk += GROUP) {
+      const int64_t m = nb_idx[q0 + kk];
+      const int64_t base = 9 * int64_t(q0) + 3 * kk;
+      const float s = m == n ? 0.f : 1.f;
+      float xd[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xd[c] = x[3 * m + c] - s * xn[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float* v = val + base + d * d3;
+        t[d] += v[0] * xd[0] + v[1] * xd[1] + v[2] * xd[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int o = GROUP / 2; o > 0; o >>= 1) t[d] += __shfl_xor(t[d], o, 64);
+    a[d] = t[d];
+  }
+  xk = (k == 0) ? xn[0] : ((k == 1) ? xn[1] : xn[2]);
 }
 
 // Stencil-pattern view over the off-diagonal values (val32m): the same pair
@@ -427,6 +476,8 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
         if constexpr (VIEW == kViewPattern)
           node_pair_rows32_offdiag(n0, n_nodes, lane, nb_ptr, nb_pat, pat_off, val, pen,
                                    pen_flag, x, a, xk);
+        else if constexpr (VIEW == kViewWide)
+          node_pair_rows32_wide(n0, n_nodes, lane, nb_ptr, nb_idx, val, x, a, xk);
         else
           node_pair_rows32<VIEW>(n0, n_nodes, lane, nb_ptr, nb_idx, nb_off, val, x, a, xk);
         const int64_t n = n0 + (lane >= GROUP ? 1 : 0);
@@ -461,7 +512,7 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
         const int64_t n = b + gi + h * GROUPS_PER_BLOCK;
         if (n >= n_nodes) break;
         T a[3];
-        node_rows_csr<T>(n, k, nb_ptr, col_idx, val, x, a);
+        node_rows_csr<T, VIEW == kViewWide>(n, k, nb_ptr, col_idx, val, x, a);
         if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
       }
     }
@@ -1164,11 +1215,21 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
                          c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
                          pap);
+    else if (c->wide)
+      hipLaunchKernelGGL((k_spmv<T, true, kViewWide>), grid, block, 0, c->stream, c->n_nodes,
+                         c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
+                         c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
+                         pap);
     else
       hipLaunchKernelGGL((k_spmv<T, true, kViewId32>), grid, block, 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
                          c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
                          pap);
+  } else if (c->wide) {
+    hipLaunchKernelGGL((k_spmv<T, false, kViewWide>), grid, block, 0, c->stream, c->n_nodes,
+                       c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
+                       c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
+                       nullptr);
   } else {
     hipLaunchKernelGGL((k_spmv<T, false, kViewId32>), grid, block, 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),

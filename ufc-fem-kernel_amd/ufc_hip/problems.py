@@ -110,6 +110,50 @@ def j2_cube(n=64, seed=SEED, strain=0.005, increments=10):
     return pr
 
 
+def polar_prism(sectors=5, rings=3, nz=4, radius=1.0, height=1.0, nu=0.3, p=1.0e6, seed=SEED,
+                name=None) -> Problem:
+    """An unstructured C3D8 mesh: a polar disk of quadrilaterals around a
+    centre node, extruded in z.  `sectors` kite quads meet at the centre
+    (c, p_i, q_i, p_{i+1}: p at radius dr, q between them at 1.15 dr), and
+    rings - 1 rings of 2 sectors quads surround them, so the axis nodes of
+    the interior layers lie in 2 sectors elements (10 for 5 sectors) with
+    3 (2 sectors + 1) neighbours (33), beyond an 8-element / 32-neighbour
+    node map.  z=0 clamped, pressure p on z=height, E_e lognormal(ln 1e9, 0.5)."""
+    ns, R = sectors, rings
+    dr = radius / R
+    pts = [(0.0, 0.0)]
+    ring = []  # ring[j][k]: 2 ns points at angle pi k / ns
+    for j in range(1, R + 1):
+        ids = []
+        for k in range(2 * ns):
+            r = j * dr * (1.15 if (j == 1 and k % 2) else 1.0)
+            t = math.pi * k / ns
+            ids.append(len(pts))
+            pts.append((r * math.cos(t), r * math.sin(t)))
+        ring.append(ids)
+    quads = []
+    r1 = ring[0]
+    for i in range(ns):  # counter-clockwise kites around the centre
+        quads.append((0, r1[2 * i], r1[2 * i + 1], r1[(2 * i + 2) % (2 * ns)]))
+    for j in range(R - 1):
+        a, b = ring[j], ring[j + 1]
+        for k in range(2 * ns):
+            k1 = (k + 1) % (2 * ns)
+            quads.append((a[k], b[k], b[k1], a[k1]))
+    n2 = len(pts)
+    xy = np.array(pts)
+    coords = np.concatenate([np.column_stack([xy, np.full(n2, height * k / nz)])
+                             for k in range(nz + 1)])
+    q = np.array(quads, np.int32)
+    conn = np.concatenate([np.concatenate([q + k * n2, q + (k + 1) * n2], axis=1)
+                           for k in range(nz)]).astype(np.int32)
+    clamp = _clamp_eqs(np.arange(n2, dtype=np.int32))
+    top = np.arange((nz - 1) * len(quads), nz * len(quads), dtype=np.int32)
+    young = S.field("lognormal", conn.shape[0], seed, math.log(1.0e9), 0.5)
+    return Problem(name or f"polar{ns}x{R}x{nz}", ns, R, nz, coords, conn, young, nu, clamp,
+                   np.zeros(clamp.size), top, np.full(top.size, 1, np.int32), "pressure", (p,))
+
+
 def setup_context(ctx: "S.Context", pr: Problem, precision, penalty=PENALTY):
     """Pattern, loads and BCs of `pr` on `ctx` (assembly is separate)."""
     ctx.build_pattern(pr.coords, pr.conn)
