@@ -265,9 +265,10 @@ int ufc_ph_j2_set_state(ufc_ctx* ctx, const double* state, ufc_status* st);
  * (RT_CSR_FromTripletMerged, RT_Solv_Sparse.f90:336-447) directly from node
  * adjacency, plus the precomputed element-to-slot map that lets assembly
  * write every CSR value once without atomics.  Replaces the full-build path of
- * RT_Asm_GlobalStiffness (RT_Asm_Solv.f90:1902-1965).  Nodes may belong to at
- * most 8 elements and have at most 32 neighbours (UFC_STATUS_UNSUPPORTED
- * otherwise). */
+ * RT_Asm_GlobalStiffness (RT_Asm_Solv.f90:1902-1965).  Any hex mesh whose
+ * nodes have at most 256 neighbours (UFC_STATUS_UNSUPPORTED otherwise); a
+ * node may belong to any number of elements (nodes beyond 8 elements or 32
+ * neighbours are assembled by general kernels, one node per workgroup). */
 int ufc_rt_asm_build_pattern(ufc_ctx* ctx, const ufc_mesh_desc* mesh, ufc_rt_asm_info* info,
                              ufc_status* st);
 
