@@ -165,6 +165,172 @@ __device__ __forceinline__ void add_iso_blocks(const T (&g)[8][3], const T (&ga)
   }
 }
 
+// The trilinear map x(xi, eta, zeta) = sum_a N_a x_a in monomial form: its
+// derivatives are dx/dxi = e1 + e12 eta + e13 zeta + e123 eta zeta and the
+// two cyclic analogues, with e_S = (1/8) sum_a (prod_{s in S} s_a) x_a over
+// the node signs s_a = (XS, YS, ZS)[a].  21 values per element replace the
+// 8 x 3 coordinates in the Jacobian at each point.
+template <typename T>
+struct TriMap {
+  T e1[3], e2[3], e3[3], e12[3], e13[3], e23[3], e123[3];
+};
+
+template <typename T>
+__device__ __forceinline__ TriMap<T> tri_map(const T (&x)[8][3]) {
+  TriMap<T> m;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    T s1 = T(0), s2 = T(0), s3 = T(0), s12 = T(0), s13 = T(0), s23 = T(0), s123 = T(0);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const T v = x[a][c];
+      s1 += T(XS[a]) * v;
+      s2 += T(YS[a]) * v;
+      s3 += T(ZS[a]) * v;
+      s12 += T(XS[a] * YS[a]) * v;
+      s13 += T(XS[a] * ZS[a]) * v;
+      s23 += T(YS[a] * ZS[a]) * v;
+      s123 += T(XS[a] * YS[a] * ZS[a]) * v;
+    }
+    m.e1[c] = T(0.125) * s1;
+    m.e2[c] = T(0.125) * s2;
+    m.e3[c] = T(0.125) * s3;
+    m.e12[c] = T(0.125) * s12;
+    m.e13[c] = T(0.125) * s13;
+    m.e23[c] = T(0.125) * s23;
+    m.e123[c] = T(0.125) * s123;
+  }
+  return m;
+}
+
+// dN_b/dxi at (xi, eta, zeta) for node signs (sx, sy, sz)
+template <typename T>
+__device__ __forceinline__ void node_dn(T sx, T sy, T sz, T xi, T eta, T zeta, T (&dn)[3]) {
+  const T fx = T(1) + sx * xi, fy = T(1) + sy * eta, fz = T(1) + sz * zeta;
+  dn[0] = T(0.125) * sx * fy * fz;
+  dn[1] = T(0.125) * fx * sy * fz;
+  dn[2] = T(0.125) * fx * fy * sz;
+}
+
+// At one point: the Jacobian J[j][k] = dx_j/dxi_k, d = det J and its
+// adjugate A (J^-1 = A / d, so dN_b/dx = A^T dn_b / d).  False when
+// |d| <= 1e-12 (the point is skipped, PH_Elem_C3D8.f90:775).
+template <typename T>
+__device__ __forceinline__ bool tri_point(const TriMap<T>& m, T xi, T eta, T zeta, T (&A)[3][3],
+                                          T& d) {
+  T J[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    J[j][0] = m.e1[j] + m.e12[j] * eta + m.e13[j] * zeta + m.e123[j] * (eta * zeta);
+    J[j][1] = m.e2[j] + m.e12[j] * xi + m.e23[j] * zeta + m.e123[j] * (xi * zeta);
+    J[j][2] = m.e3[j] + m.e13[j] * xi + m.e23[j] * eta + m.e123[j] * (xi * eta);
+  }
+  A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  A[1][0] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+  A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  d = J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+  if (tabs(d) <= T(1.0e-12)) return false;
+  A[0][1] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+  A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  A[1][2] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+  A[2][1] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+  A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  return true;
+}
+
+// h = A^T dn (= d dN/dx)
+template <typename T>
+__device__ __forceinline__ void adj_t(const T (&A)[3][3], const T (&dn)[3], T (&h)[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) h[j] = A[0][j] * dn[0] + A[1][j] * dn[1] + A[2][j] * dn[2];
+}
+
+// One integration point of weight w for the isotropic blocks: with
+// h_b = d dN_b/dx, w detJ g_a g_b^T = (w / d) h_a h_b^T, so
+// P_b += (w / d) h_a h_b^T needs one division per point and no table of
+// gradients; sg_b += w h_b (= w detJ g_b) when SG.  Node a's signs are
+// (sx, sy, sz).  Returns w d (the weighted volume), 0 for a skipped point.
+template <typename T, bool SG>
+__device__ __forceinline__ T iso_point(const TriMap<T>& m, T xi, T eta, T zeta, T w, T sx, T sy,
+                                       T sz, T (&P)[8][3][3], T (&sg)[8][3]) {
+  T A[3][3], d;
+  if (!tri_point(m, xi, eta, zeta, A, d)) return T(0);
+  T dna[3], ha[3];
+  node_dn(sx, sy, sz, xi, eta, zeta, dna);
+  adj_t(A, dna, ha);
+  const T r = w / d;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ha[i] *= r;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    T dnb[3], hb[3];
+    node_dn(T(XS[b]), T(YS[b]), T(ZS[b]), xi, eta, zeta, dnb);
+    adj_t(A, dnb, hb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) P[b][i][j] += ha[i] * hb[j];
+    if constexpr (SG) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sg[b][j] += w * hb[j];
+    }
+  }
+  return w * d;
+}
+
+// iso_point over the 2x2x2 Gauss points of PH_Elem_C3D8_GaussPoints (xi
+// outer, zeta inner); returns the volume.  The FP64 loop stays rolled:
+// unrolled, the overlap of consecutive points takes it past 256 registers
+// (one wave per SIMD), rolled it fits in 254 (two); FP32 fits either way and
+// keeps the folded constants.
+template <typename T, bool SG>
+__device__ __forceinline__ T iso_gauss8(const TriMap<T>& m, T sx, T sy, T sz, T (&P)[8][3][3],
+                                        T (&sg)[8][3]) {
+  T V = T(0);
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      const T xi = (gp & 4) ? T(kGaussPt) : T(-kGaussPt);
+      const T eta = (gp & 2) ? T(kGaussPt) : T(-kGaussPt);
+      const T zeta = (gp & 1) ? T(kGaussPt) : T(-kGaussPt);
+      V += iso_point<T, SG>(m, xi, eta, zeta, T(1), sx, sy, sz, P, sg);
+    }
+  } else {
+#pragma unroll 1
+    for (int gp = 0; gp < 8; ++gp) {
+      const T xi = (gp & 4) ? T(kGaussPt) : T(-kGaussPt);
+      const T eta = (gp & 2) ? T(kGaussPt) : T(-kGaussPt);
+      const T zeta = (gp & 1) ? T(kGaussPt) : T(-kGaussPt);
+      V += iso_point<T, SG>(m, xi, eta, zeta, T(1), sx, sy, sz, P, sg);
+    }
+  }
+  return V;
+}
+
+// kb[b] += la P_b + mw P_b^T + mw tr(P_b) I: the isotropic B_a^T D B_b
+// summed over the points, P_b = sum w detJ g_a g_b^T
+template <typename T>
+__device__ __forceinline__ void iso_blocks_from_p(const T (&P)[8][3][3], T la, T mw,
+                                                  T (&kb)[8][3][3]) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const T tr = mw * (P[b][0][0] + P[b][1][1] + P[b][2][2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        kb[b][i][j] += la * P[b][i][j] + mw * P[b][j][i] + (i == j ? tr : T(0));
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void node_signs(int a, T& sx, T& sy, T& sz) {
+  sx = ((a ^ (a >> 1)) & 1) ? T(1) : T(-1);
+  sy = ((a >> 1) & 1) ? T(1) : T(-1);
+  sz = ((a >> 2) & 1) ? T(1) : T(-1);
+}
+
 // Rows of node a of the elastic K_e, kb[b] = K_ab (added to kb), for the
 // variant VAR of PH_Elem_C3D8_StiffMatrixByVariant (PH_Elem_C3D8.f90:400-417):
 //   kVarFull      2x2x2 Gauss (PH_Elem_C3D8_StiffMatrix, :759-780)
@@ -186,15 +352,13 @@ __device__ __forceinline__ void add_iso_blocks(const T (&g)[8][3], const T (&ga)
 template <typename T, int VAR>
 __device__ __forceinline__ void elem_rows(const T (&x)[8][3], int a, T lam, T mu,
                                           const double* __restrict__ cinc, T (&kb)[8][3][3]) {
+  T sx, sy, sz;
+  node_signs(a, sx, sy, sz);
   if constexpr (VAR == kVarFull || VAR == kVarIncompat) {
-#pragma unroll
-    for (int gp = 0; gp < 8; ++gp) {
-      T g[8][3], dj;
-      if (!c3d8_dndx(x, gp, g, dj)) continue;
-      T ga[3];
-      UFC_SELECT_ROW(g, a, ga);
-      add_iso_blocks(g, ga, lam * dj, mu * dj, kb);
-    }
+    const TriMap<T> m = tri_map(x);
+    T P[8][3][3] = {}, sg[8][3];
+    iso_gauss8<T, false>(m, sx, sy, sz, P, sg);
+    iso_blocks_from_p(P, lam, mu, kb);
     if constexpr (VAR == kVarIncompat) {
       T ca[3][9];
 #pragma unroll
@@ -219,20 +383,10 @@ __device__ __forceinline__ void elem_rows(const T (&x)[8][3], int a, T lam, T mu
     }
   } else if constexpr (VAR == kVarSelective) {
     const T kap = lam + T(2) * mu / T(3);
-    T sg[8][3] = {}, V = T(0);
-#pragma unroll
-    for (int gp = 0; gp < 8; ++gp) {
-      T g[8][3], dj;
-      if (!c3d8_dndx(x, gp, g, dj)) continue;
-      T ga[3];
-      UFC_SELECT_ROW(g, a, ga);
-      add_iso_blocks(g, ga, (lam - kap) * dj, mu * dj, kb);
-      V += dj;
-#pragma unroll
-      for (int b = 0; b < 8; ++b)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) sg[b][j] += dj * g[b][j];
-    }
+    const TriMap<T> m = tri_map(x);
+    T P[8][3][3] = {}, sg[8][3] = {};
+    const T V = iso_gauss8<T, true>(m, sx, sy, sz, P, sg);
+    iso_blocks_from_p(P, lam - kap, mu, kb);
     if (V <= T(1.0e-20)) {  // the reference returns Ke = 0
 #pragma unroll
       for (int b = 0; b < 8; ++b)
@@ -252,6 +406,8 @@ __device__ __forceinline__ void elem_rows(const T (&x)[8][3], int a, T lam, T mu
 #pragma unroll
         for (int j = 0; j < 3; ++j) kb[b][i][j] += f * sa[i] * sg[b][j];
   } else if constexpr (VAR == kVarGauss27) {
+    const TriMap<T> m = tri_map(x);
+    T P[8][3][3] = {}, sg[8][3];
 #pragma unroll 1
     for (int p = 0; p < 27; ++p) {
       const int i = p / 9, j = (p / 3) % 3, k = p % 3;
@@ -262,13 +418,9 @@ __device__ __forceinline__ void elem_rows(const T (&x)[8][3], int a, T lam, T mu
       const T wi = i == 1 ? T(8.0 / 9.0) : T(5.0 / 9.0);
       const T wj = j == 1 ? T(8.0 / 9.0) : T(5.0 / 9.0);
       const T wk = k == 1 ? T(8.0 / 9.0) : T(5.0 / 9.0);
-      T g[8][3], dj;
-      if (!c3d8_dndx_at(x, xi, eta, zeta, g, dj)) continue;
-      const T w = dj * (wi * wj * wk);
-      T ga[3];
-      UFC_SELECT_ROW(g, a, ga);
-      add_iso_blocks(g, ga, lam * w, mu * w, kb);
+      iso_point<T, false>(m, xi, eta, zeta, wi * wj * wk, sx, sy, sz, P, sg);
     }
+    iso_blocks_from_p(P, lam, mu, kb);
   } else if constexpr (VAR == kVarReduced) {
     T g0[8][3], dj0;
     if (c3d8_dndx_at(x, T(0), T(0), T(0), g0, dj0)) {
@@ -526,16 +678,33 @@ __device__ __forceinline__ void elem_node_rows(const double* __restrict__ coords
     elem_rows<T, VAR>(x, a, lam, mu,
                       VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * elem : nullptr, kb);
   } else {
-#pragma unroll
+    // sum_gp detJ B_a^T D_gp B_b = sum_gp (1 / d) H_a^T D_gp H_b with the
+    // B-matrices H of h = d dN/dx (one division per point, no gradient table)
+    const TriMap<T> m = tri_map(x);
+    T sx, sy, sz;
+    node_signs(a, sx, sy, sz);
+#pragma unroll 1
     for (int gp = 0; gp < 8; ++gp) {
-      T g[8][3], dj;
-      if (!c3d8_dndx(x, gp, g, dj)) continue;
-      T ga[3];
-      UFC_SELECT_ROW(g, a, ga);
-      T bd[3][6];
-      bt_d(ga, dgp + 36 * (8 * int64_t(elem) + gp), bd);
+      const T xi = (gp & 4) ? T(kGaussPt) : T(-kGaussPt);
+      const T eta = (gp & 2) ? T(kGaussPt) : T(-kGaussPt);
+      const T zeta = (gp & 1) ? T(kGaussPt) : T(-kGaussPt);
+      T A[3][3], d;
+      if (!tri_point(m, xi, eta, zeta, A, d)) continue;
+      T dna[3], ha[3];
+      node_dn(sx, sy, sz, xi, eta, zeta, dna);
+      adj_t(A, dna, ha);
+      const T r = T(1) / d;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) btdb_general(bd, g[b], dj, kb[b]);
+      for (int i = 0; i < 3; ++i) ha[i] *= r;
+      T bd[3][6];
+      bt_d(ha, dgp + 36 * (8 * int64_t(elem) + gp), bd);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        T dnb[3], hb[3];
+        node_dn(T(XS[b]), T(YS[b]), T(ZS[b]), xi, eta, zeta, dnb);
+        adj_t(A, dnb, hb);
+        btdb_general(bd, hb, T(1), kb[b]);
+      }
     }
   }
 }
