@@ -643,10 +643,15 @@ __global__ __launch_bounds__(64) void k_eval_ke(int32_t n_elems, const double* _
 // exactly once, without atomics.  The 8 nodes' segments are contiguous in the
 // values array (node n's 3 rows start at 9*nb_ptr[n]), so the write-out is a
 // single coalesced stream.
-// Block diagonals of the symmetric FP32 view written by the FP32 assembly
-// itself (nd = 0: none): for node n and diagonal s, the block K(n, m) with m
-// the node at box number perm[n] + o_s, found in n's sorted neighbour list,
-// goes to entry (s, perm[n]) (perm = null: the node numbering is the box's).
+// What the FP32 assembly writes besides the CSR values.
+//   kdiag != null: the diagonal of K (kdiag[3n + d], from the self blocks) and
+//     the self blocks zeroed in the values: the FP32 values are kept in the
+//     difference form (K x)_n = sum_{m != n} K_nm (x_m - x_n), whose self
+//     blocks hold only the penalty diagonal.
+//   nd > 0: block diagonals of the symmetric FP32 view: for node n and
+//     diagonal s, the block K(n, m) with m the node at box number
+//     perm[n] + o_s, found in n's sorted neighbour list, goes to entry
+//     (s, perm[n]) (perm = null: the node numbering is the box's).
 struct DiagWrite {
   DiagOffsets off;
   int nd;
@@ -657,6 +662,8 @@ struct DiagWrite {
   float4* da;
   float4* db;
   float* dc;
+  float* kdiag;
+  const int64_t* diag_pos;
 };
 
 // The 8 row blocks K_e(a, b) of element elem for its local node a: elastic
@@ -718,6 +725,7 @@ __global__ __launch_bounds__(64) void k_assemble(
     const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val,
     DiagWrite dw, const uint8_t* __restrict__ gen_flag) {
   __shared__ T acc[8 * 9 * kMaxNodeNbrs];
+  __shared__ int32_t nbl[8 * kMaxNodeNbrs];  // the 8 nodes' neighbour lists (diagonal write)
   if (gen_flag && gen_flag[blockIdx.x]) return;
   const int lane = threadIdx.x;
   const int nl = lane >> 3, s = lane & 7;
@@ -726,6 +734,10 @@ __global__ __launch_bounds__(64) void k_assemble(
   const int64_t q0 = nb_ptr[node0];
   const int total = int(9 * (int64_t(nb_ptr[nodeN]) - q0));
   for (int i = lane; i < total; i += 64) acc[i] = T(0);
+  if constexpr (sizeof(T) == 4) {
+    if (dw.nd > 0)
+      for (int i = lane; i < total / 9; i += 64) nbl[i] = dw.nb_idx[q0 + i];
+  }
 
   const int64_t node = node0 + nl;
   int32_t elem = -1;
@@ -764,6 +776,25 @@ __global__ __launch_bounds__(64) void k_assemble(
     }
     __syncthreads();
   }
+  if constexpr (sizeof(T) == 4) {
+    if (dw.kdiag) {
+      for (int i = lane; i < 24; i += 64) {
+        const int64_t nn0 = node0 + i / 3;
+        const int d = i % 3;
+        if (nn0 < nodeN) dw.kdiag[3 * nn0 + d] = acc[dw.diag_pos[3 * nn0 + d] - 9 * q0];
+      }
+      __syncthreads();
+      for (int i = lane; i < 72; i += 64) {
+        const int64_t nn0 = node0 + i / 9;
+        const int r = i % 9;
+        if (nn0 < nodeN) {
+          const int dg = int(nb_ptr[nn0 + 1] - nb_ptr[nn0]);
+          acc[dw.diag_pos[3 * nn0] - 9 * q0 + (r / 3) * 3 * dg + r % 3] = T(0);
+        }
+      }
+      __syncthreads();
+    }
+  }
   T* out = val + 9 * q0;
   for (int i = lane; i < total; i += 64) out[i] = acc[i];
   if constexpr (sizeof(T) == 4) {
@@ -783,7 +814,7 @@ __global__ __launch_bounds__(64) void k_assemble(
         int lo = 0, hi = dg - 1, q = -1;
         while (lo <= hi) {
           const int mid = (lo + hi) >> 1;
-          const int32_t v = dw.nb_idx[p0 + mid];
+          const int32_t v = nbl[p0 - q0 + mid];
           if (v == m) {
             q = mid;
             break;
@@ -831,7 +862,8 @@ __global__ __launch_bounds__(64) void k_assemble_gen(
     const int32_t* __restrict__ gen_ne, const double* __restrict__ coords,
     const int32_t* __restrict__ conn, const int32_t* __restrict__ nb_ptr,
     const int32_t* __restrict__ nb_idx, const double* __restrict__ young, double nu,
-    const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val) {
+    const double* __restrict__ dgp, const double* __restrict__ inc_c, T* __restrict__ val,
+    float* __restrict__ kdiag, const int64_t* __restrict__ diag_pos) {
   __shared__ T acc[9 * kMaxGenNbrs];
   const int lane = threadIdx.x;
   const int64_t n = gen_node[blockIdx.x];
@@ -874,6 +906,14 @@ __global__ __launch_bounds__(64) void k_assemble_gen(
       __syncthreads();
     }
   }
+  if constexpr (sizeof(T) == 4) {
+    if (kdiag) {  // difference form, as in k_assemble
+      if (lane < 3) kdiag[3 * n + lane] = acc[diag_pos[3 * n + lane] - 9 * q0];
+      __syncthreads();
+      if (lane < 9) acc[diag_pos[3 * n] - 9 * q0 + (lane / 3) * d3 + lane % 3] = T(0);
+      __syncthreads();
+    }
+  }
   T* out = val + 9 * q0;
   for (int i = lane; i < total; i += 64) out[i] = acc[i];
 }
@@ -883,21 +923,6 @@ __global__ void k_diag(int64_t n, const int64_t* __restrict__ pos, const T* __re
                        T* __restrict__ diag) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) diag[i] = val[pos[i]];
-}
-
-// FP32 difference form: zero the 3x3 self block of every node (the SpMV then
-// applies sum_{m != n} K_nm (x_m - x_n) plus the penalty diagonal stored in
-// the self block)
-__global__ void k_zero_self(int32_t n_nodes, const int32_t* __restrict__ nb_ptr,
-                            const int64_t* __restrict__ diag_pos, float* __restrict__ val) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n >= n_nodes) return;
-  const int d3 = 3 * (nb_ptr[n + 1] - nb_ptr[n]);
-  const int64_t base = diag_pos[3 * n];  // self block entry (0, 0)
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) val[base + d * d3 + c] = 0.f;
 }
 
 // Off-diagonal FP32 values: a 32-lane group per node, lane k on neighbour k
@@ -975,33 +1000,20 @@ __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
   }
 }
 
-// pen32 = the self-block diagonal of the difference-form values (the
-// penalty), pen_flag bit j of word w = node 32 w + j has a nonzero one: one
-// thread per node, the bits of a wave's 64 nodes formed by a ballot (waves
-// start at multiples of 64 nodes, so they fill two whole words).  iperm !=
-// null: thread n is box number n and reads node iperm[n].
-__global__ __launch_bounds__(256) void k_offdiag_penalty(int32_t n_nodes,
-                                                         const int64_t* __restrict__ diag_pos,
-                                                         const float* __restrict__ val,
-                                                         const int32_t* __restrict__ iperm,
-                                                         float* __restrict__ pen,
-                                                         uint32_t* __restrict__ flag) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool any = false;
-  if (n < n_nodes) {
-    const int64_t src = iperm ? iperm[n] : n;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float p = val[diag_pos[3 * src + d]];
-      pen[3 * n + d] = p;
-      any = any || p != 0.f;
-    }
-  }
-  const uint64_t bits = __ballot(any);
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (n - lane) / 32;  // first word of this wave
-  if (lane == 0 && n < n_nodes) flag[w] = uint32_t(bits);
-  if (lane == 32 && n < n_nodes) flag[w + 1] = uint32_t(bits >> 32);
+// pen32 and pen_flag from the penalty list (both zeroed before): the penalty
+// diagonal of the FP32 difference form (pen32[3 b + d] = the FP32 self
+// diagonal, float(P)) and bit b of the flags for box node b = perm[node]
+// (perm = null: b = node); the equations of the list are distinct.
+__global__ void k_pen_scatter(int32_t n_bc, const int32_t* __restrict__ eq,
+                              const double* __restrict__ pen, const int32_t* __restrict__ perm,
+                              float* __restrict__ pen32, uint32_t* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_bc) return;
+  const int32_t q = eq[i];
+  const int32_t node = q / 3;
+  const int64_t b = perm ? perm[node] : node;
+  pen32[3 * b + (q - 3 * node)] = float(pen[i]);
+  atomicOr(&flag[b >> 5], 1u << uint32_t(b & 31));
 }
 
 // o[3 i + d] = a[3 iperm[i] + d]: a node-ordered FP32 vector in the box numbering
@@ -1028,7 +1040,10 @@ __global__ void k_penalty(int32_t n_bc, const int32_t* __restrict__ eq,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_bc) {
     const int64_t p = pos[eq[i]];
-    val[p] = T(double(val[p]) + sign * pen[i]);
+    if constexpr (sizeof(T) == 4)
+      val[p] = sign > 0.0 ? T(pen[i]) : T(0);  // the difference form's self diagonal is the penalty
+    else
+      val[p] = T(double(val[p]) + sign * pen[i]);
   }
 }
 
@@ -1459,7 +1474,8 @@ void assemble_general(ufc_ctx* c, const double* dgp, T* val) {
                      c->stream, c->gen_node.get(), c->gen_ne_ptr.get(), c->gen_ne.get(),
                      c->coords.get(), c->conn.get(), c->nb_ptr.get(), c->nb_idx.get(),
                      MAT == UFC_MAT_ELASTIC ? c->young.get() : nullptr, c->nu, dgp,
-                     c->inc_c.get(), val);
+                     c->inc_c.get(), val, sizeof(T) == 4 ? c->kdiag32.get() : nullptr,
+                     c->diag_pos.get());
 }
 
 template <typename T, int VAR>
@@ -1507,7 +1523,10 @@ void ensure_diag_arrays(ufc_ctx* c) {
 DiagWrite diag_write(ufc_ctx* c, bool fp32) {
   DiagWrite dw{};
   dw.nd = 0;
-  if (!fp32 || c->fp32_view != kFp32ViewSymDiag || c->n_diag == 0) return dw;
+  if (!fp32) return dw;
+  dw.kdiag = c->kdiag32.get();
+  dw.diag_pos = c->diag_pos.get();
+  if (c->fp32_view != kFp32ViewSymDiag || c->n_diag == 0) return dw;
   ensure_diag_arrays(c);
   for (int s = 0; s < kMaxDiags; ++s) dw.off.o[s] = c->h_diag_off[s];
   dw.nd = c->n_diag;
@@ -1629,11 +1648,7 @@ void launch_diag(ufc_ctx* c, bool fp32) {
 
 void launch_fp32_difference_form(ufc_ctx* c) {
   // kdiag32 = diagonal of the assembled FP32 K (no penalties)
-  hipLaunchKernelGGL(k_diag<float>, dim3(blocks_for(c->n_dof, 256)), dim3(256), 0, c->stream,
-                     c->n_dof, c->diag_pos.get(), c->val32.get(), c->kdiag32.get());
-  hipLaunchKernelGGL(k_zero_self, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
-                     c->n_nodes, c->nb_ptr.get(), c->diag_pos.get(), c->val32.get());
-  UFC_HIP_CHECK(hipGetLastError());
+  // kdiag32 and the zeroed self blocks come from the FP32 assembly itself
   launch_penalty(c, true, +1.0);
   launch_diag32_penalty(c);
   launch_fp32_offdiag(c);
@@ -1673,10 +1688,14 @@ void launch_fp32_offdiag(ufc_ctx* c) {
                          c->diag32p.get());
     }
   }
-  hipLaunchKernelGGL(k_offdiag_penalty, dim3(blocks_for(c->n_nodes, 256)), dim3(256), 0, c->stream,
-                     c->n_nodes, c->diag_pos.get(), c->val32.get(),
-                     fp32_permuted(c) ? c->sym_iperm.get() : nullptr, c->pen32.get(),
-                     c->pen_flag.get());
+  UFC_HIP_CHECK(hipMemsetAsync(c->pen32.get(), 0, c->pen32.bytes(), c->stream));
+  UFC_HIP_CHECK(hipMemsetAsync(c->pen_flag.get(), 0, c->pen_flag.bytes(), c->stream));
+  const int32_t nb = int32_t(c->h_bc_eq.size());
+  if (nb)
+    hipLaunchKernelGGL(k_pen_scatter, dim3(blocks_for(nb, 256)), dim3(256), 0, c->stream, nb,
+                       c->bc_eq.get(), c->bc_pen.get(),
+                       fp32_permuted(c) ? c->sym_perm.get() : nullptr, c->pen32.get(),
+                       c->pen_flag.get());
   UFC_HIP_CHECK(hipGetLastError());
 }
 
