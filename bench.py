@@ -193,6 +193,56 @@ def cpu_baseline(pr, gpu_sols, repeats=5, rtol_full=1e-12):
     return out, errs
 
 
+def cpu_second_opinion(repeats=3):
+    """The small reference configuration (20x4x4 cantilever, 1575 DOF) timed
+    two ways on the host: oracle/_ref/ref_cantilever, a Fortran program built
+    from the reference's own compiled sparse module (its COO element assembly
+    and CSR conversion, and NM_CSR_MatVec_Optimized inside the PCG), and the C
+    oracle (1 thread); both assemble and solve to 1e-10, `repeats` times
+    (median and min), and their displacements are compared.  None when the
+    reference-built program is absent."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    from oracle import oracle as O
+    from ufc_hip import problems as P
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_cantilever")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "u.txt")
+        subprocess.run([exe, str(repeats), out], check=True, timeout=600)
+        txt = open(out).read().split()
+    ndof, nnz, iters, rep = (int(v) for v in txt[:4])
+    vals = np.array([float(v) for v in txt[4:]])
+    t_asm, t_sol, u_f = vals[:rep], vals[rep:2 * rep], vals[2 * rep:]
+    pr = P.cantilever()
+    t_oa, t_os = [], []
+    for _ in range(max(repeats, 5)):
+        t0 = time.perf_counter()
+        rp, ci = O.csr_pattern(pr.coords.shape[0], pr.conn)
+        val = O.assemble(pr.conn, pr.coords, pr.young, pr.nu, rp, ci, nthreads=1)
+        F = np.zeros(pr.n_dof)
+        O.face_loads_traction(pr.load_elem, pr.load_face, pr.conn, pr.coords, pr.load_value, F)
+        O.apply_penalty(pr.bc_eq, pr.bc_val, 1e12, rp, ci, val, F)
+        t1 = time.perf_counter()
+        uo, st, it, _ = O.pcg(rp, ci, val, F, rtol=1e-10, nthreads=1)
+        t_os.append(time.perf_counter() - t1)
+        t_oa.append(t1 - t0)
+    return {"workload": "20x4x4 C3D8 cantilever (1575 DOF), assemble + Jacobi-PCG to 1e-10",
+            "fortran_reference_kernels": {
+                "assemble_s_median": float(np.median(t_asm)), "assemble_s_min": float(t_asm.min()),
+                "solve_s_median": float(np.median(t_sol)), "solve_s_min": float(t_sol.min()),
+                "iters": iters, "nnz": nnz, "repeats": rep, "cores": 1,
+                "note": "reference NM_COO_AddElementMatrix (a linear search of the COO list per "
+                        "entry) + NM_COO_to_CSR, NM_CSR_MatVec_Optimized; element matrices, "
+                        "loads, penalty and the PCG loop restated in Fortran"},
+            "c_oracle": {"assemble_s_median": float(np.median(t_oa)),
+                         "solve_s_median": float(np.median(t_os)), "iters": int(it),
+                         "cores": 1, "repeats": len(t_oa)},
+            "u_rel_diff": float(np.linalg.norm(u_f - uo) / np.linalg.norm(uo))}
+
+
 # J2 update per Gauss point, algorithmic: committed state read and trial
 # state written (13 doubles each), tangent written (36 doubles), flag (4 B);
 # per element its connectivity, per node coordinates, u and u_n
@@ -743,6 +793,10 @@ def main():
         log("cpu baseline + full-size errors against the oracle")
         out["cpu_baseline"], cerr = cpu_baseline(pr, sols)
         log(f"cpu {out['cpu_baseline']['value']:.3e} DOF/s; errors {cerr}")
+        so = cpu_second_opinion()
+        if so is not None:
+            out["cpu_baseline"]["small_config_second_opinion"] = so
+            log(f"cpu second opinion (cantilever): {so}")
         out.setdefault("errors", {}).update(cerr)
     print(json.dumps(out), flush=True)
 

@@ -19,7 +19,7 @@ SRCS := $(REF)/L1_IF/Error/IF_Err_Def.f90 \
 OBJS := $(OUT)/IF_Err_Def.o $(OUT)/IF_Err_Brg.o $(OUT)/IF_Prec_Core.o $(OUT)/NM_Mtx_Sparse.o \
         $(OUT)/NM_Mtx_SpMVCSRTranspose.o
 
-all: $(OUT)/ref_golden
+all: $(OUT)/ref_golden $(OUT)/ref_cantilever
 
 $(OUT)/IF_Err_Def.o: $(REF)/L1_IF/Error/IF_Err_Def.f90
 	$(FC) $(FFLAGS) -J $(OUT) -c $< -o $@
@@ -35,5 +35,9 @@ $(OUT)/NM_Mtx_SpMVCSRTranspose.o: $(REF)/L2_NM/Solver/NM_Mtx_SpMVCSRTranspose.f9
 
 $(OUT)/ref_golden: $(HERE)ref_golden.f90 $(OBJS)
 	$(FC) $(FFLAGS) -J $(OUT) -I $(OUT) $(HERE)ref_golden.f90 $(OBJS) -o $@
+
+# second-opinion CPU timing of the small cantilever (reference CSR assembly and SpMV)
+$(OUT)/ref_cantilever: $(HERE)ref_cantilever.f90 $(OBJS)
+	$(FC) $(FFLAGS) -J $(OUT) -I $(OUT) $(HERE)ref_cantilever.f90 $(OBJS) -o $@
 
 .PHONY: all

@@ -629,3 +629,41 @@ def test_fp32_symdiag_box_numbering_vs_oracle(ctx, orc):
     assert np.abs(y16 - yo2).max() <= 1e-5 * np.abs(yo2).max()
     assert ctx.set_fp32_view(-1) == A.FP32_VIEW_SYMDIAG
     assert np.array_equal(ctx.spmv(x, precision=1), y_box)
+
+
+@pytest.mark.parametrize("view", ["symdiag", "pattern", "id32"])
+def test_fp32_bc_change_after_assembly(ctx, orc, view):
+    """A new penalty set after the FP32 assembly: the FP32 self diagonals of
+    equations no longer constrained return to exactly zero (the difference
+    form holds only the penalty there), and the FP32 product and the mixed
+    solve follow the new set (oracle with the new set: K x 1e-5, u 1e-8)."""
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(8, name="cube8")
+    P.setup_context(ctx, pr, 2)
+    views = {"symdiag": A.FP32_VIEW_SYMDIAG, "pattern": A.FP32_VIEW_PATTERN,
+             "id32": A.FP32_VIEW_ID32}
+    assert ctx.set_fp32_view(views[view]) == views[view]
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    # z = 0 clamp -> x = 0 clamp plus one z-equation of the old set
+    x0 = np.nonzero(pr.coords[:, 0] == 0.0)[0]
+    new_eq = np.unique(np.concatenate([(3 * x0[:, None] + np.arange(3)).ravel(),
+                                       pr.bc_eq[:1]])).astype(np.int32)
+    ctx.apply_bc(new_eq, np.zeros(new_eq.size), 1e12)
+    say("fp32 bc change, view", view)
+    rp, ci = ctx.get_pattern()
+    v32 = ctx.get_values(1)
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    diag = v32[rows == ci]
+    expect = np.zeros(pr.n_dof, np.float32)
+    expect[new_eq] = np.float32(1e12)
+    assert np.array_equal(diag, expect)
+    pr2 = P.cube_pressure(8, name="cube8")
+    pr2.bc_eq, pr2.bc_val = new_eq, np.zeros(new_eq.size)
+    rp_o, ci_o, val_o, F_o, _ = _oracle_system(orc, pr2)
+    x = np.random.default_rng(3).normal(size=pr.n_dof)
+    yo = orc.spmv(rp_o, ci_o, val_o, x)
+    assert np.abs(ctx.spmv(x, precision=1) - yo).max() <= 1e-5 * np.abs(yo).max()
+    uo, st, _, _ = orc.pcg(rp_o, ci_o, val_o, F_o, rtol=1e-14, max_iter=200000)
+    u, info = ctx.pcg(precision=2, rtol=1e-12, max_iter=200000, replace_rtol=0.1)
+    assert info.status == 0 and np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8

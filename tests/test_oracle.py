@@ -580,3 +580,37 @@ def test_spmv_transpose_matches_reference_transpose(orc, golden_dir, mesh):
     yt = orc.spmv_t(g["ia"] - 1, g["ja"] - 1, g["a"], x)
     assert np.array_equal(yt, g["yt"])
     assert not np.array_equal(g["yt"], g["y"])
+
+
+def test_reference_sparse_kernels_cantilever_match_oracle(orc):
+    """oracle/_ref/ref_cantilever (the reference's own COO element assembly,
+    NM_COO_to_CSR and NM_CSR_MatVec_Optimized, compiled from the reference
+    tree; element matrices, loads, penalty and the PCG loop restated in
+    Fortran) solves the 20x4x4 cantilever to 1e-10 in the oracle's number of
+    iterations and to its displacements (1e-10)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                       "_ref", "ref_cantilever")
+    if not os.path.exists(exe):
+        pytest.skip("reference-built program not present (needs the reference tree to build)")
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "u.txt")
+        subprocess.run([exe, "1", out], check=True, timeout=300)
+        txt = open(out).read().split()
+    ndof, nnz, iters, rep = (int(v) for v in txt[:4])
+    u = np.array([float(v) for v in txt[4 + 2 * rep:]])
+    coords, conn = orc.mesh_box(20, 4, 4, 10.0, 2.0, 2.0)
+    rp, ci = orc.csr_pattern(coords.shape[0], conn)
+    val = orc.assemble(conn, coords, np.full(conn.shape[0], 210e9), 0.3, rp, ci)
+    assert ndof == 3 * coords.shape[0] and nnz == val.size
+    F = np.zeros(ndof)
+    tip = np.arange(19, 320, 20)  # elements at i = nx - 1
+    orc.face_loads_traction(tip.astype(np.int32), np.full(tip.size, 5, np.int32), conn, coords,
+                            (0.0, 0.0, -1e3 / 4.0), F)
+    x0 = np.nonzero(coords[:, 0] == 0.0)[0]
+    eq = (3 * x0[:, None] + np.arange(3)).ravel().astype(np.int32)
+    orc.apply_penalty(eq, np.zeros(eq.size), 1e12, rp, ci, val, F)
+    uo, st, it, _ = orc.pcg(rp, ci, val, F, rtol=1e-10)
+    assert st == 0 and it == iters
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-10
