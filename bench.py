@@ -83,6 +83,62 @@ def spmv64_bytes(n_dof, nnz):
     return nnz * (8 + 4) + (n_dof + 1) * 4 + 2 * n_dof * 8
 
 
+def spmv64_block_bytes(n_dof, nnz, n_nodes):
+    """Algorithmic bytes of one FP64 node-block SpMV: values + one 4-byte
+    neighbour id per 3x3 block + node pointers + x + y."""
+    return nnz * 8 + nnz // 9 * 4 + (n_nodes + 1) * 4 + 2 * n_dof * 8
+
+
+def storage_vs_precision(ctx, pr, torch, x, ufc_hip, headline, fp64, check_step, replace_rtol):
+    """The 4.2x of mixed over FP64 mixes two changes: precision (FP64 -> FP32
+    values and vectors) and storage (a column index per value -> one id per
+    block -> each off-diagonal block stored once).  This times the solve with
+    one change at a time on the same values: FP64 on the node-block layout
+    (the CSR's index bytes removed, same bits), then the mixed solve on the
+    FP32 4-byte-id view (the same layout in FP32) and on the stencil-pattern
+    view, beside the two headline forms."""
+    import numpy as np
+    from ufc_hip import _abi as A
+    out = {"note": ("fp64_csr -> fp64_block: index bytes only (same bits); fp64_block -> "
+                    "mixed_id32: precision only (same node-block layout); mixed_id32 -> "
+                    "mixed_symdiag: each off-diagonal block stored once")}
+
+    def timed(prec, label, steps=2):
+        repl = replace_rtol if prec == ufc_hip.PREC_MIXED else 0.0
+        run_step(ctx, pr, prec, 1e-10, torch, x, repl)
+        rs = []
+        for _ in range(steps):
+            r, _ = run_step(ctx, pr, prec, 1e-10, torch, x, repl)
+            check_step(label, r, "storage/precision step")
+            rs.append(r)
+        return {"ms": float(np.mean([r["wall_ms"] for r in rs])), "spmv_ms": rs[-1]["spmv_ms"],
+                "iters": rs[-1]["niter"], "steps": steps}
+
+    ctx.set_fp64_view(A.FP64_VIEW_BLOCK)
+    b = timed(ufc_hip.PREC_FP64, "fp64_block")
+    ctx.set_fp64_view(A.FP64_VIEW_CSR)
+    b["spmv_bytes"] = spmv64_block_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes)
+    out["fp64_csr"] = {"ms": fp64["ms"], "spmv_ms": fp64["spmv_ms"], "iters": fp64["iters"],
+                       "spmv_bytes": spmv64_bytes(pr.n_dof, ctx.nnz)}
+    out["fp64_block"] = b
+    log(f"fp64 node-block layout: {b['ms']:.1f} ms, spmv {b['spmv_ms']:.4f} ms")
+    nbc = int(np.unique(pr.bc_eq).size)
+    for view, name, ib in ((A.FP32_VIEW_ID32, "mixed_id32", 4),
+                           (A.FP32_VIEW_PATTERN, "mixed_pattern", 1)):
+        ctx.set_fp32_view(view)
+        r = timed(ufc_hip.PREC_MIXED, name)
+        r["spmv_bytes"] = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ib, nbc)
+        out[name] = r
+        log(f"{name}: {r['ms']:.1f} ms, spmv {r['spmv_ms']:.4f} ms")
+    ctx.set_fp32_view(-1)
+    out["mixed_symdiag"] = {"ms": headline["ms"], "spmv_ms": None,
+                            "iters": headline["inner_iters"]}
+    out["fp64_csr_over_fp64_block"] = fp64["ms"] / b["ms"]
+    out["fp64_block_over_mixed_id32"] = b["ms"] / out["mixed_id32"]["ms"]
+    out["mixed_id32_over_mixed_symdiag"] = out["mixed_id32"]["ms"] / headline["ms"]
+    return out
+
+
 def measured_traffic(n_dof, kernel):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, when it was taken on this workload."""
@@ -431,6 +487,9 @@ def main():
                     help="central-difference check of the adjoint gradient on N elements "
                          "(0: none)")
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 Newton workload")
+    ap.add_argument("--no-views", action="store_true",
+                    help="skip the storage-vs-precision runs (FP64 node-block layout, FP32 "
+                         "4-byte-id and stencil-pattern views)")
     ap.add_argument("--no-renumbered", action="store_true",
                     help="skip the randomly renumbered copy of the cube")
     ap.add_argument("--j2-size", type=int, default=64)
@@ -653,6 +712,16 @@ def main():
         asm["executed_flop_source"] = ("profiles/assembly128_valu_flops.md (rocprofv3 "
                                        "SQ_INSTS_VALU_FLOPS_FP32/FP64 x 64 lanes)")
     out["assembly"] = asm
+    if not args.no_views and "fp64" in out:
+        log("storage vs precision")
+        out["storage_vs_precision"] = storage_vs_precision(ctx, pr, torch, x, ufc_hip,
+                                                           out["mixed"], out["fp64"], check_step,
+                                                           args.replace_rtol)
+        out["storage_vs_precision"]["mixed_symdiag"]["spmv_ms"] = out["roofline"]["ms_per_launch"]
+        if failed:
+            print(json.dumps({"metric": METRIC, "error": "a storage/precision solve did not "
+                              "converge", "failed": failed}), flush=True)
+            sys.exit(3)
     ctx.close()
     del pr.young_dev
 

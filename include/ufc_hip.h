@@ -353,6 +353,17 @@ int ufc_rt_asm_set_values(ufc_ctx* ctx, int32_t precision, const double* values,
  * storage format: for tests and measurements of the individual views. */
 int ufc_rt_asm_set_fp32_view(ufc_ctx* ctx, int32_t view, ufc_status* st);
 
+/* Selects the layout the FP64 SpMV reads (no counterpart in the reference,
+ * whose SpMV has one storage format): UFC_FP64_VIEW_CSR (default), the CSR
+ * with a column index per value as NM_CSR_MatVec_Optimized
+ * (NM_Mtx_Sparse.f90:341-365) reads it, 12 bytes per value; or
+ * UFC_FP64_VIEW_BLOCK, the same values with one 4-byte neighbour id per 3x3
+ * block, 8.4 bytes per value.  Both give the same bits.  Kept for the mesh's
+ * lifetime (FP64 SpMVs, PCG, adjoint). */
+#define UFC_FP64_VIEW_CSR 0
+#define UFC_FP64_VIEW_BLOCK 1
+int ufc_rt_asm_set_fp64_view(ufc_ctx* ctx, int32_t view, ufc_status* st);
+
 /* Sets / reads the load vector F (n_dof); penalty terms of apply_bc are
  * included when read. */
 int ufc_rt_set_rhs(ufc_ctx* ctx, const double* f, ufc_status* st);

@@ -73,7 +73,8 @@ def test_info_struct_layout_matches_header(tmp_path):
         + '  printf("%d %d %d %d\\n", UFC_FP32_VIEW_ID32, UFC_FP32_VIEW_OFF16, '
           'UFC_FP32_VIEW_PATTERN, UFC_FP32_VIEW_SYMDIAG);\n'
         + '  printf("%d %d %d %d\\n", UFC_J2_HARD_LINEAR, UFC_J2_HARD_SWIFT, UFC_J2_HARD_VOCE, '
-          'UFC_J2_HARD_AF);\n  return 0;\n}\n')
+          'UFC_J2_HARD_AF);\n'
+        + '  printf("%d %d\\n", UFC_FP64_VIEW_CSR, UFC_FP64_VIEW_BLOCK);\n  return 0;\n}\n')
     exe = tmp_path / "layout"
     subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
@@ -89,6 +90,7 @@ def test_info_struct_layout_matches_header(tmp_path):
         A.FP32_VIEW_ID32, A.FP32_VIEW_OFF16, A.FP32_VIEW_PATTERN, A.FP32_VIEW_SYMDIAG]
     assert [int(v) for v in out[k + 1].split()] == [
         A.J2_HARD_LINEAR, A.J2_HARD_SWIFT, A.J2_HARD_VOCE, A.J2_HARD_AF]
+    assert [int(v) for v in out[k + 2].split()] == [A.FP64_VIEW_CSR, A.FP64_VIEW_BLOCK]
 
 
 def test_generators_match_oracle(orc):

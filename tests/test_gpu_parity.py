@@ -667,3 +667,30 @@ def test_fp32_bc_change_after_assembly(ctx, orc, view):
     uo, st, _, _ = orc.pcg(rp_o, ci_o, val_o, F_o, rtol=1e-14, max_iter=200000)
     u, info = ctx.pcg(precision=2, rtol=1e-12, max_iter=200000, replace_rtol=0.1)
     assert info.status == 0 and np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+
+
+@pytest.mark.parametrize("kind", ["cube8", "polar"])
+def test_fp64_block_view_bitwise_equals_csr(ctx, orc, kind):
+    """The FP64 node-block layout (one neighbour id per 3x3 block) gives the
+    CSR SpMV's bits (same products, same order), so the FP64 PCG run on it is
+    bit-identical too; the polar mesh (33 neighbours) runs its long-list form."""
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    pr = P.cube_pressure(8, name="cube8") if kind == "cube8" else P.polar_prism(5, 3, 4)
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    say("fp64 block view", kind)
+    x = np.random.default_rng(5).normal(size=pr.n_dof)
+    y_csr = ctx.spmv(x, precision=0)
+    u_csr, i_csr = ctx.pcg(precision=0, rtol=1e-12, max_iter=100000)
+    ctx.set_fp64_view(A.FP64_VIEW_BLOCK)
+    y_blk = ctx.spmv(x, precision=0)
+    u_blk, i_blk = ctx.pcg(precision=0, rtol=1e-12, max_iter=100000)
+    assert np.array_equal(y_blk, y_csr)
+    assert i_blk.niter == i_csr.niter and np.array_equal(u_blk, u_csr)
+    yo = orc.spmv(rp, ci, val, x)
+    assert np.abs(y_blk - yo).max() <= 1e-12 * np.abs(yo).max()
+    with pytest.raises(A.UfcError) as e:
+        ctx.set_fp64_view(2)
+    assert e.value.code == A.STATUS_INVALID

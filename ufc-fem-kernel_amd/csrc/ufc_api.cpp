@@ -603,6 +603,17 @@ int ufc_rt_asm_set_fp32_view(ufc_ctx* c, int32_t view, ufc_status* st) {
   });
 }
 
+int ufc_rt_asm_set_fp64_view(ufc_ctx* c, int32_t view, ufc_status* st) {
+  return guarded(st, "ufc_rt_asm_set_fp64_view", [&](std::string& msg) {
+    check_mesh(c);
+    UFC_REQUIRE(view == ufc::kFp64ViewCsr || view == ufc::kFp64ViewBlock, UFC_STATUS_INVALID,
+                "view must be 0 (CSR) or 1 (node blocks)");
+    c->fp64_view = view;
+    msg = "FP64 view " + std::to_string(view);
+    return UFC_STATUS_OK;
+  });
+}
+
 int ufc_ctx_enable_timing(ufc_ctx* c, int32_t per_kernel, ufc_status* st) {
   return guarded(st, "ufc_ctx_enable_timing", [&](std::string&) {
     check_ctx(c);
@@ -1384,7 +1395,7 @@ int ufc_nm_spmv(ufc_ctx* c, int32_t precision, const double* x, double* y, ufc_s
     const double* dx = dev_in(c, x, size_t(c->n_dof), tx);
     if (precision == UFC_PREC_FP64) {
       UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
-      if (!c->col_idx.get()) ufc::build_col_idx(c);
+      if (!c->col_idx.get() && c->fp64_view == ufc::kFp64ViewCsr) ufc::build_col_idx(c);
       ufc::spmv64(c, dx, c->w_t.get());
       copy_out(c, y, c->w_t.get(), size_t(c->n_dof));
     } else {
@@ -1437,7 +1448,7 @@ static int solve_impl(ufc_ctx* c, const ufc_nm_solver_algo* algo, const double* 
   double rnorm = 0.0;
   if (prec == UFC_PREC_FP64) {
     UFC_REQUIRE(c->have64, UFC_STATUS_INVALID, "no FP64 values assembled");
-    if (!c->col_idx.get()) ufc::build_col_idx(c);
+    if (!c->col_idx.get() && c->fp64_view == ufc::kFp64ViewCsr) ufc::build_col_idx(c);
     if (!s->x0_zero)
       UFC_HIP_CHECK(hipMemcpyAsync(c->w_x.get(), s->x, n * 8, hipMemcpyDefault, c->stream));
     code = ufc::pcg64(c, b, c->w_x.get(), s->x0_zero != 0, algo->max_iter, algo->rtol,

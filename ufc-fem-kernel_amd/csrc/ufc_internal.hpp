@@ -99,6 +99,9 @@ constexpr int kDiagTile = 256;      // nodes per tile of the block-diagonal layo
 constexpr int kDiagPlanes = 16;
 // FP32 SpMV views (ufc_rt_asm_info.fp32_view)
 constexpr int kFp32ViewId32 = 0, kFp32ViewOff16 = 1, kFp32ViewPattern = 2, kFp32ViewSymDiag = 3;
+// FP64 SpMV layouts: the CSR with a column index per value (the reference's
+// NM_CSR_MatVec_Optimized), or the same values with one neighbour id per block
+constexpr int kFp64ViewCsr = 0, kFp64ViewBlock = 1;
 
 // offsets o_s of the symmetric FP32 view's block diagonals (kernel argument)
 struct DiagOffsets {
@@ -229,6 +232,7 @@ struct ufc_ctx {
   ufc::DBuf<double> w_perm2;     // FP64 iterate in the box numbering (restarted form)
   int32_t fp32_view = -1;       // view of the FP32 SpMV (kFp32View*); -1 before the pattern
   int32_t fp32_view_request = -1;  // -1: the fastest available
+  int32_t fp64_view = 0;           // kFp64View*: layout the FP64 SpMV reads
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1) when nnz < 2^31 (the exported CSR)
   ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
   ufc::DBuf<int64_t> diag_pos;    // (n_dof) position of K(i,i) in values

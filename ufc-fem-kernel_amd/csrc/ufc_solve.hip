@@ -61,6 +61,9 @@ constexpr int kViewPattern = 2;  // one stencil-pattern byte per node
 // 4-byte ids, neighbour lists longer than a 32-lane group (FP32 view 0 and
 // the FP64 CSR on meshes with nodes of more than kMaxNodeNbrs neighbours)
 constexpr int kViewWide = 3;
+// FP64 node-block view: one 4-byte neighbour id per 3x3 block instead of a
+// column index per value (narrow: neighbour lists within a 32-lane group)
+constexpr int kViewBlock64 = 4, kViewBlock64Wide = 5;
 constexpr int WAVES_PER_BLOCK = kSolveThreads / WAVE;
 constexpr int GROUPS_PER_BLOCK = kSpmvThreads / GROUP;
 
@@ -183,6 +186,34 @@ __device__ __forceinline__ void node_rows_csr(int64_t n, int k, const int32_t* _
       const T* v = val + base + d * d3;
       const int32_t* c = col_idx + base + d * d3;
       a[d] += v[0] * x[c[0]] + v[1] * x[c[1]] + v[2] * x[c[2]];
+    }
+    if constexpr (!WIDE) break;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) a[d] = sum_group32(a[d]);
+}
+
+// FP64 node-block rows of node n: lane k on neighbour k (and k + 32, ... when
+// WIDE) reads its id once and its three 3-value runs; the products and their
+// order are those of node_rows_csr, so both give the same bits.
+template <typename T, bool WIDE>
+__device__ __forceinline__ void node_rows_block(int64_t n, int k,
+                                                const int32_t* __restrict__ nb_ptr,
+                                                const int32_t* __restrict__ nb_idx,
+                                                const T* __restrict__ val, const T* __restrict__ x,
+                                                T (&a)[3]) {
+  const int32_t q0 = nb_ptr[n];
+  const int deg = nb_ptr[n + 1] - q0;
+  a[0] = a[1] = a[2] = T(0);
+  for (int kk = k; kk < deg; kk += GROUP) {
+    const int64_t m = nb_idx[q0 + kk];
+    const int64_t base = 9 * int64_t(q0) + 3 * kk;
+    const int d3 = 3 * deg;
+    const T x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T* v = val + base + d * d3;
+      a[d] += v[0] * x0 + v[1] * x1 + v[2] * x2;
     }
     if constexpr (!WIDE) break;
   }
@@ -512,7 +543,10 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(
         const int64_t n = b + gi + h * GROUPS_PER_BLOCK;
         if (n >= n_nodes) break;
         T a[3];
-        node_rows_csr<T, VIEW == kViewWide>(n, k, nb_ptr, col_idx, val, x, a);
+        if constexpr (VIEW == kViewBlock64 || VIEW == kViewBlock64Wide)
+          node_rows_block<T, VIEW == kViewBlock64Wide>(n, k, nb_ptr, nb_idx, val, x, a);
+        else
+          node_rows_csr<T, VIEW == kViewWide>(n, k, nb_ptr, col_idx, val, x, a);
         if (k < 3) y[3 * n + k] = (k == 0) ? a[0] : ((k == 1) ? a[1] : a[2]);
       }
     }
@@ -1225,6 +1259,17 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
                          c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),
                          c->pat_off.get(), c->col_idx.get(), val, nullptr, nullptr, x, y, sc,
                          pap);
+  } else if (c->fp64_view == kFp64ViewBlock) {
+    if (c->wide)
+      hipLaunchKernelGGL((k_spmv<T, false, kViewBlock64Wide>), grid, block, 0, c->stream,
+                         c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(),
+                         c->nb_pat.get(), c->pat_off.get(), c->col_idx.get(), val, nullptr,
+                         nullptr, x, y, sc, nullptr);
+    else
+      hipLaunchKernelGGL((k_spmv<T, false, kViewBlock64>), grid, block, 0, c->stream,
+                         c->n_nodes, c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(),
+                         c->nb_pat.get(), c->pat_off.get(), c->col_idx.get(), val, nullptr,
+                         nullptr, x, y, sc, nullptr);
   } else if (c->wide) {
     hipLaunchKernelGGL((k_spmv<T, false, kViewWide>), grid, block, 0, c->stream, c->n_nodes,
                        c->nb_ptr.get(), c->nb_idx.get(), c->nb_off16.get(), c->nb_pat.get(),

@@ -32,6 +32,8 @@ MODULE ufc_hip_iface
   ! FP32 operator views (ufc_rt_asm_info%fp32_view, ufc_rt_asm_set_fp32_view)
   INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_FP32_VIEW_ID32 = 0, UFC_FP32_VIEW_OFF16 = 1, &
       UFC_FP32_VIEW_PATTERN = 2, UFC_FP32_VIEW_SYMDIAG = 3
+  ! FP64 SpMV layouts (ufc_rt_asm_set_fp64_view)
+  INTEGER(C_INT32_T), PARAMETER, PUBLIC :: UFC_FP64_VIEW_CSR = 0, UFC_FP64_VIEW_BLOCK = 1
 
   TYPE, BIND(C), PUBLIC :: ufc_status
     INTEGER(C_INT32_T) :: status_code = 0
@@ -210,7 +212,7 @@ MODULE ufc_hip_iface
             ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_ph_j2_get_state, &
             ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_ap_read_inp, ufc_ap_free, &
             ufc_rt_restart_save, ufc_rt_restart_load, ufc_nm_spmv_t, ufc_rt_asm_set_values, &
-            ufc_rt_asm_set_fp32_view, ufc_status_message
+            ufc_rt_asm_set_fp32_view, ufc_rt_asm_set_fp64_view, ufc_status_message
 
   INTERFACE
     INTEGER(C_INT) FUNCTION ufc_ctx_create(device, ctx, st) BIND(C, NAME='ufc_ctx_create')
@@ -313,6 +315,14 @@ MODULE ufc_hip_iface
 
     INTEGER(C_INT) FUNCTION ufc_rt_asm_set_fp32_view(ctx, view, st) &
         BIND(C, NAME='ufc_rt_asm_set_fp32_view')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT32_T), VALUE :: view
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    INTEGER(C_INT) FUNCTION ufc_rt_asm_set_fp64_view(ctx, view, st) &
+        BIND(C, NAME='ufc_rt_asm_set_fp64_view')
       IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
       TYPE(C_PTR), VALUE :: ctx
       INTEGER(C_INT32_T), VALUE :: view

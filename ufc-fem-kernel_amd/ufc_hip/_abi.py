@@ -33,6 +33,9 @@ FP32_VIEW_ID32 = 0
 FP32_VIEW_OFF16 = 1
 FP32_VIEW_PATTERN = 2
 FP32_VIEW_SYMDIAG = 3
+# FP64 SpMV layouts (ufc_rt_asm_set_fp64_view)
+FP64_VIEW_CSR = 0
+FP64_VIEW_BLOCK = 1
 
 MAT_ELASTIC = 0
 MAT_J2 = 1
@@ -180,6 +183,7 @@ _SIGS = {
     "ufc_nm_spmv_t": [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],
     "ufc_rt_asm_set_values": [C.c_void_p, C.c_int32, C.c_void_p, P(Status)],
     "ufc_rt_asm_set_fp32_view": [C.c_void_p, C.c_int32, P(Status)],
+    "ufc_rt_asm_set_fp64_view": [C.c_void_p, C.c_int32, P(Status)],
     "ufc_nm_adjoint_solve": [C.c_void_p, P(SolverAlgo), C.c_void_p, P(SolverState), P(Status)],
     "ufc_mesh_box": [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                      C.c_double, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p, P(Status)],

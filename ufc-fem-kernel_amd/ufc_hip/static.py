@@ -190,6 +190,12 @@ class Context:
         self.fp32_view, self.nb_index_bytes = i.fp32_view, i.nb_index_bytes
         return self.fp32_view
 
+    def set_fp64_view(self, view):
+        """Layout of the FP64 SpMV: A.FP64_VIEW_CSR (a column index per value,
+        the reference's) or A.FP64_VIEW_BLOCK (one neighbour id per 3x3
+        block); both give the same bits."""
+        self._call("ufc_rt_asm_set_fp64_view", int(view))
+
     def get_pattern(self, index_base=0):
         rp = np.empty(self.n_dof + 1, np.int32)
         ci = np.empty(self.nnz, np.int32)
