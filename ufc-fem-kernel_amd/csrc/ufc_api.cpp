@@ -329,6 +329,48 @@ std::vector<int32_t> box_numbering(int32_t nn, const std::vector<int32_t>& conn,
 bool collect_sym_offsets(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
                          const std::vector<int32_t>* perm, std::vector<int32_t>& all);
 
+// The LDS x window of the symmetric SpMV (XWindow): the offsets {0, +-o_s}
+// sorted and merged into runs wherever consecutive offsets are at most a tile
+// apart; none when more than kMaxXClusters runs or kMaxXWin nodes are needed
+// (the SpMV then gathers x from global memory).
+void build_x_window(ufc_ctx* c) {
+  c->have_xwin = false;
+  std::vector<int32_t> offs{0};
+  for (int s = 0; s < c->n_diag; ++s) {
+    offs.push_back(c->h_diag_off[s]);
+    offs.push_back(-c->h_diag_off[s]);
+  }
+  std::sort(offs.begin(), offs.end());
+  ufc::XWindow w{};
+  int32_t total = 0;
+  size_t i = 0;
+  while (i < offs.size()) {
+    size_t j = i;
+    while (j + 1 < offs.size() && offs[j + 1] - offs[j] <= ufc::kDiagTile) ++j;
+    if (w.nc == ufc::kMaxXClusters) return;
+    w.lo[w.nc] = offs[i];
+    w.base[w.nc] = total;
+    w.len[w.nc] = offs[j] - offs[i] + ufc::kDiagTile;
+    total += w.len[w.nc];
+    ++w.nc;
+    i = j + 1;
+  }
+  if (total > ufc::kMaxXWin) return;
+  auto slot = [&](int32_t o) {
+    for (int k = 0; k < w.nc; ++k)
+      if (o >= w.lo[k] && o < w.lo[k] + w.len[k] - ufc::kDiagTile + 1) return w.base[k] + o - w.lo[k];
+    return -1;
+  };
+  w.self = slot(0);
+  for (int s = 0; s < ufc::kMaxDiags; ++s) {
+    w.up[s] = s < c->n_diag ? slot(c->h_diag_off[s]) : w.self;
+    w.dn[s] = s < c->n_diag ? slot(-c->h_diag_off[s]) : w.self;
+    if (w.up[s] < 0 || w.dn[s] < 0) return;
+  }
+  c->xwin = w;
+  c->have_xwin = true;
+}
+
 void reset_sym_diagonals(ufc_ctx* c) {
   c->n_diag = 0;
   c->sym_permuted = false;
@@ -340,6 +382,7 @@ void reset_sym_diagonals(ufc_ctx* c) {
   c->diag_b.release();
   c->diag_c.release();
   c->diag_from_assembly = false;
+  c->have_xwin = false;
 }
 
 void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
@@ -368,6 +411,7 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
   c->n_diag = int32_t(all.size());
   for (int s = 0; s < ufc::kMaxDiags; ++s) c->h_diag_off[s] = s < c->n_diag ? all[size_t(s)] : 0;
   c->diag_pad_tiles = int32_t(pad_tiles);
+  build_x_window(c);
   c->diag_entries = entries;
   if (c->sym_permuted) {
     std::vector<int32_t> iperm(static_cast<size_t>(nn));

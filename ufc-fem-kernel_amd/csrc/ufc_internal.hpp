@@ -108,6 +108,20 @@ struct DiagOffsets {
   int32_t o[kMaxDiags];
 };
 
+// x window of a symmetric-SpMV tile in LDS: the nodes n + o for the tile's
+// nodes n and every offset o in {0, +-o_s}, as at most kMaxXClusters runs of
+// consecutive nodes (offsets closer than a tile share a run).  Run c starts
+// lo[c] nodes after the tile's first node, holds len[c] nodes and sits at
+// LDS node slot base[c]; up[s], dn[s] and self are the slots of n + o_s,
+// n - o_s and n for the tile's first node (lane l adds l).
+constexpr int kMaxXClusters = 4;
+constexpr int kMaxXWin = 1664;  // nodes (16 B each: 26 KB); a structured box needs 3 x 516
+struct XWindow {
+  int32_t nc;
+  int32_t lo[kMaxXClusters], base[kMaxXClusters], len[kMaxXClusters];
+  int32_t up[kMaxDiags], dn[kMaxDiags], self;
+};
+
 // C3D8 variants, PH_ELEM_C3D8_VARIANT_* (PH_Elem_C3D8.f90:111-116)
 constexpr int kVarFull = 1, kVarReduced = 2, kVarSelective = 3, kVarIncompat = 4,
               kVarGauss27 = 6;
@@ -233,6 +247,8 @@ struct ufc_ctx {
   int32_t fp32_view = -1;       // view of the FP32 SpMV (kFp32View*); -1 before the pattern
   int32_t fp32_view_request = -1;  // -1: the fastest available
   int32_t fp64_view = 0;           // kFp64View*: layout the FP64 SpMV reads
+  bool have_xwin = false;          // the symmetric SpMV stages its x window in LDS
+  ufc::XWindow xwin{};
   ufc::DBuf<int32_t> row_ptr;     // (n_dof + 1) when nnz < 2^31 (the exported CSR)
   ufc::DBuf<int32_t> col_idx;     // (nnz)  built on demand (FP64 CSR SpMV, export)
   ufc::DBuf<int64_t> diag_pos;    // (n_dof) position of K(i,i) in values

@@ -588,12 +588,18 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 // tile order, 0.279 XCD order, 0.250 with the blocked tile-major order).
 // pap != null: the workgroup partials of p.Ap as in k_spmv (fixed order,
 // barrier-free LDS combine).
+// XW: the tile's x window (XWindow: the 27 neighbour offsets' nodes as a few
+// runs of consecutive nodes, 1548 nodes on a structured box) is first loaded
+// into LDS with coalesced loads, and every x_{n +- o_s} and x_n is then an LDS
+// read: 27 global gathers per node become 6 streaming loads per thread.
+// Window nodes outside the mesh hold 0; their blocks are zero.
 // diagonals whose loads are issued together (128^3: 4 and 7 0.266 ms, 13
 // 0.272 ms; one at a time, as the compiler schedules it unstaged, drains the
 // loads after every diagonal)
 constexpr int kDiagBatch = 4;
+template <bool XW>
 __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
-    int32_t n_nodes, DiagOffsets off, int nd, int32_t pad_tiles,
+    int32_t n_nodes, DiagOffsets off, int nd, int32_t pad_tiles, XWindow win,
     const float4* __restrict__ da, const float4* __restrict__ db, const float* __restrict__ dc,
     const float* __restrict__ pen, const uint32_t* __restrict__ pen_flag,
     const float* __restrict__ x, float* __restrict__ y, const PcgScalars* __restrict__ sc,
@@ -602,6 +608,7 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
   constexpr int kWaves = kDiagThreads / WAVE;
   __shared__ double slot[kWaves];
   __shared__ int arrived;
+  __shared__ float4 xw[XW ? kMaxXWin : 1];
   if (pap) {
     if (threadIdx.x == 0) arrived = 0;
     __syncthreads();
@@ -622,9 +629,31 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
     // 32-bit indices: n < n_nodes < 2^31 and the entries < 2^31 (the host
     // checks), which keeps the address arithmetic off 64-bit VALU pairs
     const int32_t n = t * kDiagTile + int32_t(threadIdx.x);
-    if (t >= ntiles || n >= n_nodes) continue;
+    if (t >= ntiles) continue;  // uniform over the workgroup
+    const int tid = int(threadIdx.x);
+    if constexpr (XW) {
+      const int32_t t0 = t * kDiagTile;
+      for (int c = 0; c < win.nc; ++c)
+        for (int i = tid; i < win.len[c]; i += kDiagThreads) {
+          const int32_t m = t0 + win.lo[c] + i;
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (m >= 0 && m < n_nodes) {
+            const float3 q = load3(x + 3 * m);
+            v = make_float4(q.x, q.y, q.z, 0.f);
+          }
+          xw[win.base[c] + i] = v;
+        }
+      __syncthreads();
+    }
+    if (n < n_nodes) {
     const int32_t tu = (t + pad_tiles) * nd;  // tile row of the upper entries
-    const float3 xn = load3(x + 3 * n);
+    float3 xn;
+    if constexpr (XW) {
+      const float4 q = xw[win.self + tid];
+      xn = make_float3(q.x, q.y, q.z);
+    } else {
+      xn = load3(x + 3 * n);
+    }
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
     // Branch-free over kMaxDiags diagonals, in batches of kDiagBatch whose
     // loads are all issued before the batch's arithmetic (without the
@@ -642,10 +671,19 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
         const int s = s0 + g < kMaxDiags ? s0 + g : kMaxDiags - 1;
         const int32_t se = s < nd ? s : nd - 1;
         const int32_t o = (s0 + g < kMaxDiags && s < nd) ? off.o[s] : 0;
-        const int32_t mu = n < n_nodes - o ? n + o : n;
-        const int32_t ml = n >= o ? n - o : n;
-        xu[g] = load3(x + 3 * mu);
-        xl[g] = load3(x + 3 * ml);
+        if constexpr (XW) {
+          // a padding diagonal (o = 0) reads x_n: its differences are zero
+          const bool real = s0 + g < kMaxDiags && s < nd;
+          const float4 qu = xw[(real ? win.up[s] : win.self) + tid];
+          const float4 ql = xw[(real ? win.dn[s] : win.self) + tid];
+          xu[g] = make_float3(qu.x, qu.y, qu.z);
+          xl[g] = make_float3(ql.x, ql.y, ql.z);
+        } else {
+          const int32_t mu = n < n_nodes - o ? n + o : n;
+          const int32_t ml = n >= o ? n - o : n;
+          xu[g] = load3(x + 3 * mu);
+          xl[g] = load3(x + 3 * ml);
+        }
         const int32_t e = (tu + se) * kDiagTile + int32_t(threadIdx.x);
         const int32_t m = n - o + pad_tiles * kDiagTile;  // >= 0: o <= pad_tiles kDiagTile
         const int32_t el = ((m / kDiagTile) * nd + se) * kDiagTile + m % kDiagTile;
@@ -678,6 +716,8 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
     }
     store3(y + 3 * n, a0, a1, a2);
     dot += (double(xn.x) * double(a0) + double(xn.y) * double(a1)) + double(xn.z) * double(a2);
+    }
+    if constexpr (XW) __syncthreads();  // the next tile's window reuses the LDS
   }
   if (pap) {
     dot = sum_lanes(dot, WAVE);
@@ -1231,11 +1271,18 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
     if (c->fp32_view == kFp32ViewSymDiag) {
       DiagOffsets off;
       for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
-      hipLaunchKernelGGL(k_spmv_sym, dim3(sym_blocks(c)), dim3(kDiagThreads), 0, c->stream,
-                         c->n_nodes, off, int(c->n_diag), c->diag_pad_tiles,
-                         reinterpret_cast<const float4*>(c->diag_a.get()),
-                         reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
-                         c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
+      if (c->have_xwin)
+        hipLaunchKernelGGL(k_spmv_sym<true>, dim3(sym_blocks(c)), dim3(kDiagThreads), 0,
+                           c->stream, c->n_nodes, off, int(c->n_diag), c->diag_pad_tiles,
+                           c->xwin, reinterpret_cast<const float4*>(c->diag_a.get()),
+                           reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
+                           c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
+      else
+        hipLaunchKernelGGL(k_spmv_sym<false>, dim3(sym_blocks(c)), dim3(kDiagThreads), 0,
+                           c->stream, c->n_nodes, off, int(c->n_diag), c->diag_pad_tiles,
+                           c->xwin, reinterpret_cast<const float4*>(c->diag_a.get()),
+                           reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
+                           c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
       return;
     }
     // the stencil-pattern view reads the off-diagonal copy of val (val32m)
