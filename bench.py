@@ -70,9 +70,9 @@ def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4, n_penalty=0):
 SPMV_EVENT_EVERY = 64
 
 # kernel name of the FP32 SpMV for each neighbour-index width
-SPMV32_KERNEL = {0: "k_spmv_sym", 1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
+SPMV32_KERNEL = {0: "k_spmv_sym<true>", 1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
                  4: "k_spmv<float, true, 0>"}
-SPMV32_FORM = {0: "FP32 symmetric block-diagonal SpMV, difference form",
+SPMV32_FORM = {0: "FP32 symmetric block-diagonal SpMV, x window in LDS, difference form",
                1: "FP32 stencil-pattern node-block SpMV, difference form",
                2: "FP32 node-block SpMV (16-bit offsets), difference form",
                4: "FP32 node-block SpMV (4-byte ids), difference form"}
