@@ -694,3 +694,27 @@ def test_fp64_block_view_bitwise_equals_csr(ctx, orc, kind):
     with pytest.raises(A.UfcError) as e:
         ctx.set_fp64_view(2)
     assert e.value.code == A.STATUS_INVALID
+
+
+@pytest.mark.parametrize("nx", [140, 150])
+def test_fp32_symdiag_x_window_limit(ctx, orc, nx):
+    """The symmetric SpMV stages its tile's x window in LDS when the 27
+    offsets fit in four runs of at most 1664 nodes: an nx x 3 x 3 beam (rows of
+    nx + 1 nodes) needs three runs of 2 (nx + 2) + 256 nodes, 1620 at nx = 140
+    (window) and 1680 at nx = 150 (global gathers).  Both forms match the
+    oracle's K x and solve to its displacements."""
+    from ufc_hip import _abi as A
+    from ufc_hip import problems as P
+    pr = P.cantilever(nx=nx, ny=3, nz=3, lx=nx / 4.0, ly=0.75, lz=0.75)
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    P.setup_context(ctx, pr, 2)
+    assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    x = np.random.default_rng(8).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    say("symdiag x window, nx", nx)
+    assert np.abs(ctx.spmv(x, precision=1) - yo).max() <= 1e-5 * np.abs(yo).max()
+    ctx.assemble_k(pr.young, pr.nu, precision=2)
+    u, info = ctx.pcg(precision=2, rtol=1e-12, max_iter=200000, replace_rtol=0.1)
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-14, max_iter=400000)
+    assert info.converged and np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
