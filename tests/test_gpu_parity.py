@@ -731,3 +731,57 @@ def test_fp32_symdiag_x_window_limit(ctx, orc, nx):
     say("wall iterations: oracle", it, "gpu inner", info.niter)
     assert st == 0 and info.converged
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+
+
+def test_degenerate_and_inverted_elements_vs_oracle(ctx, orc):
+    """Gauss points with |det J| <= 1e-12 are skipped (PH_Elem_C3D8.f90:775)
+    and a negative det J weighs its point negatively, as the reference's
+    integration does.  A 3x3x2 box whose middle node plane is moved onto the
+    bottom (the lower layer has zero volume: every point skipped, K_e = 0),
+    and one upper element listed with its faces swapped (det J < 0
+    everywhere): FP64 and FP32 values and both SpMVs against the oracle;
+    with both node planes at z = 0 clamped, the upper layer carries the
+    load and the FP64 solve converges to the oracle's displacements."""
+    import ufc_hip
+    from ufc_hip import problems as P
+    coords, conn = ufc_hip.mesh_box(3, 3, 2, 1.0, 1.0, 1.0)
+    coords = coords.copy()
+    coords[np.abs(coords[:, 2] - 0.5) < 1e-12, 2] = 0.0
+    pr = P.cube_pressure(1, name="degenerate")
+    pr.coords, pr.conn = coords, conn
+    pr.young = np.full(conn.shape[0], 1e9)
+    z0 = np.nonzero(coords[:, 2] == 0.0)[0]   # both node planes at z = 0
+    assert z0.size == 32
+    pr.bc_eq = (3 * z0[:, None] + np.arange(3)).ravel().astype(np.int32)
+    pr.bc_val = np.zeros(pr.bc_eq.size)
+    top = np.arange(9, 18, dtype=np.int32)
+    pr.load_elem, pr.load_face = top, np.full(top.size, 1, np.int32)
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    lower = orc.ke(coords[conn[0]], 1e9, 0.3)
+    assert np.abs(lower).max() == 0.0               # every point skipped
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    say("degenerate layer")
+    scale = np.abs(val).max()
+    assert np.abs(ctx.get_values(0) - val).max() <= 1e-12 * scale
+    x = np.random.default_rng(9).normal(size=pr.n_dof)
+    yo = orc.spmv(rp, ci, val, x)
+    assert np.abs(ctx.spmv(x, precision=0) - yo).max() <= 1e-12 * np.abs(yo).max()
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    assert np.abs(ctx.spmv(x, precision=1) - yo).max() <= 1e-5 * np.abs(yo).max()
+    uo, st, _, _ = orc.pcg(rp, ci, val, F, rtol=1e-13, max_iter=100000)
+    assert st == 0
+    u, info = ctx.pcg(precision=0, rtol=1e-13, max_iter=100000)
+    assert info.converged and np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-9
+    # one upper element with its bottom and top faces swapped: det J < 0
+    inv = conn.copy()
+    inv[13] = conn[13][[4, 5, 6, 7, 0, 1, 2, 3]]
+    pr.conn = inv
+    rp, ci, val, F, _ = _oracle_system(orc, pr)
+    assert np.trace(orc.ke(coords[inv[13]], 1e9, 0.3)) < 0.0
+    P.setup_context(ctx, pr, 0)
+    ctx.assemble_k(pr.young, pr.nu, precision=0)
+    assert np.abs(ctx.get_values(0) - val).max() <= 1e-12 * np.abs(val).max()
+    ctx.assemble_k(pr.young, pr.nu, precision=1)
+    yo = orc.spmv(rp, ci, val, x)
+    assert np.abs(ctx.spmv(x, precision=1) - yo).max() <= 1e-5 * np.abs(yo).max()
