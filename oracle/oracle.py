@@ -66,7 +66,7 @@ def lib():
         L.orc_csr_from_triplets.argtypes = [C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp]
         L.orc_assemble.restype = C.c_int
         L.orc_assemble.argtypes = [C.c_int32, vp, vp, vp, C.c_double, C.c_int32, vp, vp, vp,
-                                   C.c_int]
+                                   C.c_int, vp]
         L.orc_assemble_dgp.restype = C.c_int
         L.orc_assemble_dgp.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, vp, vp, vp, C.c_int]
         L.orc_apply_penalty.restype = C.c_int
@@ -252,13 +252,19 @@ def csr_from_triplets(n_rows, ti, tj, tv=None):
     return rp, ci, val
 
 
-def assemble(conn, coords, E, nu, rp, ci, nthreads=1):
+def assemble(conn, coords, E, nu, rp, ci, nthreads=1, allow_degenerate=False):
+    """CSR values of the elastic K on (rp, ci).  Elements with a Gauss point
+    of |det J| <= 1e-12 are an error unless allow_degenerate, in which case
+    the point is skipped as the reference skips it."""
     conn, coords, E = _i32(conn), _f64(coords), _f64(E)
     val = np.empty(ci.size)
+    degen = C.c_int32(0)
     bad = lib().orc_assemble(conn.shape[0], _p(conn), _p(coords), _p(E), nu, rp.size - 1,
-                             _p(rp), _p(ci), _p(val), nthreads)
+                             _p(rp), _p(ci), _p(val), nthreads, C.byref(degen))
     if bad:
-        raise RuntimeError(f"orc_assemble: {bad} degenerate Gauss points / missing slots")
+        raise RuntimeError(f"orc_assemble: {bad} element entries without a pattern slot")
+    if degen.value and not allow_degenerate:
+        raise RuntimeError(f"orc_assemble: {degen.value} elements with degenerate Gauss points")
     return val
 
 

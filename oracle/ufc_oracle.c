@@ -712,17 +712,17 @@ static void gather_coords(const int32_t* conn, const double* coords, int64_t e, 
  * reference (summation order then varies run to run). */
 int orc_assemble(int32_t n_elems, const int32_t* conn, const double* coords,
                  const double* E, double nu, int32_t n_rows, const int32_t* row_ptr,
-                 const int32_t* col_idx, double* val, int nthreads) {
+                 const int32_t* col_idx, double* val, int nthreads, int32_t* n_degenerate) {
   int64_t nnz = row_ptr[n_rows];
   memset(val, 0, sizeof(double) * (size_t)nnz);
-  int bad = 0;
+  int bad = 0, degen = 0;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : bad)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1) reduction(+ : bad, degen)
 #endif
   for (int64_t e = 0; e < n_elems; ++e) {
     double xe[24], Ke[576];
     gather_coords(conn, coords, e, xe);
-    if (!orc_c3d8_ke(xe, E[e], nu, Ke)) bad++;
+    if (!orc_c3d8_ke(xe, E[e], nu, Ke)) degen++;
     for (int j = 0; j < 24; ++j) {
       int32_t cj = 3 * conn[8 * e + j / 3] + j % 3;
       for (int i = 0; i < 24; ++i) {
@@ -740,6 +740,7 @@ int orc_assemble(int32_t n_elems, const int32_t* conn, const double* coords,
       }
     }
   }
+  if (n_degenerate) *n_degenerate = degen;
   return bad;
 }
 

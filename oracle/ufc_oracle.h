@@ -55,9 +55,13 @@ int64_t orc_csr_pattern(int32_t n_nodes, int32_t n_elems, const int32_t* conn,
 int64_t orc_csr_from_triplets(int32_t n_rows, int64_t n_trip, const int32_t* ti,
                               const int32_t* tj, const double* tv,
                               int32_t* row_ptr, int32_t* col_idx, double* val);
+/* returns the number of element entries with no slot in the pattern (an
+ * error); *n_degenerate (if not NULL) receives the number of elements with a
+ * Gauss point skipped for |det J| <= 1e-12, as PH_Elem_C3D8_StiffMatrix skips
+ * it (PH_Elem_C3D8.f90:775) */
 int  orc_assemble(int32_t n_elems, const int32_t* conn, const double* coords,
                   const double* E, double nu, int32_t n_rows, const int32_t* row_ptr,
-                  const int32_t* col_idx, double* val, int nthreads);
+                  const int32_t* col_idx, double* val, int nthreads, int32_t* n_degenerate);
 int  orc_assemble_dgp(int32_t n_elems, const int32_t* conn, const double* coords,
                       const double* Dgp, int32_t n_rows, const int32_t* row_ptr,
                       const int32_t* col_idx, double* val, int nthreads);

@@ -614,3 +614,49 @@ def test_reference_sparse_kernels_cantilever_match_oracle(orc):
     uo, st, it, _ = orc.pcg(rp, ci, val, F, rtol=1e-10)
     assert st == 0 and it == iters
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-10
+
+
+def test_degenerate_points_skipped_as_reference(orc):
+    """PH_Elem_C3D8_StiffMatrix skips a Gauss point with |det J| <= 1e-12
+    (PH_Elem_C3D8.f90:775): a zero-height element has K_e = 0; an element
+    collapsed to a wedge (two top nodes on the bottom) keeps the points
+    with a nonzero det J; the assembly reports degenerate elements and
+    skips their points only when asked to."""
+    flat = UNIT.copy()
+    flat[4:, 2] = 0.0
+    assert np.abs(orc.ke(flat, 1e9, 0.3)).max() == 0.0
+    wedge = UNIT.copy()
+    wedge[[5, 6], 2] = 0.0    # the x = 1 top edge on the bottom face
+    Kw = orc.ke(wedge, 1e9, 0.3)
+    # the points whose det J stays above 1e-12 contribute: independent sum
+    Kr = np.zeros((24, 24))
+    g = 1 / math.sqrt(3)
+    for xi in (-g, g):
+        for eta in (-g, g):
+            for zeta in (-g, g):
+                G, dj, ok = orc.dndx(wedge, xi, eta, zeta)
+                if not ok or abs(dj) <= 1e-12:
+                    continue
+                B = np.zeros((6, 24))
+                for n in range(8):
+                    gx, gy, gz = G[n]
+                    B[0, 3 * n], B[1, 3 * n + 1], B[2, 3 * n + 2] = gx, gy, gz
+                    B[3, 3 * n], B[3, 3 * n + 1] = gy, gx
+                    B[4, 3 * n + 1], B[4, 3 * n + 2] = gz, gy
+                    B[5, 3 * n], B[5, 3 * n + 2] = gz, gx
+                lam, mu = 1e9 * 0.3 / (1.3 * 0.4), 1e9 / 2.6
+                D = np.zeros((6, 6))
+                D[:3, :3] = lam
+                D[np.arange(3), np.arange(3)] += 2 * mu
+                D[np.arange(3, 6), np.arange(3, 6)] = mu
+                Kr += dj * B.T @ D @ B
+    assert np.abs(Kw - Kr).max() <= 1e-12 * np.abs(Kr).max()
+    coords = np.concatenate([UNIT, UNIT + [0, 0, 1.0]])
+    coords[4:8, 2] = 0.0      # the first element's top face on its bottom
+    conn = np.array([[0, 1, 2, 3, 4, 5, 6, 7], [4, 5, 6, 7, 12, 13, 14, 15]], np.int32)
+    coords[12:, 2] = 1.0
+    rp, ci = orc.csr_pattern(16, conn)
+    with pytest.raises(RuntimeError):
+        orc.assemble(conn, coords, np.full(2, 1e9), 0.3, rp, ci)
+    val = orc.assemble(conn, coords, np.full(2, 1e9), 0.3, rp, ci, allow_degenerate=True)
+    assert np.abs(val).max() > 0.0
