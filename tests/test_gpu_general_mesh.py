@@ -123,20 +123,36 @@ def test_valence10_deck_solves_vs_oracle(ctx, orc, tmp_path, precision):
     assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
 
 
-@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("variant", [2, 3, 4, 6])
 def test_general_mesh_element_variants(ctx, orc, variant):
-    """Reduced integration (2) and incompatible modes (4) on the polar mesh:
-    FP64 values against the oracle's variant assembly."""
+    """Reduced integration (2), selective (3), incompatible modes (4) and the
+    27-point rule (6) on the polar mesh: FP64 values against the oracle's
+    variant assembly (1e-12), FP32 values against its difference form
+    (1e-6), and the mixed solve against the oracle's displacements (1e-8)."""
     from ufc_hip import problems as P
     pr = _mesh("polar")
     rp, ci = orc.csr_pattern(pr.coords.shape[0], pr.conn)
     vo = orc.assemble_variant(pr.conn, pr.coords, pr.young, pr.nu, variant, rp, ci)
-    orc.apply_penalty(pr.bc_eq, pr.bc_val, 1e12, rp, ci, vo, None)
+    vd = vo.copy()
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    vd[(ci // 3) == (rows // 3)] = 0.0
+    orc.apply_penalty(pr.bc_eq, pr.bc_val, 1e12, rp, ci, vd, None)
+    F = np.zeros(pr.n_dof)
+    orc.face_loads_pressure(pr.load_elem, pr.load_face, pr.conn, pr.coords, pr.load_value[0], F)
+    orc.apply_penalty(pr.bc_eq, pr.bc_val, 1e12, rp, ci, vo, F)
     P.setup_context(ctx, pr, 0)
     say("general mesh variant", variant)
     ctx.assemble_k(pr.young, pr.nu, precision=0, variant=variant)
     v = ctx.get_values(0)
-    assert np.abs(v - vo).max() <= 1e-12 * np.abs(vo).max()
+    scale = np.abs(vo).max()
+    assert np.abs(v - vo).max() <= 1e-12 * scale
+    ctx.assemble_k(pr.young, pr.nu, precision=1, variant=variant)
+    assert np.abs(ctx.get_values(1).astype(np.float64) - vd).max() <= 1e-6 * scale
+    uo, st, _, _ = orc.pcg(rp, ci, vo, F, rtol=1e-14, max_iter=200000)
+    ctx.assemble_k(pr.young, pr.nu, precision=2, variant=variant)
+    u, info = ctx.pcg(precision=2, rtol=1e-12, max_iter=200000, replace_rtol=0.1)
+    assert st == 0 and info.converged
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
 
 
 def test_general_mesh_transpose_and_sensitivity(ctx, orc):
