@@ -328,7 +328,7 @@ struct ufc_ctx {
   ufc::DBuf<double> w_t;
   // reductions
   ufc::DBuf<double> partials;            // kSolveBlocks * 4
-  ufc::DBuf<double> spmv_partials;       // kSpmvBlocks32: p.Ap per FP32 SpMV workgroup
+  ufc::DBuf<double> spmv_partials;       // kSpmvBlocks32: p.Ap per SpMV (FP32) or k_dot_pap (FP64) workgroup
   ufc::DBuf<double> pap_piece;           // 8: p.Ap in eight ordered pieces
   ufc::DBuf<ufc::PcgScalars> scalars;    // 1
   ufc::PcgScalars* h_scalars = nullptr;  // pinned mirror

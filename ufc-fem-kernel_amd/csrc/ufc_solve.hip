@@ -822,7 +822,10 @@ __device__ __forceinline__ void report(int32_t* prog, int slot, int32_t v) {
 }
 
 // alpha of iteration it = (r.z of its iterate) / p.Ap, with p.Ap the ordered
-// sum of the kFinPapBlocks pieces (the same in every workgroup); p.Ap <= 0 or
+// sum of the kFinPapBlocks pieces (the same in every workgroup), or, nspart >
+// 0, the ordered sum of the nspart <= kSolveBlocks workgroup partials the
+// p.Ap pass wrote, which every workgroup adds itself (one launch less per
+// iteration; 64^3: 1088 partials, 8.7 KB from L2 per workgroup); p.Ap <= 0 or
 // not finite is a breakdown (done = 2, set by workgroup 0).  r -= alpha Ap
 // and the partials of r.r and r.z with z = r/diag; the iterate is updated by
 // the next pass (k_xpupdate), which reads p anyway.  Workgroup 0 leaves
@@ -835,8 +838,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
                                                          const T* __restrict__ diag,
                                                          double* __restrict__ partials,
                                                          const double* __restrict__ pap_piece,
-                                                         PcgScalars* __restrict__ sc, int it,
-                                                         int32_t* prog) {
+                                                         const double* __restrict__ spart,
+                                                         int nspart, PcgScalars* __restrict__ sc,
+                                                         int it, int32_t* prog) {
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (sc->done) {
     if (lead) sc->gate = 1;
@@ -856,8 +860,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
     load_vec(diag, tid * V, dv0);
   }
   double pap = 0.0;
+  if (nspart > 0) {
+    __shared__ double sh_pap;
+    double s[1];
+    sum_partials_bs<1, kSolveThreads, kSolveBlocks / kSolveThreads>(s, spart, nspart);
+    if (threadIdx.x == 0) sh_pap = s[0];
+    __syncthreads();
+    pap = sh_pap;
+  } else {
 #pragma unroll
-  for (int g = 0; g < kFinPapBlocks; ++g) pap += pap_piece[g];
+    for (int g = 0; g < kFinPapBlocks; ++g) pap += pap_piece[g];
+  }
   if (!(pap > 0.0) || !isfinite(pap)) {
     if (lead) {
       sc->pap = pap;
@@ -908,8 +921,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
 
 // x += alpha p (alpha of iteration it), then, with the finalize of iteration
 // it folded in, p = r/diag + beta p: every workgroup adds the update's r.r
-// and r.z partials itself (2 x 2048 doubles from L2, the same ordered sum in
-// every workgroup) and forms ||r||/||b||, the stop decision and
+// and r.z partials itself (2 x gridDim doubles from L2, at most 2 x 2048; the
+// update pass runs on the same grid; the same ordered sum in every workgroup) and forms ||r||/||b||, the stop decision and
 // beta = (r.z)_new / (r.z)_it.  The iterate is updated in every iteration
 // that ran, the stopping one included; p only while the solve goes on.
 // Workgroup 0 publishes iter, rnorm, rr, done and, while running, the new r.z
@@ -950,7 +963,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_xpupdate(int64_t n, X* __rest
     load_vec(diag, tid * V, dv0);
   }
   double s[2];
-  sum_partials_bs<2, kSolveThreads, kSolveBlocks / kSolveThreads>(s, partials, kSolveBlocks);
+  sum_partials_bs<2, kSolveThreads, kSolveBlocks / kSolveThreads>(s, partials, int(gridDim.x));
   __shared__ double sh_beta;
   __shared__ int sh_stop;
   if (threadIdx.x == 0) {
@@ -1246,6 +1259,17 @@ int spmv_blocks(const ufc_ctx* c) {
   return int(need < cap ? need : cap);
 }
 
+// grid of the per-iteration vector passes (p.Ap, update, x/p): kSolveBlocks
+// workgroups, fewer when the vector has fewer 16-byte chunks than that many
+// threads (64^3 FP32: 805), so that no workgroup only adds partials; a
+// function of n alone, so the partial sums keep their order run to run
+template <typename T>
+int vec_blocks(int64_t n) {
+  constexpr int V = 16 / sizeof(T);
+  const int64_t need = (n / V + kSolveThreads - 1) / kSolveThreads;
+  return int(std::max<int64_t>(1, std::min<int64_t>(need, kSolveBlocks)));
+}
+
 // symmetric block-diagonal SpMV grid: 8 x the tile positions per XCD, capped (a
 // multiple of 8 either way)
 int sym_blocks(const ufc_ctx* c) {
@@ -1364,26 +1388,32 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
                        p, r, diag, sc);
   UFC_HIP_CHECK(hipGetLastError());
   const int64_t first_launch = c->spmv_launches;
+  // p.Ap partials: written by the FP32 SpMV itself (its grid), or by
+  // k_dot_pap (FP64, the vector grid); at most kSolveBlocks of them are added
+  // by the update pass directly, more go through k_fin_pap's eight pieces
+  const int nvb = vec_blocks<T>(n);
+  double* spart = c->spmv_partials.get();
+  const int npap = BLOCK_INDEX ? fp32_spmv_blocks(c) : nvb;
+  const int fused = npap <= kSolveBlocks ? npap : 0;
   auto launch_iteration = [&](int it) {
     spmv_event_begin(c);
     if constexpr (BLOCK_INDEX) {
       // the FP32 SpMV forms the p.Ap partials itself
-      launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, c->spmv_partials.get());
+      launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true, spart);
       spmv_event_end(c);
-      hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream,
-                         c->spmv_partials.get(), fp32_spmv_blocks(c), pap_piece, sc);
     } else {
       launch_spmv<T, BLOCK_INDEX>(c, val, p, ap, true);
       spmv_event_end(c);
-      hipLaunchKernelGGL(k_dot_pap<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n,
-                         p, ap, part, sc);
-      hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream, part,
-                         kSolveBlocks, pap_piece, sc);
+      hipLaunchKernelGGL(k_dot_pap<T>, dim3(nvb), dim3(kSolveThreads), 0, c->stream, n, p, ap,
+                         spart, sc);
     }
-    hipLaunchKernelGGL(k_update<T>, dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream, n, r,
-                       ap, diag, part, pap_piece, sc, it, dprog);
-    hipLaunchKernelGGL((k_xpupdate<T, X>), dim3(kSolveBlocks), dim3(kSolveThreads), 0, c->stream,
-                       n, x, p, r, diag, part, sc, it, dprog);
+    if (!fused)
+      hipLaunchKernelGGL(k_fin_pap, dim3(kFinPapBlocks), dim3(kFinThreads), 0, c->stream, spart,
+                         npap, pap_piece, sc);
+    hipLaunchKernelGGL(k_update<T>, dim3(nvb), dim3(kSolveThreads), 0, c->stream, n, r, ap, diag,
+                       part, pap_piece, spart, fused, sc, it, dprog);
+    hipLaunchKernelGGL((k_xpupdate<T, X>), dim3(nvb), dim3(kSolveThreads), 0, c->stream, n, x, p,
+                       r, diag, part, sc, it, dprog);
   };
   if (flagged) {
     for (int launched = 0; launched < max_iter; ++launched) {
@@ -1444,8 +1474,8 @@ void ensure_solver_buffers(ufc_ctx* c, bool need64, bool need32) {
     c->s_p.resize(n);
     c->s_ap.resize(n);
     c->s_x.resize(n);
-    c->spmv_partials.resize(size_t(kSpmvBlocks32));
   }
+  c->spmv_partials.resize(size_t(kSpmvBlocks32));
   c->partials.resize(size_t(kMaxPartials) * 4);
   c->pap_piece.resize(size_t(kFinPapBlocks));
   if (!c->scalars.get()) {
