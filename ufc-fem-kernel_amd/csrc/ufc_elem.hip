@@ -1407,31 +1407,54 @@ __device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* s
 }
 
 // one thread per Gauss point: strain increment B(u - u_n), J2 update from the
-// committed state into the trial state and tangent
-__global__ __launch_bounds__(128) void k_j2_update(int32_t n_elems,
-                                                  const double* __restrict__ coords,
-                                                  const int32_t* __restrict__ conn,
-                                                  ufc_ph_j2_props pr, const double* __restrict__ u,
-                                                  const double* __restrict__ u_n,
-                                                  const double* __restrict__ commit,
-                                                  double* __restrict__ trial,
-                                                  double* __restrict__ tangent,
-                                                  int32_t* __restrict__ flags) {
-  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= int64_t(n_elems) * 8) return;
-  const int64_t e = idx >> 3;
-  const int gp = int(idx & 7);
+// committed state into the trial state and tangent.  A workgroup holds 16
+// whole elements (8 points each); their nodes' coordinates and u - u_n are
+// staged in LDS first, one node slot per lane (slot 8e + a is node a of
+// element e, the same index range as the points), so each lane gathers one
+// node instead of all eight of its element (0.327 -> 0.300 ms per update at
+// 64^3).  The state and tangent are read and written directly: staged through
+// LDS as well, their loads would all have to land before a barrier instead of
+// overlapping the Jacobian work (0.37 ms).
+constexpr int kJ2Threads = 128;
+__global__ __launch_bounds__(kJ2Threads) void k_j2_update(int32_t n_elems,
+                                                         const double* __restrict__ coords,
+                                                         const int32_t* __restrict__ conn,
+                                                         ufc_ph_j2_props pr,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ u_n,
+                                                         const double* __restrict__ commit,
+                                                         double* __restrict__ trial,
+                                                         double* __restrict__ tangent,
+                                                         int32_t* __restrict__ flags) {
+  static_assert(kJ2Threads % 8 == 0, "whole elements per workgroup");
+  __shared__ double sx[kJ2Threads][3];
+  __shared__ double sd[kJ2Threads][3];
+  const int64_t npts = int64_t(n_elems) * 8;
+  const int64_t idx = int64_t(blockIdx.x) * kJ2Threads + threadIdx.x;
+  const int t = int(threadIdx.x);
+  if (idx < npts) {
+    const int64_t n = conn[idx];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sx[t][k] = coords[3 * n + k];
+      sd[t][k] = u[3 * n + k] - u_n[3 * n + k];
+    }
+  }
+  __syncthreads();
+  if (idx >= npts) return;
+  const int e0 = t & ~7;  // the element's first slot
+  const int gp = t & 7;
   double x[8][3];
-  gather_coords(coords, conn, e, x);
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[a][k] = sx[e0 + a][k];
   double g[8][3], dj;
   double deps[6] = {0, 0, 0, 0, 0, 0};
   if (c3d8_dndx(x, gp, g, dj)) {
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-      const int64_t n = conn[8 * e + a];
-      const double d0 = u[3 * n] - u_n[3 * n];
-      const double d1 = u[3 * n + 1] - u_n[3 * n + 1];
-      const double d2 = u[3 * n + 2] - u_n[3 * n + 2];
+      const double d0 = sd[e0 + a][0], d1 = sd[e0 + a][1], d2 = sd[e0 + a][2];
       deps[0] += g[a][0] * d0;
       deps[1] += g[a][1] * d1;
       deps[2] += g[a][2] * d2;
@@ -1444,12 +1467,11 @@ __global__ __launch_bounds__(128) void k_j2_update(int32_t n_elems,
 #pragma unroll
   for (int k = 0; k < 13; ++k) st[k] = commit[13 * idx + k];
   double Dt[36];
-  const int r = j2_point(pr, deps, st, Dt);
+  flags[idx] = j2_point(pr, deps, st, Dt);
 #pragma unroll
   for (int k = 0; k < 13; ++k) trial[13 * idx + k] = st[k];
 #pragma unroll
   for (int k = 0; k < 36; ++k) tangent[36 * idx + k] = Dt[k];
-  flags[idx] = r;
 }
 
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
@@ -1764,8 +1786,8 @@ void launch_sensitivity(ufc_ctx* c, const double* u, const double* lam, const do
 
 void launch_j2_update(ufc_ctx* c, const double* u, const double* u_n) {
   const int64_t ngp = int64_t(c->n_elems) * 8;
-  hipLaunchKernelGGL(k_j2_update, dim3(blocks_for(ngp, 128)), dim3(128), 0, c->stream,
-                     c->n_elems, c->coords.get(), c->conn.get(), c->j2, u, u_n,
+  hipLaunchKernelGGL(k_j2_update, dim3(blocks_for(ngp, kJ2Threads)), dim3(kJ2Threads), 0,
+                     c->stream, c->n_elems, c->coords.get(), c->conn.get(), c->j2, u, u_n,
                      c->j2_commit.get(), c->j2_trial.get(), c->j2_tangent.get(),
                      c->j2_flags.get());
   UFC_HIP_CHECK(hipGetLastError());
