@@ -300,14 +300,15 @@ def cpu_second_opinion(repeats=3):
 
 
 # J2 update per Gauss point, algorithmic: committed state read and trial
-# state written (13 doubles each), tangent written (36 doubles), flag (4 B);
-# per element its connectivity, per node coordinates, u and u_n
-J2_BYTES_PER_GP = 13 * 8 * 2 + 36 * 8 + 4
+# state written (13 doubles each), the factored tangent written (mu', lam', c,
+# n(6): 9 doubles), flag (4 B); per element its connectivity, per node
+# coordinates, u and u_n
+J2_BYTES_PER_GP = 13 * 8 * 2 + 9 * 8 + 4
 # FLOPs per Gauss point: J = dN/dxi x (9 x 8 FMA = 144), det + inverse (50),
 # dN/dx = J^-T dN/dxi (8 x 3 x 5 = 120), strain increment from 8 nodes
 # (8 x 27 = 216), elastic predictor, deviator and q (40), radial return and
-# state update (40), consistent tangent (36 x 4 + 26 = 170)
-J2_FLOP_PER_GP = 144 + 50 + 120 + 216 + 40 + 40 + 170
+# state update (40), factored consistent tangent (10)
+J2_FLOP_PER_GP = 144 + 50 + 120 + 216 + 40 + 40 + 10
 
 
 def j2_roofline(n_elems, n_nodes, ms):

@@ -771,7 +771,7 @@ int ufc_ph_j2_init(ufc_ctx* c, const ufc_ph_j2_props* p, ufc_status* st) {
     c->j2 = law;
     c->j2_commit.resize(ngp * 13);
     c->j2_trial.resize(ngp * 13);
-    c->j2_tangent.resize(ngp * 36);
+    c->j2_tangent.resize(ngp * ufc::kJ2Tangent);
     c->j2_flags.resize(ngp);
     UFC_HIP_CHECK(hipMemsetAsync(c->j2_commit.get(), 0, c->j2_commit.bytes(), c->stream));
     UFC_HIP_CHECK(hipMemsetAsync(c->j2_trial.get(), 0, c->j2_trial.bytes(), c->stream));

@@ -107,32 +107,38 @@ __device__ __forceinline__ void gather_coords(const double* __restrict__ coords,
   }
 }
 
-// Block (a, b) of B_a^T D B_b for a general (6,6) column-major D.
+// The J2 consistent tangent of a Gauss point, stored factored as kJ2Tangent
+// doubles tc = (mu', lam', c, n(6)):
+//   Dt = lam' 1x1 + 2 mu' I_s - c n x n
+// (Voigt [xx,yy,zz,xy,yz,xz], engineering shear, I_s = diag(1,1,1,1/2,1/2,1/2)),
+// which is the form the radial return produces it in (elastic points: mu,
+// lam, c = 0): 72 bytes per point instead of the 288 of the 6x6 matrix, read
+// by the tangent assembly and the FP64 tangent residual, written by the update.
+// v = B(h)^T n for the B-matrix rows [hx,0,0], [0,hy,0], [0,0,hz], [hy,hx,0],
+// [0,hz,hy], [hz,0,hx].
 template <typename T>
-__device__ __forceinline__ void btdb_general(const T (&bd)[3][6], const T* gb, T w,
-                                             T (&k)[3][3]) {
-  const T bx = gb[0], by = gb[1], bz = gb[2];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    k[i][0] += w * (bd[i][0] * bx + bd[i][3] * by + bd[i][5] * bz);
-    k[i][1] += w * (bd[i][1] * by + bd[i][3] * bx + bd[i][4] * bz);
-    k[i][2] += w * (bd[i][2] * bz + bd[i][4] * by + bd[i][5] * bx);
-  }
+__device__ __forceinline__ void bt_n(const T* h, const T (&n)[6], T (&v)[3]) {
+  v[0] = h[0] * n[0] + h[1] * n[3] + h[2] * n[5];
+  v[1] = h[1] * n[1] + h[0] * n[3] + h[2] * n[4];
+  v[2] = h[2] * n[2] + h[1] * n[4] + h[0] * n[5];
 }
 
-// B_a^T D for column-major D (6,6): rows of B_a^T are
-// [gx,0,0,gy,0,gz], [0,gy,0,gx,gz,0], [0,0,gz,0,gy,gx].
-template <typename T>
-__device__ __forceinline__ void bt_d(const T* ga, const double* __restrict__ D, T (&bd)[3][6]) {
-  const T gx = ga[0], gy = ga[1], gz = ga[2];
+// sigma = Dt eps for the factored tangent tc
+__device__ __forceinline__ void j2_tangent_apply(const double* __restrict__ tc,
+                                                 const double (&eps)[6], double (&sig)[6]) {
+  const double mu = tc[0], lam = tc[1], c = tc[2];
+  double n[6];
 #pragma unroll
-  for (int m = 0; m < 6; ++m) {
-    const T d0 = T(D[0 + 6 * m]), d1 = T(D[1 + 6 * m]), d2 = T(D[2 + 6 * m]);
-    const T d3 = T(D[3 + 6 * m]), d4 = T(D[4 + 6 * m]), d5 = T(D[5 + 6 * m]);
-    bd[0][m] = gx * d0 + gy * d3 + gz * d5;
-    bd[1][m] = gy * d1 + gx * d3 + gz * d4;
-    bd[2][m] = gz * d2 + gy * d4 + gx * d5;
-  }
+  for (int i = 0; i < 6; ++i) n[i] = tc[3 + i];
+  double ne = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) ne += n[i] * eps[i];
+  ne *= c;
+  const double tr = lam * (eps[0] + eps[1] + eps[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) sig[i] = tr + 2.0 * mu * eps[i] - ne * n[i];
+#pragma unroll
+  for (int i = 3; i < 6; ++i) sig[i] = mu * eps[i] - ne * n[i];
 }
 
 // ga = row a of the (8, 3) register table g without dynamic register
@@ -686,7 +692,9 @@ __device__ __forceinline__ void elem_node_rows(const double* __restrict__ coords
                       VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * elem : nullptr, kb);
   } else {
     // sum_gp detJ B_a^T D_gp B_b = sum_gp (1 / d) H_a^T D_gp H_b with the
-    // B-matrices H of h = d dN/dx (one division per point, no gradient table)
+    // B-matrices H of h = d dN/dx (one division per point, no gradient
+    // table); with the factored tangent each block is the isotropic one of
+    // (lam', mu') less the rank-one c v_a v_b^T, v = H^T n
     const TriMap<T> m = tri_map(x);
     T sx, sy, sz;
     node_signs(a, sx, sy, sz);
@@ -703,14 +711,28 @@ __device__ __forceinline__ void elem_node_rows(const double* __restrict__ coords
       const T r = T(1) / d;
 #pragma unroll
       for (int i = 0; i < 3; ++i) ha[i] *= r;
-      T bd[3][6];
-      bt_d(ha, dgp + 36 * (8 * int64_t(elem) + gp), bd);
+      const double* tc = dgp + kJ2Tangent * (8 * int64_t(elem) + gp);
+      const T mu = T(tc[0]), lam = T(tc[1]), cn = T(tc[2]);
+      T n[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) n[i] = T(tc[3 + i]);
+      T va[3];
+      bt_n(ha, n, va);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) va[i] *= cn;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        T dnb[3], hb[3];
+        T dnb[3], hb[3], vb[3];
         node_dn(T(XS[b]), T(YS[b]), T(ZS[b]), xi, eta, zeta, dnb);
         adj_t(A, dnb, hb);
-        btdb_general(bd, hb, T(1), kb[b]);
+        bt_n(hb, n, vb);
+        const T dot = mu * (ha[0] * hb[0] + ha[1] * hb[1] + ha[2] * hb[2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            kb[b][i][j] += lam * ha[i] * hb[j] + mu * hb[i] * ha[j] - va[i] * vb[j] +
+                           (i == j ? dot : T(0));
       }
     }
   }
@@ -1052,55 +1074,74 @@ __global__ void k_penalty(int32_t n_bc, const int32_t* __restrict__ eq,
 //   mode 0: D(E_e, nu) B u_e            (elastic K_e u_e)
 //   mode 1: D_gp B u_e                  (J2 tangent K_e u_e)
 //   mode 2: the J2 trial stress         (internal force, RT_Asm_ComputeResidual)
-// one thread per element, result in w[24 e + j].
+// result in w[24 e + j].  Eight lanes per element, lane = Gauss point: a
+// workgroup's 16 elements first stage their nodes' coordinates and u in LDS,
+// one node slot per lane (slot 8e + a = node a of element e, the same index
+// range as the points), each lane forms its point's 24 force terms, and the
+// eight lanes sum them in a fixed order of halving exchanges (nodes 4-7 <->
+// 0-3 with the lane 4 apart, then 2 apart, then 1 apart), after which lane a
+// holds node a's three sums and stores them (one thread per element looping
+// over the points took 0.29 ms at 128^3 at one wave per SIMD).
+constexpr int kForceThreads = 128;
 template <int MODE>
-__global__ __launch_bounds__(128) void k_elem_force(int32_t n_elems,
-                                                   const double* __restrict__ coords,
-                                                   const int32_t* __restrict__ conn,
-                                                   const double* __restrict__ young, double nu,
-                                                   const double* __restrict__ dgp,
-                                                   const double* __restrict__ stress,
-                                                   const double* __restrict__ u,
-                                                   double* __restrict__ w) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n_elems) return;
-  double x[8][3], ue[8][3], fe[8][3];
-  gather_coords(coords, conn, e, x);
+__global__ __launch_bounds__(kForceThreads) void k_elem_force(int32_t n_elems,
+                                                             const double* __restrict__ coords,
+                                                             const int32_t* __restrict__ conn,
+                                                             const double* __restrict__ young,
+                                                             double nu,
+                                                             const double* __restrict__ dgp,
+                                                             const double* __restrict__ stress,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ w) {
+  static_assert(kForceThreads % 8 == 0, "whole elements per workgroup");
+  __shared__ double sx[kForceThreads][3];
+  __shared__ double su[MODE == 2 ? 1 : kForceThreads][3];
+  const int64_t npts = int64_t(n_elems) * 8;
+  const int64_t idx = int64_t(blockIdx.x) * kForceThreads + threadIdx.x;
+  const int t = int(threadIdx.x);
+  if (idx < npts) {
+    const int64_t n = conn[idx];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int64_t n = conn[8 * e + a];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      ue[a][d] = (MODE == 2) ? 0.0 : u[3 * n + d];
-      fe[a][d] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      sx[t][k] = coords[3 * n + k];
+      if constexpr (MODE != 2) su[t][k] = u[3 * n + k];
     }
   }
-  double lam = 0.0, mu = 0.0;
-  if (MODE == 0) {
-    const double E = young[e];
-    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    mu = E / (2.0 * (1.0 + nu));
-  }
-  for (int gp = 0; gp < 8; ++gp) {
-    double g[8][3], dj;
-    if (!c3d8_dndx(x, gp, g, dj)) continue;
+  __syncthreads();
+  if (idx >= npts) return;  // whole elements: npts and the workgroup are multiples of 8
+  const int64_t e = idx >> 3;
+  const int gp = t & 7, e0 = t & ~7;
+  double x[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[a][k] = sx[e0 + a][k];
+  double fe[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) fe[a][0] = fe[a][1] = fe[a][2] = 0.0;
+  double g[8][3], dj;
+  if (c3d8_dndx(x, gp, g, dj)) {
     double sig[6];
-    if (MODE == 2) {
-      const double* sp = stress + 13 * (8 * e + gp);
+    if constexpr (MODE == 2) {
+      const double* sp = stress + 13 * idx;
 #pragma unroll
       for (int m = 0; m < 6; ++m) sig[m] = sp[m];
     } else {
       double eps[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
-        eps[0] += g[a][0] * ue[a][0];
-        eps[1] += g[a][1] * ue[a][1];
-        eps[2] += g[a][2] * ue[a][2];
-        eps[3] += g[a][1] * ue[a][0] + g[a][0] * ue[a][1];
-        eps[4] += g[a][2] * ue[a][1] + g[a][1] * ue[a][2];
-        eps[5] += g[a][2] * ue[a][0] + g[a][0] * ue[a][2];
+        const double u0 = su[e0 + a][0], u1 = su[e0 + a][1], u2 = su[e0 + a][2];
+        eps[0] += g[a][0] * u0;
+        eps[1] += g[a][1] * u1;
+        eps[2] += g[a][2] * u2;
+        eps[3] += g[a][1] * u0 + g[a][0] * u1;
+        eps[4] += g[a][2] * u1 + g[a][1] * u2;
+        eps[5] += g[a][2] * u0 + g[a][0] * u2;
       }
-      if (MODE == 0) {
+      if constexpr (MODE == 0) {
+        const double E = young[e];
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        const double mu = E / (2.0 * (1.0 + nu));
         const double tr = lam * (eps[0] + eps[1] + eps[2]);
         sig[0] = tr + 2.0 * mu * eps[0];
         sig[1] = tr + 2.0 * mu * eps[1];
@@ -1109,28 +1150,46 @@ __global__ __launch_bounds__(128) void k_elem_force(int32_t n_elems,
         sig[4] = mu * eps[4];
         sig[5] = mu * eps[5];
       } else {
-        const double* D = dgp + 36 * (8 * e + gp);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          double s = 0.0;
-#pragma unroll
-          for (int m = 0; m < 6; ++m) s += D[i + 6 * m] * eps[m];
-          sig[i] = s;
-        }
+        j2_tangent_apply(dgp + kJ2Tangent * idx, eps, sig);
       }
     }
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-      fe[a][0] += dj * (g[a][0] * sig[0] + g[a][1] * sig[3] + g[a][2] * sig[5]);
-      fe[a][1] += dj * (g[a][1] * sig[1] + g[a][0] * sig[3] + g[a][2] * sig[4]);
-      fe[a][2] += dj * (g[a][2] * sig[2] + g[a][1] * sig[4] + g[a][0] * sig[5]);
+      fe[a][0] = dj * (g[a][0] * sig[0] + g[a][1] * sig[3] + g[a][2] * sig[5]);
+      fe[a][1] = dj * (g[a][1] * sig[1] + g[a][0] * sig[3] + g[a][2] * sig[4]);
+      fe[a][2] = dj * (g[a][2] * sig[2] + g[a][1] * sig[4] + g[a][0] * sig[5]);
     }
   }
-  double* out = w + 24 * e;
+  // halving exchanges: keep the half of the nodes whose index bit matches the
+  // lane's, send the other half to the partner lane and add what it sends
+  const bool b2 = gp & 4, b1 = gp & 2, b0 = gp & 1;
+  double h1[4][3], h2[2][3], h3[3];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int d = 0; d < 3; ++d) out[3 * a + d] = fe[a][d];
+    for (int d = 0; d < 3; ++d) {
+      const double keep = b2 ? fe[a + 4][d] : fe[a][d];
+      const double send = b2 ? fe[a][d] : fe[a + 4][d];
+      h1[a][d] = keep + __shfl_xor(send, 4, 8);
+    }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double keep = b1 ? h1[a + 2][d] : h1[a][d];
+      const double send = b1 ? h1[a][d] : h1[a + 2][d];
+      h2[a][d] = keep + __shfl_xor(send, 2, 8);
+    }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double keep = b0 ? h2[1][d] : h2[0][d];
+    const double send = b0 ? h2[0][d] : h2[1][d];
+    h3[d] = keep + __shfl_xor(send, 1, 8);
+  }
+  double* out = w + 24 * e + 3 * gp;
+  out[0] = h3[0];
+  out[1] = h3[1];
+  out[2] = h3[2];
 }
 
 // y[3n+d] = sum over n's elements of w[24 e + 3 a + d]: eight lanes per node,
@@ -1315,7 +1374,8 @@ __device__ __forceinline__ double j2_hard_tan(const ufc_ph_j2_props& p, double e
 // the tangent is the consistent tangent
 //   Dt = K 1x1 + 2G th1 (I_s - 1x1/3) - 2G (th1 + th2 - 1) n x n
 // (th1 = 1 - 3G dg/q, th2 = 3G/(3G + H'(peeq_n + dg)),
-// I_s = diag(1,1,1,1/2,1/2,1/2)), which replaces the reference's :509-537.
+// I_s = diag(1,1,1,1/2,1/2,1/2)), which replaces the reference's :509-537,
+// kept factored as tc = (G th1, K - 2G th1/3, 2G (th1 + th2 - 1), n).
 // A point that has yielded and whose trial state lies on the yield surface
 // (f > -1e-8 sigma_y0: the zero strain increment of an increment's first
 // Newton iteration) gets the plastic branch's tangent at dg = 0 (th1 = 1,
@@ -1323,22 +1383,18 @@ __device__ __forceinline__ double j2_hard_tan(const ufc_ph_j2_props& p, double e
 // D_el, which would make that iteration an elastic predictor; the stress is
 // the trial stress either way.
 // st: sigma(6), peeq, eps_p(6).  Returns 0 elastic, 1 plastic, 2 failure
-// (degenerate trial state, zero Jacobian or no local convergence: st and Dt
+// (degenerate trial state, zero Jacobian or no local convergence: st and tc
 // are then not meaningful).
 __device__ __forceinline__ void j2_tangent(double K, double G, double th1, double th2,
-                                           const double (&n)[6], double* Dt) {
-  const double cn = 2.0 * G * (th1 + th2 - 1.0);
+                                           const double (&n)[6], double* tc) {
+  tc[0] = G * th1;
+  tc[1] = K - 2.0 * G * th1 / 3.0;
+  tc[2] = 2.0 * G * (th1 + th2 - 1.0);
 #pragma unroll
-  for (int j = 0; j < 6; ++j)
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double v = (i < 3 && j < 3) ? K - 2.0 * G * th1 / 3.0 : 0.0;
-      if (i == j) v += 2.0 * G * th1 * (i < 3 ? 1.0 : 0.5);
-      Dt[i + 6 * j] = v - cn * n[i] * n[j];
-    }
+  for (int i = 0; i < 6; ++i) tc[3 + i] = n[i];
 }
 
-__device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* st, double* Dt) {
+__device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* st, double* tc) {
   const double E = pr.young, nu = pr.nu;
   const double mu = E / (2.0 * (1.0 + nu));
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
@@ -1365,13 +1421,12 @@ __device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* s
       const double sn0 = q / sqrt(1.5);
 #pragma unroll
       for (int i = 0; i < 6; ++i) n0[i] = s[i] / sn0;
-      j2_tangent(K, G, 1.0, 3.0 * G / (3.0 * G + j2_hard_tan(pr, peeq_n)), n0, Dt);
+      j2_tangent(K, G, 1.0, 3.0 * G / (3.0 * G + j2_hard_tan(pr, peeq_n)), n0, tc);
     } else {
+      tc[0] = mu;
+      tc[1] = lam;
 #pragma unroll
-      for (int j = 0; j < 6; ++j)
-#pragma unroll
-        for (int i = 0; i < 6; ++i)
-          Dt[i + 6 * j] = (i < 3 && j < 3 ? lam : 0.0) + (i == j ? (i < 3 ? 2.0 * mu : mu) : 0.0);
+      for (int i = 2; i < kJ2Tangent; ++i) tc[i] = 0.0;
     }
     return 0;
   }
@@ -1402,7 +1457,7 @@ __device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* s
   st[6] = peeq_n + dg;
 #pragma unroll
   for (int i = 0; i < 6; ++i) st[7 + i] += dg * n[i];
-  j2_tangent(K, G, beta, 3.0 * G / (3.0 * G + j2_hard_tan(pr, st[6])), n, Dt);
+  j2_tangent(K, G, beta, 3.0 * G / (3.0 * G + j2_hard_tan(pr, st[6])), n, tc);
   return 1;
 }
 
@@ -1466,12 +1521,12 @@ __global__ __launch_bounds__(kJ2Threads) void k_j2_update(int32_t n_elems,
   double st[13];
 #pragma unroll
   for (int k = 0; k < 13; ++k) st[k] = commit[13 * idx + k];
-  double Dt[36];
-  flags[idx] = j2_point(pr, deps, st, Dt);
+  double tc[kJ2Tangent];
+  flags[idx] = j2_point(pr, deps, st, tc);
 #pragma unroll
   for (int k = 0; k < 13; ++k) trial[13 * idx + k] = st[k];
 #pragma unroll
-  for (int k = 0; k < 36; ++k) tangent[36 * idx + k] = Dt[k];
+  for (int k = 0; k < kJ2Tangent; ++k) tangent[kJ2Tangent * idx + k] = tc[k];
 }
 
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
@@ -1746,19 +1801,19 @@ void launch_penalty(ufc_ctx* c, bool fp32, double sign) {
 
 void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y) {
   c->elem_w.resize(size_t(c->n_elems) * 24);
-  const unsigned grid = blocks_for(c->n_elems, 128);
+  const unsigned grid = blocks_for(int64_t(c->n_elems) * 8, kForceThreads);
   if (mode == 0 && c->variant != kVarFull)
     apply_variant<false>(c, u, nullptr, nullptr, c->elem_w.get());
   else if (mode == 0)
-    hipLaunchKernelGGL(k_elem_force<0>, dim3(grid), dim3(128), 0, c->stream, c->n_elems,
+    hipLaunchKernelGGL(k_elem_force<0>, dim3(grid), dim3(kForceThreads), 0, c->stream, c->n_elems,
                        c->coords.get(), c->conn.get(), c->young.get(), c->nu, nullptr, nullptr, u,
                        c->elem_w.get());
   else if (mode == 1)
-    hipLaunchKernelGGL(k_elem_force<1>, dim3(grid), dim3(128), 0, c->stream, c->n_elems,
+    hipLaunchKernelGGL(k_elem_force<1>, dim3(grid), dim3(kForceThreads), 0, c->stream, c->n_elems,
                        c->coords.get(), c->conn.get(), nullptr, 0.0, c->j2_tangent.get(), nullptr,
                        u, c->elem_w.get());
   else
-    hipLaunchKernelGGL(k_elem_force<2>, dim3(grid), dim3(128), 0, c->stream, c->n_elems,
+    hipLaunchKernelGGL(k_elem_force<2>, dim3(grid), dim3(kForceThreads), 0, c->stream, c->n_elems,
                        c->coords.get(), c->conn.get(), nullptr, 0.0, nullptr, c->j2_trial.get(),
                        nullptr, c->elem_w.get());
   UFC_HIP_CHECK(hipGetLastError());

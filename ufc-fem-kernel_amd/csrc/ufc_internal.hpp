@@ -92,6 +92,9 @@ constexpr int kMaxNodeNbrs = 32;    // neighbours per node of the slot map (5-bi
 constexpr int kMaxGenNbrs = 256;
 constexpr int kMaxPatterns = 256;   // stencil patterns of the FP32 view (one-byte ids)
 constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP32 view
+// doubles per Gauss point of the stored J2 tangent (mu', lam', c, n(6)):
+// Dt = lam' 1x1 + 2 mu' I_s - c n x n
+constexpr int kJ2Tangent = 9;
 constexpr int kDiagThreads = 256;   // symmetric FP32 SpMV: one node per thread
 constexpr int kDiagTile = 256;      // nodes per tile of the block-diagonal layout (= kDiagThreads)
 // tiles one "plane" (ceil(o_last / kDiagTile) tiles) apart that run in a row
@@ -316,7 +319,7 @@ struct ufc_ctx {
   ufc_ph_j2_props j2{};
   ufc::DBuf<double> j2_commit;   // (13, 8, n_elems)
   ufc::DBuf<double> j2_trial;    // (13, 8, n_elems)
-  ufc::DBuf<double> j2_tangent;  // (36, 8, n_elems)
+  ufc::DBuf<double> j2_tangent;  // (kJ2Tangent, 8, n_elems): factored consistent tangent
   ufc::DBuf<int32_t> j2_flags;   // (8, n_elems) 1 = plastic branch, 2 = failure
 
   // solver work (FP64)
