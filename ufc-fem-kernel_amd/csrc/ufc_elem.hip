@@ -1070,30 +1070,30 @@ __global__ void k_penalty(int32_t n_bc, const int32_t* __restrict__ eq,
 }
 
 // ---------------------------------------------------------------- element forces
-// Fe = sum_gp w detJ B^T sigma_gp with sigma from
-//   mode 0: D(E_e, nu) B u_e            (elastic K_e u_e)
-//   mode 1: D_gp B u_e                  (J2 tangent K_e u_e)
-//   mode 2: the J2 trial stress         (internal force, RT_Asm_ComputeResidual)
-// result in w[24 e + j].  Eight lanes per element, lane = Gauss point: a
-// workgroup's 16 elements first stage their nodes' coordinates and u in LDS,
-// one node slot per lane (slot 8e + a = node a of element e, the same index
-// range as the points), each lane forms its point's 24 force terms, and the
-// eight lanes sum them in a fixed order of halving exchanges (nodes 4-7 <->
-// 0-3 with the lane 4 apart, then 2 apart, then 1 apart), after which lane a
-// holds node a's three sums and stores them (one thread per element looping
-// over the points took 0.29 ms at 128^3 at one wave per SIMD).
+// Fe = sum_gp w detJ B^T sigma_gp, result in w[24 e + j], with sigma from
+//   mode 0: D(E_e, nu) B u_e            (elastic K_e u_e: k_elem_force_elastic)
+//   mode 1: D_gp B u_e                  (J2 tangent K_e u_e: k_elem_force<1>)
+//   mode 2: the J2 trial stress         (internal force, RT_Asm_ComputeResidual:
+//                                        k_elem_force<2>)
+// The J2 modes: eight lanes per element, lane = Gauss point.  A workgroup's
+// 16 elements first stage their nodes' coordinates and u in LDS, one node slot
+// per lane (slot 8e + a = node a of element e, the same index range as the
+// points), each lane forms its point's 24 force terms, and the eight lanes
+// sum them in a fixed order of halving exchanges (nodes 4-7 <-> 0-3 with the
+// lane 4 apart, then 2 apart, then 1 apart), after which lane a holds node
+// a's three sums and stores them.  64^3: mode 1 57 us, mode 2 60 us against
+// 71 and 75 us for one thread per element.
 constexpr int kForceThreads = 128;
 template <int MODE>
 __global__ __launch_bounds__(kForceThreads) void k_elem_force(int32_t n_elems,
                                                              const double* __restrict__ coords,
                                                              const int32_t* __restrict__ conn,
-                                                             const double* __restrict__ young,
-                                                             double nu,
                                                              const double* __restrict__ dgp,
                                                              const double* __restrict__ stress,
                                                              const double* __restrict__ u,
                                                              double* __restrict__ w) {
   static_assert(kForceThreads % 8 == 0, "whole elements per workgroup");
+  static_assert(MODE == 1 || MODE == 2, "the J2 modes");
   __shared__ double sx[kForceThreads][3];
   __shared__ double su[MODE == 2 ? 1 : kForceThreads][3];
   const int64_t npts = int64_t(n_elems) * 8;
@@ -1138,20 +1138,7 @@ __global__ __launch_bounds__(kForceThreads) void k_elem_force(int32_t n_elems,
         eps[4] += g[a][2] * u1 + g[a][1] * u2;
         eps[5] += g[a][2] * u0 + g[a][0] * u2;
       }
-      if constexpr (MODE == 0) {
-        const double E = young[e];
-        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-        const double mu = E / (2.0 * (1.0 + nu));
-        const double tr = lam * (eps[0] + eps[1] + eps[2]);
-        sig[0] = tr + 2.0 * mu * eps[0];
-        sig[1] = tr + 2.0 * mu * eps[1];
-        sig[2] = tr + 2.0 * mu * eps[2];
-        sig[3] = mu * eps[3];
-        sig[4] = mu * eps[4];
-        sig[5] = mu * eps[5];
-      } else {
-        j2_tangent_apply(dgp + kJ2Tangent * idx, eps, sig);
-      }
+      j2_tangent_apply(dgp + kJ2Tangent * idx, eps, sig);
     }
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -1190,6 +1177,63 @@ __global__ __launch_bounds__(kForceThreads) void k_elem_force(int32_t n_elems,
   out[0] = h3[0];
   out[1] = h3[1];
   out[2] = h3[2];
+}
+
+// Mode 0 (elastic K_e u_e): one thread per element looping over its 8
+// points.  The eight-lane form above re-reads the element's 48 node values
+// from LDS in every lane and is slower here at every size (48^3 to 128^3:
+// 0.029 / 0.056 / 0.177 / 0.410 ms against 0.028 / 0.048 / 0.138 / 0.306).
+__global__ __launch_bounds__(128) void k_elem_force_elastic(int32_t n_elems,
+                                                           const double* __restrict__ coords,
+                                                           const int32_t* __restrict__ conn,
+                                                           const double* __restrict__ young,
+                                                           double nu,
+                                                           const double* __restrict__ u,
+                                                           double* __restrict__ w) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_elems) return;
+  double x[8][3], ue[8][3], fe[8][3];
+  gather_coords(coords, conn, e, x);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t n = conn[8 * e + a];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      ue[a][d] = u[3 * n + d];
+      fe[a][d] = 0.0;
+    }
+  }
+  const double E = young[e];
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E / (2.0 * (1.0 + nu));
+  for (int gp = 0; gp < 8; ++gp) {
+    double g[8][3], dj;
+    if (!c3d8_dndx(x, gp, g, dj)) continue;
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      eps[0] += g[a][0] * ue[a][0];
+      eps[1] += g[a][1] * ue[a][1];
+      eps[2] += g[a][2] * ue[a][2];
+      eps[3] += g[a][1] * ue[a][0] + g[a][0] * ue[a][1];
+      eps[4] += g[a][2] * ue[a][1] + g[a][1] * ue[a][2];
+      eps[5] += g[a][2] * ue[a][0] + g[a][0] * ue[a][2];
+    }
+    const double tr = lam * (eps[0] + eps[1] + eps[2]);
+    const double sig[6] = {tr + 2.0 * mu * eps[0], tr + 2.0 * mu * eps[1],
+                           tr + 2.0 * mu * eps[2], mu * eps[3], mu * eps[4], mu * eps[5]};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      fe[a][0] += dj * (g[a][0] * sig[0] + g[a][1] * sig[3] + g[a][2] * sig[5]);
+      fe[a][1] += dj * (g[a][1] * sig[1] + g[a][0] * sig[3] + g[a][2] * sig[4]);
+      fe[a][2] += dj * (g[a][2] * sig[2] + g[a][1] * sig[4] + g[a][0] * sig[5]);
+    }
+  }
+  double* out = w + 24 * e;
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[3 * a + d] = fe[a][d];
 }
 
 // y[3n+d] = sum over n's elements of w[24 e + 3 a + d]: eight lanes per node,
@@ -1801,20 +1845,20 @@ void launch_penalty(ufc_ctx* c, bool fp32, double sign) {
 
 void launch_apply_elem64(ufc_ctx* c, int mode, const double* u, double* y) {
   c->elem_w.resize(size_t(c->n_elems) * 24);
-  const unsigned grid = blocks_for(int64_t(c->n_elems) * 8, kForceThreads);
+  const unsigned gpt = blocks_for(int64_t(c->n_elems) * 8, kForceThreads);
   if (mode == 0 && c->variant != kVarFull)
     apply_variant<false>(c, u, nullptr, nullptr, c->elem_w.get());
   else if (mode == 0)
-    hipLaunchKernelGGL(k_elem_force<0>, dim3(grid), dim3(kForceThreads), 0, c->stream, c->n_elems,
-                       c->coords.get(), c->conn.get(), c->young.get(), c->nu, nullptr, nullptr, u,
-                       c->elem_w.get());
+    hipLaunchKernelGGL(k_elem_force_elastic, dim3(blocks_for(c->n_elems, 128)), dim3(128), 0,
+                       c->stream, c->n_elems, c->coords.get(), c->conn.get(), c->young.get(),
+                       c->nu, u, c->elem_w.get());
   else if (mode == 1)
-    hipLaunchKernelGGL(k_elem_force<1>, dim3(grid), dim3(kForceThreads), 0, c->stream, c->n_elems,
-                       c->coords.get(), c->conn.get(), nullptr, 0.0, c->j2_tangent.get(), nullptr,
+    hipLaunchKernelGGL(k_elem_force<1>, dim3(gpt), dim3(kForceThreads), 0, c->stream,
+                       c->n_elems, c->coords.get(), c->conn.get(), c->j2_tangent.get(), nullptr,
                        u, c->elem_w.get());
   else
-    hipLaunchKernelGGL(k_elem_force<2>, dim3(grid), dim3(kForceThreads), 0, c->stream, c->n_elems,
-                       c->coords.get(), c->conn.get(), nullptr, 0.0, nullptr, c->j2_trial.get(),
+    hipLaunchKernelGGL(k_elem_force<2>, dim3(gpt), dim3(kForceThreads), 0, c->stream,
+                       c->n_elems, c->coords.get(), c->conn.get(), nullptr, c->j2_trial.get(),
                        nullptr, c->elem_w.get());
   UFC_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_node_gather, dim3(blocks_for(int64_t(c->n_nodes) * 8, 256)), dim3(256), 0,
