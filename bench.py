@@ -466,6 +466,73 @@ def copy_bandwidth(torch, dev, nbytes=1 << 31):
     return 2.0 * nbytes / (ms * 1e6)
 
 
+def run_j2(args, out, P, ufc_hip, torch):
+    """Config 5: the 64^3 jittered J2 job, inexact and exact Newton, into out["j2"]."""
+    import numpy as np
+    from ufc_hip.nonlinear import j2_static
+    nj = args.j2_size
+    log(f"J2 {nj}^3 jittered, 10 increments to 0.5% strain")
+    pj = P.j2_cube(nj)
+    target = np.zeros(pj.bc_eq.size)
+    target[pj.extra["n_clamp"]:] = pj.extra["strain"]
+    with ufc_hip.Context(0) as cj:
+        cj.build_pattern(pj.coords, pj.conn)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
+                        precision=ufc_hip.PREC_MIXED)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        # exact Newton (linear solves to 1e-10) on the same job: the
+        # consistent tangent's iteration count per increment
+        cj.build_pattern(pj.coords, pj.conn)
+        t2 = time.perf_counter()
+        res_ex = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
+                           precision=ufc_hip.PREC_MIXED, lin_rtol=1e-10)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+    if not (res.converged and res_ex.converged):
+        print(json.dumps({"metric": METRIC, "error": "the J2 Newton job did not converge"}),
+              flush=True)
+        sys.exit(3)
+    out["j2"] = {"workload": (f"{nj}^3 C3D8 cube, interior nodes jittered +-10% h, "
+                              "E 200e9, nu 0.3, sigma_y 250e6, H 2e9, u_z on z=1 to "
+                              "0.5% strain in 10 increments; inexact Newton on the "
+                              "consistent tangent to |R| <= 1e-8 |F_int| (linear tolerance "
+                              "1e-6 on the first iteration, then Eisenstat-Walker <= 1e-3), "
+                              "inner solve FP32-in-FP64"),
+                 "n_dof": pj.n_dof, "converged": res.converged, "total_ms": (t1 - t0) * 1e3,
+                 "newton_iters": [i.newton_iters for i in res.increments],
+                 "linear_iters": int(sum(i.linear_iters for i in res.increments)),
+                 "yielded_gauss_points_final": int(res.increments[-1].n_yield)
+                 if res.increments else 0,
+                 "exact_newton": {"linear_rtol": 1e-10, "total_ms": (t3 - t2) * 1e3,
+                                  "newton_iters": [i.newton_iters for i in res_ex.increments],
+                                  "rnorm_rel": [i.rnorm_rel for i in res_ex.increments],
+                                  "history_last_increment":
+                                      res_ex.increments[-1].history,
+                                  "linear_iters": int(sum(i.linear_iters
+                                                          for i in res_ex.increments))},
+                 "u_exact_vs_inexact_rel": float(np.linalg.norm(res.u - res_ex.u) /
+                                                 np.linalg.norm(res_ex.u))}
+    log(f"J2: {(t1 - t0):.2f} s, newton {out['j2']['newton_iters']}")
+    upd = res.j2_update_ms + res_ex.j2_update_ms
+    out["j2"]["roofline"] = j2_roofline(pj.n_elems, pj.coords.shape[0],
+                                        float(np.median(upd)))
+    out["j2"]["roofline"]["launches"] = len(upd)
+    out["j2"]["tangent_assembly_ms_median"] = float(np.median(res.assemble_ms))
+    log(f"J2 update {np.median(upd):.3f} ms, tangent assembly "
+        f"{np.median(res.assemble_ms):.3f} ms")
+    if not args.no_cpu:
+        n_newton = int(sum(i.newton_iters for i in res.increments))
+        cb = cpu_j2_baseline(pj, pj.extra["props"], res.u, n_newton,
+                             out["j2"]["linear_iters"])
+        cb["value_job_s_gpu"] = (t1 - t0)
+        out["j2"]["cpu_baseline"] = cb
+        log(f"cpu J2: {cb}")
+
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -494,6 +561,8 @@ def main():
     ap.add_argument("--no-renumbered", action="store_true",
                     help="skip the randomly renumbered copy of the cube")
     ap.add_argument("--j2-size", type=int, default=64)
+    ap.add_argument("--j2-only", action="store_true",
+                    help="run only the J2 Newton workload (profiling); prints its JSON")
     ap.add_argument("--pmc-sample", action="store_true",
                     help="short run for hardware-counter passes: 10 PCG iterations in each "
                          "precision plus a 2 GiB device copy for calibration")
@@ -511,6 +580,11 @@ def main():
     from ufc_hip import problems as P
 
     dev = torch.device("cuda:0")
+    if args.j2_only:
+        out = {"metric": METRIC, "note": "J2 workload only (--j2-only); not the headline"}
+        run_j2(args, out, P, ufc_hip, torch)
+        print(json.dumps(out), flush=True)
+        return
     n = args.size
     log(f"setup {n}^3")
     t0 = time.perf_counter()
@@ -797,67 +871,7 @@ def main():
             log(f"cpu adjoint: {out['adjoint']['cpu_baseline']}")
 
     if not args.no_j2:
-        from ufc_hip.nonlinear import j2_static
-        nj = args.j2_size
-        log(f"J2 {nj}^3 jittered, 10 increments to 0.5% strain")
-        pj = P.j2_cube(nj)
-        target = np.zeros(pj.bc_eq.size)
-        target[pj.extra["n_clamp"]:] = pj.extra["strain"]
-        with ufc_hip.Context(0) as cj:
-            cj.build_pattern(pj.coords, pj.conn)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            res = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
-                            precision=ufc_hip.PREC_MIXED)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            # exact Newton (linear solves to 1e-10) on the same job: the
-            # consistent tangent's iteration count per increment
-            cj.build_pattern(pj.coords, pj.conn)
-            t2 = time.perf_counter()
-            res_ex = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
-                               precision=ufc_hip.PREC_MIXED, lin_rtol=1e-10)
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-        if not (res.converged and res_ex.converged):
-            print(json.dumps({"metric": METRIC, "error": "the J2 Newton job did not converge"}),
-                  flush=True)
-            sys.exit(3)
-        out["j2"] = {"workload": (f"{nj}^3 C3D8 cube, interior nodes jittered +-10% h, "
-                                  "E 200e9, nu 0.3, sigma_y 250e6, H 2e9, u_z on z=1 to "
-                                  "0.5% strain in 10 increments; inexact Newton on the "
-                                  "consistent tangent to |R| <= 1e-8 |F_int| (linear tolerance "
-                                  "1e-6 on the first iteration, then Eisenstat-Walker <= 1e-3), "
-                                  "inner solve FP32-in-FP64"),
-                     "n_dof": pj.n_dof, "converged": res.converged, "total_ms": (t1 - t0) * 1e3,
-                     "newton_iters": [i.newton_iters for i in res.increments],
-                     "linear_iters": int(sum(i.linear_iters for i in res.increments)),
-                     "yielded_gauss_points_final": int(res.increments[-1].n_yield)
-                     if res.increments else 0,
-                     "exact_newton": {"linear_rtol": 1e-10, "total_ms": (t3 - t2) * 1e3,
-                                      "newton_iters": [i.newton_iters for i in res_ex.increments],
-                                      "rnorm_rel": [i.rnorm_rel for i in res_ex.increments],
-                                      "history_last_increment":
-                                          res_ex.increments[-1].history,
-                                      "linear_iters": int(sum(i.linear_iters
-                                                              for i in res_ex.increments))},
-                     "u_exact_vs_inexact_rel": float(np.linalg.norm(res.u - res_ex.u) /
-                                                     np.linalg.norm(res_ex.u))}
-        log(f"J2: {(t1 - t0):.2f} s, newton {out['j2']['newton_iters']}")
-        upd = res.j2_update_ms + res_ex.j2_update_ms
-        out["j2"]["roofline"] = j2_roofline(pj.n_elems, pj.coords.shape[0],
-                                            float(np.median(upd)))
-        out["j2"]["roofline"]["launches"] = len(upd)
-        out["j2"]["tangent_assembly_ms_median"] = float(np.median(res.assemble_ms))
-        log(f"J2 update {np.median(upd):.3f} ms, tangent assembly "
-            f"{np.median(res.assemble_ms):.3f} ms")
-        if not args.no_cpu:
-            n_newton = int(sum(i.newton_iters for i in res.increments))
-            cb = cpu_j2_baseline(pj, pj.extra["props"], res.u, n_newton,
-                                 out["j2"]["linear_iters"])
-            cb["value_job_s_gpu"] = (t1 - t0)
-            out["j2"]["cpu_baseline"] = cb
-            log(f"cpu J2: {cb}")
+        run_j2(args, out, P, ufc_hip, torch)
 
     if not args.no_cpu:
         log("cpu baseline + full-size errors against the oracle")
