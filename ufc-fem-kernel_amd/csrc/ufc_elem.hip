@@ -1314,16 +1314,44 @@ __global__ __launch_bounds__(64) void k_elem_apply(int32_t n_elems,
   const double E = young[e];
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   const double mu = E / (2.0 * (1.0 + nu));
+  // incompatible modes: K_e = K_std - C C^T (C = K_ua L^-T, 24 x 9), applied
+  // as K_std u - C (C^T u) without forming C C^T: each lane adds its node's
+  // rows of C^T u and the eight lanes sum them (three exchanges)
+  constexpr int RV = VAR == kVarIncompat ? kVarFull : VAR;
   double kb[8][3][3] = {};
-  elem_rows<double, VAR>(x, a, lam, mu,
-                         VAR == kVarIncompat ? inc_c + int64_t(kIncompatC) * e : nullptr, kb);
+  elem_rows<double, RV>(x, a, lam, mu, nullptr, kb);
   double f[3] = {0.0, 0.0, 0.0};
+  double ua[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
     const int64_t n = conn[8 * e + b];
     const double u0 = u[3 * n], u1 = u[3 * n + 1], u2 = u[3 * n + 2];
+    if (b == a) {
+      ua[0] = u0;
+      ua[1] = u1;
+      ua[2] = u2;
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) f[i] += kb[b][i][0] * u0 + kb[b][i][1] * u1 + kb[b][i][2] * u2;
+  }
+  if constexpr (VAR == kVarIncompat) {
+    const double* ca = inc_c + int64_t(kIncompatC) * e + 27 * a;  // rows 3a..3a+2 of C
+    double w[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      double t = ca[k] * ua[0] + ca[9 + k] * ua[1] + ca[18 + k] * ua[2];
+      t += __shfl_xor(t, 1, 8);
+      t += __shfl_xor(t, 2, 8);
+      t += __shfl_xor(t, 4, 8);
+      w[k] = t;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) t += ca[9 * i + k] * w[k];
+      f[i] -= t;
+    }
   }
   if constexpr (!SENS) {
     if (live) {
