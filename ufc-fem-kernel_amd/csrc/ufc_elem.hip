@@ -906,26 +906,60 @@ __global__ __launch_bounds__(64) void k_assemble_gen(
 #pragma unroll
         for (int j = 0; j < 3; ++j) kb[b][r][j] = T(0);
     int pos[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // incompatible modes: the standard rows first, then -C_a C_b^T in a second
+    // pass, so that the two are never live together (one pass spilled here)
+    constexpr int RV = VAR == kVarIncompat ? kVarFull : VAR;
+    int32_t pk = 0;
     if (i < ne) {
-      const int32_t pk = gen_ne[p0 + i];
+      pk = gen_ne[p0 + i];
       const int32_t elem = pk >> 3;
-      elem_node_rows<T, MAT, VAR>(coords, conn, elem, pk & 7, young, nu, dgp, inc_c, kb);
+      elem_node_rows<T, MAT, RV>(coords, conn, elem, pk & 7, young, nu, dgp, inc_c, kb);
 #pragma unroll
       for (int b = 0; b < 8; ++b) pos[b] = list_pos(nb_idx + q0, deg, conn[8 * int64_t(elem) + b]);
     }
     const int cnt = ne - c0 < 64 ? ne - c0 : 64;
-    for (int t = 0; t < cnt; ++t) {
-      if (lane == t) {
+    auto add_rows = [&]() {
+      for (int t = 0; t < cnt; ++t) {
+        if (lane == t) {
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          T* dst = acc + 3 * pos[b];
+          for (int b = 0; b < 8; ++b) {
+            T* dst = acc + 3 * pos[b];
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
+            for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) dst[r * d3 + j] += kb[b][r][j];
+              for (int j = 0; j < 3; ++j) dst[r * d3 + j] += kb[b][r][j];
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
+    };
+    add_rows();
+    if constexpr (VAR == kVarIncompat) {
+      if (i < ne) {
+        const double* cinc = inc_c + int64_t(kIncompatC) * (pk >> 3);
+        const int a = pk & 7;
+        T ca[3][9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int k = 0; k < 9; ++k) ca[r][k] = T(cinc[9 * (3 * a + r) + k]);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            T cb[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) cb[k] = T(cinc[9 * (3 * b + j) + k]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              T t = T(0);
+#pragma unroll
+              for (int k = 0; k < 9; ++k) t += ca[r][k] * cb[k];
+              kb[b][r][j] = -t;
+            }
+          }
+      }
+      add_rows();
     }
   }
   if constexpr (sizeof(T) == 4) {
