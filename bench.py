@@ -324,17 +324,22 @@ def j2_roofline(n_elems, n_nodes, ms):
             "fp64_frac_of_78.6": f / (ms * 1e9) / 78.6}
 
 
-def cpu_adjoint_baseline(pa):
+def cpu_adjoint_baseline(pa, young_gpu=None, u_gpu=None, grads=None):
     """Config-4 workload on the FP64 oracle (threads as cpu_baseline), not
     sampled: assembly, forward Jacobi-PCG to 1e-10, the adjoint solve
     K^T lambda = f (NM_Adjoint_Solve on symmetric K, Jacobi-PCG from 0 to
-    1e-10) and the element sensitivities."""
+    1e-10) and the element sensitivities.  With the GPU's E field, forward
+    solution and gradients: the oracle's FP64 residual of the GPU's u, the
+    oracle's sensitivities from the GPU's u (lambda = u for compliance)
+    against the GPU gradient, and each GPU gradient against the oracle's
+    (errors, outside the timed legs)."""
     import numpy as np
     from oracle import oracle as O
     threads, aff = cpu_threads()
     rp, ci = O.csr_pattern(pa.coords.shape[0], pa.conn)
     t0 = time.perf_counter()
-    val = O.assemble(pa.conn, pa.coords, pa.young, pa.nu, rp, ci, nthreads=threads)
+    young = pa.young if young_gpu is None else young_gpu
+    val = O.assemble(pa.conn, pa.coords, young, pa.nu, rp, ci, nthreads=threads)
     t1 = time.perf_counter()
     F = np.zeros(pa.n_dof)
     O.face_loads_pressure(pa.load_elem, pa.load_face, pa.conn, pa.coords, pa.load_value[0], F)
@@ -353,6 +358,21 @@ def cpu_adjoint_baseline(pa):
     t4 = time.perf_counter()
     if st != 0 or st2 != 0:
         raise RuntimeError("CPU oracle adjoint workload did not converge")
+    err = {}
+    if u_gpu is not None:
+        err["gpu_u_oracle_residual"] = float(np.linalg.norm(F - O.spmv(rp, ci, val, u_gpu,
+                                                                       threads)) /
+                                             np.linalg.norm(F))
+        err["gpu_u_vs_cpu_oracle_rel"] = float(np.linalg.norm(u_gpu - u) / np.linalg.norm(u))
+    for name, gg in (grads or {}).items():
+        if gg is None:
+            continue
+        err[f"grad_{name}_vs_cpu_oracle_rel"] = float(np.abs(gg - g).max() / np.abs(g).max())
+        if u_gpu is not None and name == "mixed":
+            go = O.sensitivity(pa.conn, pa.coords, pa.nu, 3.0 * rho ** 2 * 1e9, u_gpu, u_gpu,
+                               nthreads=threads)
+            err["grad_mixed_vs_oracle_sensitivity_of_gpu_u"] = float(np.abs(gg - go).max() /
+                                                                     np.abs(go).max())
     return {"kind": "port", "cores": threads,
             "sample": (f"FP64 C oracle ({threads} threads) on the same {pa.nx}^3 SIMP cube, not "
                        "sampled: assembly, forward PCG to 1e-10, adjoint PCG to 1e-10, "
@@ -360,7 +380,7 @@ def cpu_adjoint_baseline(pa):
             "forward_ms": (t2 - t0) * 1e3, "grad_ms": (t4 - t2) * 1e3,
             "assemble_ms": (t1 - t0) * 1e3, "forward_iters": int(it_f),
             "adjoint_iters": int(it_a), "sensitivity_ms": (t4 - t3) * 1e3,
-            "grad_checksum": float(np.abs(g).sum())}
+            "grad_checksum": float(np.abs(g).sum()), "errors": err}
 
 
 def cpu_j2_baseline(pj, props, u_end, n_newton, n_linear):
@@ -398,6 +418,38 @@ def cpu_j2_baseline(pj, props, u_end, n_newton, n_linear):
                        f"{n_newton} Newton and {n_linear} linear iterations"),
             "update_ms": (t1 - t0) * 1e3, "assemble_ms": (t2 - t1) * 1e3,
             "pcg_iter_ms": per_it * 1e3, "job_s_projected": job}
+
+
+def oracle_j2_replay(pj, props, target, u_incs, state_gpu):
+    """The oracle's replay of a GPU J2 job: from the GPU's converged u of
+    every increment, orc_j2_internal_force recomputes the J2 state from the
+    previous increment's and the internal force; the residual R = P ubar -
+    F_int - P u of each increment relative to ||F_int|| (the GPU's Newton
+    stops at 1e-8 by its own FP64 residual), and the GPU's final state
+    (stresses; equivalent and plastic strains) against the replayed one."""
+    import numpy as np
+    from oracle import oracle as O
+    threads, _ = cpu_threads()
+    nn = pj.coords.shape[0]
+    state = np.zeros(pj.n_elems * 8 * 13)
+    u_prev = np.zeros(pj.n_dof)
+    rels, fails = [], 0
+    n = len(u_incs)
+    for k, uk in enumerate(u_incs, start=1):
+        nf, state, _, fi = O.j2_internal_force(nn, pj.conn, pj.coords, props, uk, u_prev, state,
+                                               nthreads=threads)
+        fails += int(nf)
+        r = -fi
+        r[pj.bc_eq] += 1e12 * (target * (k / n) - uk[pj.bc_eq])
+        rels.append(float(np.linalg.norm(r) / np.linalg.norm(fi)))
+        u_prev = uk
+    so, sg = state.reshape(-1, 13), state_gpu.reshape(-1, 13)
+    return {"oracle_residual_rel_max": max(rels), "oracle_residual_rel": rels,
+            "local_failures": fails,
+            "stress_vs_oracle_replay_rel": float(np.abs(sg[:, :6] - so[:, :6]).max() /
+                                                 np.abs(so[:, :6]).max()),
+            "plastic_strain_vs_oracle_replay_rel": float(np.abs(sg[:, 6:] - so[:, 6:]).max() /
+                                                         np.abs(so[:, 6:]).max())}
 
 
 def renumbered_run(args, torch, dev, ufc_hip, P, n, sols):
@@ -480,17 +532,19 @@ def run_j2(args, out, P, ufc_hip, torch):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
-                        precision=ufc_hip.PREC_MIXED)
+                        precision=ufc_hip.PREC_MIXED, keep_increments=True)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        state_inexact = cj.j2_get_state() if not args.no_cpu else None
         # exact Newton (linear solves to 1e-10) on the same job: the
         # consistent tangent's iteration count per increment
         cj.build_pattern(pj.coords, pj.conn)
         t2 = time.perf_counter()
         res_ex = j2_static(cj, pj.bc_eq, target, pj.extra["props"], n_increments=10,
-                           precision=ufc_hip.PREC_MIXED, lin_rtol=1e-10)
+                           precision=ufc_hip.PREC_MIXED, lin_rtol=1e-10, keep_increments=True)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
+        state_exact = cj.j2_get_state() if not args.no_cpu else None
     if not (res.converged and res_ex.converged):
         print(json.dumps({"metric": METRIC, "error": "the J2 Newton job did not converge"}),
               flush=True)
@@ -524,6 +578,13 @@ def run_j2(args, out, P, ufc_hip, torch):
     log(f"J2 update {np.median(upd):.3f} ms, tangent assembly "
         f"{np.median(res.assemble_ms):.3f} ms")
     if not args.no_cpu:
+        log("J2: oracle replay of both jobs")
+        out["j2"]["errors"] = {
+            "inexact": oracle_j2_replay(pj, pj.extra["props"], target, res.u_increments,
+                                        state_inexact),
+            "exact": oracle_j2_replay(pj, pj.extra["props"], target, res_ex.u_increments,
+                                      state_exact)}
+        log(f"J2 oracle replay: {json.dumps(out['j2']['errors'])[:400]}")
         n_newton = int(sum(i.newton_iters for i in res.increments))
         cb = cpu_j2_baseline(pj, pj.extra["props"], res.u, n_newton,
                              out["j2"]["linear_iters"])
@@ -822,6 +883,12 @@ def main():
             tc = time.perf_counter()
         fwd_ms = (tb - ta) * 1e3
         adj_ms = (tc - tb) * 1e3
+        # the GPU's E field, forward solution and mixed gradient of the last
+        # step, for the oracle's checks at full size
+        young_h = (1e-3 + rho0 ** 3 * 1e9).cpu().numpy()
+        u_h = u.detach().cpu().numpy()
+        g_mixed = rho.grad.detach().cpu().numpy()
+        g_fp64 = None
         out["adjoint"] = {"workload": (f"{na}^3 SIMP cube, rho~U[0.3,1] seed {P.SEED + 1}, "
                                        "E = 1e-3 + rho^3 1e9; c = f.u; dc/drho by "
                                        "torch.autograd (adjoint PCG + element kernel)"),
@@ -840,6 +907,7 @@ def main():
             c64, _ = compliance(m64, 1e-3 + rho ** 3 * 1e9)
             c64.backward()
             g = rho.grad.detach().cpu().numpy()
+            g_fp64 = g
             gm = rho0.clone().requires_grad_(True)
             cm, _ = compliance(model, 1e-3 + gm ** 3 * 1e9)
             cm.backward()
@@ -867,7 +935,9 @@ def main():
             m64.close()
         model.close()
         if not args.no_cpu:
-            out["adjoint"]["cpu_baseline"] = cpu_adjoint_baseline(pa)
+            out["adjoint"]["cpu_baseline"] = cpu_adjoint_baseline(
+                pa, young_h, u_h, {"mixed": g_mixed, "fp64": g_fp64})
+            out["adjoint"]["errors"] = out["adjoint"]["cpu_baseline"].pop("errors")
             log(f"cpu adjoint: {out['adjoint']['cpu_baseline']}")
 
     if not args.no_j2:

@@ -19,6 +19,15 @@ material at a size the oracle's Newton loop runs in seconds): the GPU
 Newton loop with exact linear solves against orc_j2_newton_step over all 10
 increments, displacements within 1e-7, stresses and plastic strains within
 1e-6.
+
+96^3 SIMP adjoint workload (config 4 at full size): the oracle's residuals of
+the GPU's forward and adjoint solutions, lambda = u for compliance, and the
+oracle's element sensitivities from the GPU's u and lambda against the
+torch.autograd gradient.
+
+64^3 J2 job (config 5 at full size): the oracle replays the J2 state from the
+GPU's converged displacement of every increment and checks the residual of
+each increment and the final state of every Gauss point.
 """
 import os
 
@@ -153,6 +162,104 @@ def test_j2_cube16_newton_vs_oracle(ctx, orc):
     assert np.abs(sg[:, :6] - so[:, :6]).max() < 1e-6 * np.abs(so[:, :6]).max()
     assert np.abs(sg[:, 6] - so[:, 6]).max() < 1e-6 * so[:, 6].max()
     assert np.abs(sg[:, 7:] - so[:, 7:]).max() < 1e-6 * np.abs(so[:, 7:]).max()
+
+
+def test_adjoint96_vs_oracle(orc):
+    """The 96^3 SIMP adjoint workload at full size (2,738,019 DOF): the mixed
+    forward solve and the mixed adjoint solve K^T lambda = f behind
+    torch.autograd, checked by the oracle on the same mesh and field:
+      - the oracle's FP64 residuals ||F - K u|| / ||F|| and ||f - K lambda|| /
+        ||f|| below 2e-10 (both solves stop at 1e-10 on their recurrence
+        residuals; NM_Adjoint_Solve solves K^T lambda = dJ/du with K^T = K,
+        NM_Solv_GMRESTranspose.f90:229-291);
+      - lambda = u for compliance (J = f.u, f the external load) to 1e-8;
+      - the element sensitivities the oracle forms from the GPU's u and
+        lambda within 1e-10 of the autograd gradient."""
+    import torch
+    import ufc_hip
+    from ufc_hip import problems as P
+    from ufc_hip.autograd import StaticModel, compliance
+    pa = P.simp_cube(96)
+    model = StaticModel.from_problem(pa, precision=ufc_hip.PREC_MIXED, rtol=1e-10)
+    try:
+        rho = torch.as_tensor(pa.extra["rho"], device="cuda:0").requires_grad_(True)
+        young = 1e-3 + rho ** 3 * 1e9
+        say("adjoint 96^3: forward + backward")
+        c, u = compliance(model, young)
+        c.backward()
+        g_gpu = rho.grad.detach().cpu().numpy()
+        lam = torch.zeros_like(u)
+        _, info = model.ctx.adjoint_solve(model.f, precision=ufc_hip.PREC_MIXED, rtol=1e-10,
+                                          x=lam)
+        assert info.converged
+        u_h, lam_h = u.detach().cpu().numpy(), lam.cpu().numpy()
+        f_ext = model.f.cpu().numpy()
+        e_h = young.detach().cpu().numpy()
+    finally:
+        model.close()
+    say("adjoint 96^3: oracle pattern + assembly")
+    rp, ci = orc.csr_pattern(pa.coords.shape[0], pa.conn)
+    val = orc.assemble(pa.conn, pa.coords, e_h, pa.nu, rp, ci, nthreads=_threads())
+    F = np.zeros(pa.n_dof)
+    orc.face_loads_pressure(pa.load_elem, pa.load_face, pa.conn, pa.coords,
+                            pa.load_value[0], F)
+    orc.apply_penalty(pa.bc_eq, pa.bc_val, P.PENALTY, rp, ci, val, F)
+    ru = np.linalg.norm(F - orc.spmv(rp, ci, val, u_h, _threads())) / np.linalg.norm(F)
+    rl = np.linalg.norm(f_ext - orc.spmv(rp, ci, val, lam_h, _threads())) / np.linalg.norm(f_ext)
+    d = np.linalg.norm(lam_h - u_h) / np.linalg.norm(u_h)
+    g_o = orc.sensitivity(pa.conn, pa.coords, pa.nu, 3.0 * pa.extra["rho"] ** 2 * 1e9, u_h,
+                          lam_h, nthreads=_threads())
+    eg = np.abs(g_o - g_gpu).max() / np.abs(g_o).max()
+    say("adjoint 96^3: residual u", ru, "lambda", rl, "lambda vs u", d, "gradient", eg)
+    assert ru < 2e-10 and rl < 2e-10
+    assert d < 1e-8
+    assert eg < 1e-10
+
+
+def test_j2_cube64_oracle_replay(ctx, orc):
+    """The 64^3 J2 job itself (config 5: jittered mesh, 10 increments to 0.5%
+    strain, inexact Newton with the mixed inner solve), replayed by the
+    oracle: from the GPU's converged displacement of each increment the
+    oracle recomputes the J2 state from the previous increment's
+    (orc_j2_internal_force: strain increment, radial return,
+    PH_Mat_Plast_J2_Iso_Core.f90:254-456) and the residual R = P ubar -
+    F_int - P u (RT_Asm_ComputeResidual, RT_Asm_Solv.f90:1311-1426):
+      - ||R|| / ||F_int|| below 2e-8 at every increment (the GPU's Newton
+        stops at 1e-8 by its own FP64 residual);
+      - no local failure, and the GPU's committed state after the last
+        increment (stress, equivalent and plastic strains of all 2,097,152
+        Gauss points) within 1e-9 of the oracle's."""
+    from ufc_hip import problems as P
+    from ufc_hip.nonlinear import j2_static
+    pr = P.j2_cube(64)
+    props = pr.extra["props"]
+    target = np.zeros(pr.bc_eq.size)
+    target[pr.extra["n_clamp"]:] = pr.extra["strain"]
+    ctx.build_pattern(pr.coords, pr.conn)
+    say("j2 64^3 gpu job (mixed, inexact Newton)")
+    res = j2_static(ctx, pr.bc_eq, target, props, n_increments=10, precision=2,
+                    keep_increments=True)
+    assert res.converged and len(res.u_increments) == 10 and res.cutbacks == 0
+    sg = ctx.j2_get_state().reshape(-1, 13)
+    nn = pr.coords.shape[0]
+    state = np.zeros(pr.n_elems * 8 * 13)
+    u_prev = np.zeros(pr.n_dof)
+    rels = []
+    for k, uk in enumerate(res.u_increments, start=1):
+        nf, state, _, fi = orc.j2_internal_force(nn, pr.conn, pr.coords, props, uk, u_prev,
+                                                 state, nthreads=_threads())
+        assert nf == 0
+        r = -fi
+        r[pr.bc_eq] += P.PENALTY * (target * (k / 10.0) - uk[pr.bc_eq])
+        rels.append(np.linalg.norm(r) / np.linalg.norm(fi))
+        u_prev = uk
+    so = state.reshape(-1, 13)
+    es = np.abs(sg[:, :6] - so[:, :6]).max() / np.abs(so[:, :6]).max()
+    ep = np.abs(sg[:, 6:] - so[:, 6:]).max() / np.abs(so[:, 6:]).max()
+    say("j2 64^3 oracle residuals", ["%.2e" % x for x in rels], "state", es, ep,
+        "yielded", int(np.count_nonzero(so[:, 6])))
+    assert max(rels) < 2e-8
+    assert es < 1e-9 and ep < 1e-9
 
 
 def test_nnz_beyond_int32():

@@ -51,6 +51,7 @@ class NewtonResult:
     failures: list = field(default_factory=list)  # (load factor tried, reason) per cut
     j2_update_ms: list = field(default_factory=list)  # device time of every J2 update
     assemble_ms: list = field(default_factory=list)   # device time of every tangent assembly
+    u_increments: list = field(default_factory=list)  # converged u per increment (keep_increments)
 
 
 def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
@@ -75,7 +76,8 @@ def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
 
 def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
               tol_r=1e-8, lin_rtol=None, max_newton=25, penalty=1.0e12, max_cutbacks=4,
-              restart_file=None, checkpoint_freq=1, resume_from=None, stop_after=None):
+              restart_file=None, checkpoint_freq=1, resume_from=None, stop_after=None,
+              keep_increments=False):
     """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
     in n_increments equal increments.  ctx must hold the mesh (build_pattern).
     props: (E, nu, sigma_y0, H) or a dict of Context.j2_init keywords (the
@@ -91,7 +93,9 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     resume_from: a restart file to continue from (load factor, increment count,
     u and the J2 state); increments then resume at the nominal size
     1/n_increments.  stop_after: return after that many converged increments
-    in total (a job that ends early, for restarts)."""
+    in total (a job that ends early, for restarts).  keep_increments: a copy of
+    the converged u of every increment in res.u_increments (the path a
+    history-dependent material followed, e.g. to replay it elsewhere)."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
     if isinstance(props, dict):
@@ -154,6 +158,8 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             lam = target
             count += 1
             res.increments.append(IncrementInfo(lam, its, lin, rrel, ny, hist))
+            if keep_increments:
+                res.u_increments.append(np.array(u, copy=True))
             cutbacks = 0
             if restart_file is not None and count % checkpoint_freq == 0:
                 R.save(restart_file, u, R.StepState(lam, count, lam, rabs, its, True), ctx=ctx,
