@@ -500,6 +500,52 @@ def renumbered_run(args, torch, dev, ufc_hip, P, n, sols):
     return res
 
 
+def shard_run(args, torch, dev, ufc_hip, P, n, check_step):
+    """One GPU's share of the n^3 cube split into 8 z-slabs (n x n x n/8
+    hexes, the same seeded E field; z=0 clamped, pressure on the slab's top):
+    what a GPU of an 8-GPU run computes per step, without the halo exchange
+    and the two all-reduces per iteration that only a multi-GPU run has.
+    Mixed and FP64 steps timed as the headline (3 steps after one warm-up),
+    the FP32 SpMV's launch time (events around every 8th launch) and rate."""
+    import numpy as np
+    pr = P.slab_share(n, 8)
+    log(f"shard {pr.name}: setup")
+    ctx = ufc_hip.Context(0)
+    P.setup_context(ctx, pr, ufc_hip.PREC_MIXED)
+    ctx.enable_timing(8)
+    pr.young_dev = torch.as_tensor(pr.young, device=dev)
+    x = torch.zeros(pr.n_dof, dtype=torch.float64, device=dev)
+    res = {"workload": (f"{pr.name}: the bottom of 8 z-slabs of the {n}^3 lognormal cube "
+                        "(same E field), z=0 clamped, p=1e6 on the slab top; no halo exchange "
+                        "or all-reduce (one GPU)"),
+           "n_dof": pr.n_dof, "nnz": ctx.nnz}
+    for name, prec, repl in (("mixed", ufc_hip.PREC_MIXED, args.replace_rtol),
+                             ("fp64", ufc_hip.PREC_FP64, 0.0)):
+        r, _ = run_step(ctx, pr, prec, 1e-10, torch, x, repl)
+        check_step(f"shard {name}", r, "warmup")
+        runs = []
+        for k in range(3):
+            r, _ = run_step(ctx, pr, prec, 1e-10, torch, x, repl)
+            check_step(f"shard {name}", r, f"step {k}")
+            runs.append(r)
+        ms = float(np.mean([q["wall_ms"] for q in runs]))
+        it = runs[-1]["niter"]
+        res[name] = {"ms": ms, "dof_per_s": pr.n_dof / (ms / 1e3), "iters": it,
+                     "us_per_iteration": runs[-1]["solve_ms"] / max(it, 1) * 1e3,
+                     "spmv_us": runs[-1]["spmv_ms"] * 1e3}
+        log(f"shard {name}: {ms:.1f} ms, it {it}, "
+            f"{res[name]['us_per_iteration']:.1f} us/iteration, spmv "
+            f"{res[name]['spmv_us']:.1f} us")
+    b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ctx.nb_index_bytes,
+                       int(np.unique(pr.bc_eq).size))
+    res["mixed"]["spmv_gbs"] = b32 / (res["mixed"]["spmv_us"] * 1e3)
+    res["mixed"]["spmv_frac"] = res["mixed"]["spmv_gbs"] / HBM_PEAK_GBS
+    res["fp64"]["spmv_gbs"] = spmv64_bytes(pr.n_dof, ctx.nnz) / (res["fp64"]["spmv_us"] * 1e3)
+    res["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+    ctx.close()
+    return res
+
+
 def copy_bandwidth(torch, dev, nbytes=1 << 31):
     """Device-to-device copy of nbytes (reads and writes nbytes): the
     attainable-bandwidth denominator beside the 8 TB/s peak."""
@@ -621,6 +667,8 @@ def main():
                          "4-byte-id and stencil-pattern views)")
     ap.add_argument("--no-renumbered", action="store_true",
                     help="skip the randomly renumbered copy of the cube")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="skip one GPU's share of an 8-way z-slab split of the cube")
     ap.add_argument("--j2-size", type=int, default=64)
     ap.add_argument("--j2-only", action="store_true",
                     help="run only the J2 Newton workload (profiling); prints its JSON")
@@ -863,6 +911,13 @@ def main():
 
     if not args.no_renumbered:
         out["renumbered"] = renumbered_run(args, torch, dev, ufc_hip, P, n, sols)
+
+    if not args.no_shard:
+        out["shard"] = shard_run(args, torch, dev, ufc_hip, P, n, check_step)
+        if failed:
+            print(json.dumps({"metric": METRIC, "error": "a shard solve did not converge",
+                              "failed": failed}), flush=True)
+            sys.exit(3)
 
     if not args.no_adjoint:
         na = args.adjoint_size

@@ -514,13 +514,22 @@ def test_fp32_symdiag_view(ctx, orc):
     assert ctx.info().fp32_view == before
 
 
-def test_fp32_symdiag_view_tile_blocks(ctx, orc):
-    """A 24^3 cube: 62 tiles of 256 nodes, tile blocks of 16 planes of 3 tiles
-    (48), so the symmetric view's tile order runs a full block and a partial
-    one; SpMV and the mixed solve match the oracle."""
+@pytest.mark.parametrize("shape", ["cube24", "slab48x48x6", "slab40x40x2"])
+def test_fp32_symdiag_view_tile_blocks(ctx, orc, shape):
+    """The symmetric view's tile order over planes of F tiles in blocks of 16
+    planes, the last block holding the planes that remain: a 24^3 cube (62
+    tiles, F = 3: 21 planes, one full block and a block of 5), a 48 x 48 x 6
+    slab (one GPU's share of an 8-way z-split of a 48^3 cube: 66 tiles, F =
+    10: a block of 7 planes only) and a 40 x 40 x 2 slab (20 tiles, F = 7: a
+    block of 3 planes); SpMV and the mixed solve match the oracle."""
     from ufc_hip import _abi as A
     from ufc_hip import problems as P
-    pr = P.cube_pressure(24, name="cube24")
+    if shape == "cube24":
+        pr = P.cube_pressure(24, name="cube24")
+    elif shape == "slab48x48x6":
+        pr = P.slab_share(48, 8)
+    else:
+        pr = P.slab_share(40, 20)
     rp, ci, val, F, _ = _oracle_system(orc, pr)
     P.setup_context(ctx, pr, 2)
     assert ctx.fp32_view == A.FP32_VIEW_SYMDIAG
@@ -528,7 +537,7 @@ def test_fp32_symdiag_view_tile_blocks(ctx, orc):
     x = np.random.default_rng(8).normal(size=pr.n_dof)
     yo = orc.spmv(rp, ci, val, x)
     y32 = ctx.spmv(x, precision=1)
-    say("symdiag cube24 spmv", np.abs(y32 - yo).max() / np.abs(yo).max())
+    say("symdiag", shape, "spmv", np.abs(y32 - yo).max() / np.abs(yo).max())
     assert np.abs(y32 - yo).max() <= 1e-5 * np.abs(yo).max()
     ctx.assemble_k(pr.young, pr.nu, precision=2)
     u, info = ctx.pcg(precision=2, rtol=1e-10)

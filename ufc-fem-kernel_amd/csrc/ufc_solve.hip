@@ -593,6 +593,33 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 // into LDS with coalesced loads, and every x_{n +- o_s} and x_n is then an LDS
 // read: 27 global gathers per node become 6 streaming loads per thread.
 // Window nodes outside the mesh hold 0; their blocks are zero.
+// Tile order of the symmetric view, shared by k_spmv_sym and its grid: the
+// P = ceil(ntiles / F) planes of F tiles (F = diag_pad_tiles) are visited in
+// blocks of kDiagPlanes planes, the last block holding the P mod kDiagPlanes
+// that remain; within a block of nb planes, position r is tile
+// (first plane + r mod nb) F + r / nb.  P F positions, of which fewer than F
+// (in the last plane) hold no tile.  Rounding the last block up to
+// kDiagPlanes planes instead left up to 15 planes of empty positions at the
+// end, i.e. whole XCDs idle on a box a few blocks thick (a 128 x 128 x 16
+// slab: half of its positions empty, the FP32 SpMV 1.5x slower than on a
+// 64^3 cube of the same size).
+struct SymOrder {
+  int32_t F, nfull, rem, npos;
+};
+__host__ __device__ inline SymOrder sym_order(int32_t n_nodes, int32_t F) {
+  const int32_t ntiles = (n_nodes + kDiagTile - 1) / kDiagTile;
+  const int32_t P = (ntiles + F - 1) / F;
+  return SymOrder{F, P / kDiagPlanes, P % kDiagPlanes, P * F};
+}
+__device__ __forceinline__ int32_t sym_tile(const SymOrder& o, int32_t w) {
+  const int32_t blk = o.F * kDiagPlanes;
+  const int32_t zb = w / blk;
+  const int32_t nb = zb < o.nfull ? kDiagPlanes : o.rem;
+  const int32_t r = w - zb * blk;
+  const int32_t p = r / nb, zi = r - p * nb;
+  return (zb * kDiagPlanes + zi) * o.F + p;
+}
+
 // diagonals whose loads are issued together (128^3: 4 and 7 0.266 ms, 13
 // 0.272 ms; one at a time, as the compiler schedules it unstaged, drains the
 // loads after every diagonal)
@@ -615,17 +642,15 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
   }
   static_assert(kDiagTile == kDiagThreads, "one node per thread and tile entry");
   const int32_t ntiles = (n_nodes + kDiagTile - 1) / kDiagTile;
-  const int32_t blk = pad_tiles * kDiagPlanes;
-  const int32_t npos = (ntiles + blk - 1) / blk * blk;
+  const SymOrder ord = sym_order(n_nodes, pad_tiles);
+  const int32_t npos = ord.npos;
   const int32_t ppx = (npos + 7) / 8;
   const int32_t lo = int32_t(blockIdx.x % 8) * ppx;
   const int32_t hi = lo + ppx < npos ? lo + ppx : npos;
   const int32_t step = int32_t(gridDim.x / 8);
   double dot = 0.0;
   for (int32_t w = lo + int32_t(blockIdx.x / 8); w < hi; w += step) {
-    const int32_t zb = w / blk, r = w - zb * blk;
-    const int32_t p = r / kDiagPlanes, zi = r - p * kDiagPlanes;
-    const int32_t t = (zb * kDiagPlanes + zi) * pad_tiles + p;
+    const int32_t t = sym_tile(ord, w);
     // 32-bit indices: n < n_nodes < 2^31 and the entries < 2^31 (the host
     // checks), which keeps the address arithmetic off 64-bit VALU pairs
     const int32_t n = t * kDiagTile + int32_t(threadIdx.x);
@@ -1273,9 +1298,7 @@ int vec_blocks(int64_t n) {
 // symmetric block-diagonal SpMV grid: 8 x the tile positions per XCD, capped (a
 // multiple of 8 either way)
 int sym_blocks(const ufc_ctx* c) {
-  const int64_t ntiles = (int64_t(c->n_nodes) + kDiagTile - 1) / kDiagTile;
-  const int64_t blk = int64_t(c->diag_pad_tiles) * kDiagPlanes;
-  const int64_t npos = (ntiles + blk - 1) / blk * blk;
+  const int64_t npos = sym_order(c->n_nodes, c->diag_pad_tiles).npos;
   const int64_t ppx = (npos + 7) / 8;
   return int(8 * std::min<int64_t>(ppx, kSpmvBlocks32 / 8));
 }

@@ -159,7 +159,9 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             count += 1
             res.increments.append(IncrementInfo(lam, its, lin, rrel, ny, hist))
             if keep_increments:
-                res.u_increments.append(np.array(u, copy=True))
+                # u is never modified in place (each Newton step makes a new
+                # array), so keeping the reference costs nothing
+                res.u_increments.append(u)
             cutbacks = 0
             if restart_file is not None and count % checkpoint_freq == 0:
                 R.save(restart_file, u, R.StepState(lam, count, lam, rabs, its, True), ctx=ctx,

@@ -87,6 +87,21 @@ def lognormal_cube(n=128, seed=SEED) -> Problem:
     return cube_pressure(n, name=f"cube{n}_lognormal", seed=seed)
 
 
+def slab_share(n=128, parts=8, seed=SEED) -> Problem:
+    """One GPU's share of the n^3 lognormal cube cut into `parts` z-slabs: the
+    bottom slab, n x n x n/parts hexes of size 1/n (the first elements of the
+    cube's numbering, so the same seeded E field), z=0 clamped, p=1e6 on its
+    top face (where the next slab's halo would act)."""
+    nz = n // parts
+    coords, conn = S.mesh_box(n, n, nz, 1.0, 1.0, nz / n, 0.0, seed + 2)
+    clamp = _clamp_eqs(_nodes_where(coords, 2, 0.0))
+    elem = _elem_grid(n, n, nz)[nz - 1, :, :].ravel()
+    young = S.field("lognormal", conn.shape[0], seed, math.log(1.0e9), 0.5)
+    return Problem(f"slab{n}x{n}x{nz}", n, n, nz, coords, conn, young, 0.3, clamp,
+                   np.zeros(clamp.size), elem, np.full(elem.size, 1, np.int32), "pressure",
+                   (1.0e6,))
+
+
 def simp_cube(n=96, seed=SEED):
     """96^3 SIMP cube: rho_e ~ U[0.3, 1], E_e = 1e-3 + rho_e^3 1e9."""
     rho = S.field("uniform", n ** 3, seed + 1, 0.3, 1.0)
