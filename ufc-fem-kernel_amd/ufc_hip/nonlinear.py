@@ -77,7 +77,7 @@ def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
 def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
               tol_r=1e-8, lin_rtol=None, max_newton=25, penalty=1.0e12, max_cutbacks=4,
               restart_file=None, checkpoint_freq=1, resume_from=None, stop_after=None,
-              keep_increments=False):
+              keep_increments=False, predictor=False):
     """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
     in n_increments equal increments.  ctx must hold the mesh (build_pattern).
     props: (E, nu, sigma_y0, H) or a dict of Context.j2_init keywords (the
@@ -95,7 +95,18 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     1/n_increments.  stop_after: return after that many converged increments
     in total (a job that ends early, for restarts).  keep_increments: a copy of
     the converged u of every increment in res.u_increments (the path a
-    history-dependent material followed, e.g. to replay it elsewhere)."""
+    history-dependent material followed, e.g. to replay it elsewhere).
+
+    predictor: the first Newton iterate of an increment is extrapolated from
+    the previous converged increment, u_n + (dlam / dlam_prev) (u_n - u_n-1),
+    instead of u_n (RT_NLSolver_NewtonRaph starts from u_n): it meets the new
+    prescribed values (proportional to the load factor) and carries the
+    previous increment's deformation, so the first linear solve is an
+    ordinary inexact-Newton step (tolerance eta_max) instead of the
+    boundary-jump solve to eta_first.  The stress update still goes from the
+    committed state over the whole increment u - u_n, so the converged
+    increment is the same to the Newton tolerance.  Not used for the first
+    increment, after a resume, or on the retry of a failed attempt."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
     if isinstance(props, dict):
@@ -114,6 +125,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     step = 1.0 / n_increments
     res = NewtonResult(converged=True)
     cutbacks = 0
+    du_prev = dlam_prev = None  # the last converged increment (predictor)
     while lam < 1.0 - 1e-12 and (stop_after is None or count < stop_after):
         dlam = min(step, 1.0 - lam)
         target = lam + dlam
@@ -121,7 +133,8 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
         u_n = u.copy()
         ok, its, lin, rrel, ny, rabs = False, 0, 0, np.inf, 0, np.inf
         hist = []
-        u_try = u_n.copy()
+        predicted = predictor and du_prev is not None
+        u_try = u_n + (dlam / dlam_prev) * du_prev if predicted else u_n.copy()
         r_prev = eta = None
         reason = f"no convergence in {max_newton} Newton iterations"
         for it in range(1, max_newton + 1):
@@ -140,7 +153,8 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
                 ok = True
                 break
             if lin_rtol is None:
-                eta = forcing_term(norms[0], r_prev, eta, tol_r * norms[1])
+                eta = forcing_term(norms[0], r_prev, eta, tol_r * norms[1],
+                                   eta_first=1e-3 if predicted else 1e-6)
             else:
                 eta = lin_rtol
             r_prev = norms[0]
@@ -154,6 +168,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             u_try = u_try + du
         if ok:
             ctx.j2_commit()
+            du_prev, dlam_prev = u_try - u_n, dlam
             u = u_try
             lam = target
             count += 1
@@ -171,6 +186,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             # the committed state and u_n (RT_Step_Exec.f90:455-471)
             cutbacks += 1
             res.cutbacks += 1
+            du_prev = None  # the retry starts from u_n
             res.failures.append((target, reason))
             if cutbacks > max_cutbacks:
                 res.converged = False
