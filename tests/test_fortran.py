@@ -62,14 +62,15 @@ def test_driver_cantilever_matches_oracle(tmp_path, orc, golden_dir, prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_j2_driver_matches_oracle(tmp_path, orc, prec):
+@pytest.mark.parametrize("prec,opts", [("fp64", []), ("mixed", []), ("mixed", ["predict=1"])])
+def test_j2_driver_matches_oracle(tmp_path, orc, prec, opts):
     """J2 cube 4^3 (jittered), 10 increments to 0.5% strain, through the
-    Fortran Newton driver; the oracle runs the same increments."""
+    Fortran Newton driver (also with the extrapolated first Newton iterate,
+    predict=1); the oracle runs the same increments from u_n."""
     from ufc_hip import problems as P
     out = tmp_path / "u.bin"
-    print(f"[gpu-test] fortran j2 driver 4^3 {prec}", flush=True)
-    r = subprocess.run([_exe(EXE_J2), "4", prec, "10", str(out)], capture_output=True,
+    print(f"[gpu-test] fortran j2 driver 4^3 {prec} {opts}", flush=True)
+    r = subprocess.run([_exe(EXE_J2), "4", prec, "10", str(out)] + opts, capture_output=True,
                        text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr

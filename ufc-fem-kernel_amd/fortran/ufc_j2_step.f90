@@ -29,7 +29,11 @@
 !   PH_MAT_J2_HARD_* law; swift: sigma_y = K (eps0 + e)^n, voce: sigma_y =
 !   sigma_y0 + Q (1 - exp(-delta e))), sy0=, h=, k=, n=, eps0=, q=, delta=
 !   (law parameters), local_iter= (local Newton limit, 0: 25), max_newton=
-!   (Newton iterations per attempt, default 25).
+!   (Newton iterations per attempt, default 25), predict=1 (the first Newton
+!   iterate of an increment extrapolated from the previous converged one,
+!   u_n + (dlam / dlam_prev) (u_n - u_n-1), its linear solve at the capped
+!   forcing term 1e-3; default 0: from u_n, as RT_NLSolver_NewtonRaph; not
+!   used for the first increment, after a resume or on a cut-back retry).
 !   precision fp64 | mixed (the linear solve); u_out: n_dof doubles, stream.
 !   restart: checkpoint file written after every converged increment with u,
 !   the step data and the J2 state (ufc_rt_restart_save; RT_Step_Exec.f90:
@@ -47,7 +51,7 @@ PROGRAM ufc_j2_step
   INTEGER(C_INT64_T), PARAMETER :: mesh_seed = 20251205_C_INT64_T
   REAL(C_DOUBLE), PARAMETER :: strain = 0.005_C_DOUBLE, tol_r = 1.0E-8_C_DOUBLE
   INTEGER, PARAMETER :: max_cutbacks = 4
-  INTEGER :: max_newton, n_cut, npos, ia, eqpos
+  INTEGER :: max_newton, n_cut, npos, ia, eqpos, predict
   CHARACTER(LEN=64) :: key, reason
   CHARACTER(LEN=512) :: pos(7)
 
@@ -60,12 +64,13 @@ PROGRAM ufc_j2_step
   INTEGER(C_INT32_T) :: nn, ne, prec, rc
   INTEGER(C_INT64_T) :: n_yield
   INTEGER(C_INT64_T) :: ndof
-  REAL(C_DOUBLE) :: lam, dlam, step, target, rrel, eta, eta_prev, r_prev
+  REAL(C_DOUBLE) :: lam, dlam, step, target, rrel, eta, eta_prev, r_prev, dlam_prev
   REAL(C_DOUBLE), TARGET :: norms(2)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: coords(:, :), u(:), u_n(:), u_try(:), r(:), du(:)
+  REAL(C_DOUBLE), ALLOCATABLE :: du_prev(:)
   REAL(C_DOUBLE), ALLOCATABLE, TARGET :: bc_target(:), bc_val(:)
   INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: conn(:, :), bc_eq(:)
-  LOGICAL :: ok, failed
+  LOGICAL :: ok, failed, have_prev, predicted
   TYPE(C_PTR) :: ctx
   TYPE(ufc_status) :: st
   TYPE(ufc_mesh_desc) :: mesh
@@ -84,6 +89,7 @@ PROGRAM ufc_j2_step
   props%hardening = 2.0E9_C_DOUBLE
   props%hardening_type = UFC_J2_HARD_LINEAR
   max_newton = 25
+  predict = 0
   npos = 0
   pos = ' '
   DO ia = 1, COMMAND_ARGUMENT_COUNT()
@@ -115,6 +121,7 @@ PROGRAM ufc_j2_step
     CASE ('delta'); READ(arg, *) props%voce_delta
     CASE ('local_iter'); READ(arg, *) props%max_local_iter
     CASE ('max_newton'); READ(arg, *) max_newton
+    CASE ('predict'); READ(arg, *) predict
     CASE DEFAULT
       WRITE(*, '(A,A)') 'unknown option ', TRIM(key)
       CALL EXIT(2)
@@ -142,7 +149,7 @@ PROGRAM ufc_j2_step
   ne = INT(n**3, C_INT32_T)
   ndof = 3_C_INT64_T * nn
   ALLOCATE(coords(3, nn), conn(8, ne))
-  ALLOCATE(u(ndof), u_n(ndof), u_try(ndof), r(ndof), du(ndof))
+  ALLOCATE(u(ndof), u_n(ndof), u_try(ndof), r(ndof), du(ndof), du_prev(ndof))
   rc = ufc_mesh_box(INT(n, C_INT32_T), INT(n, C_INT32_T), INT(n, C_INT32_T), 1.0_C_DOUBLE, &
                     1.0_C_DOUBLE, 1.0_C_DOUBLE, 0.1_C_DOUBLE, mesh_seed, 1_C_INT32_T, &
                     C_LOC(coords), C_LOC(conn), st)
@@ -198,6 +205,8 @@ PROGRAM ufc_j2_step
   total_newton = 0
   total_linear = 0
   failed = .FALSE.
+  have_prev = .FALSE.
+  dlam_prev = 0.0_C_DOUBLE
   IF (TRIM(rstmode) == 'resume') THEN
     rst%u = C_LOC(u)
     rc = ufc_rt_restart_load(ctx, TRIM(rstfile) // C_NULL_CHAR, rst, 1_C_INT32_T, st)
@@ -217,7 +226,12 @@ PROGRAM ufc_j2_step
     rc = ufc_rt_asm_apply_bc(ctx, bc, st)
     CALL check('ufc_rt_asm_apply_bc')
     u_n = u
-    u_try = u_n
+    predicted = predict /= 0 .AND. have_prev
+    IF (predicted) THEN
+      u_try = u_n + (dlam / dlam_prev) * du_prev
+    ELSE
+      u_try = u_n
+    END IF
     ok = .FALSE.
     lin = 0
     rrel = HUGE(1.0_C_DOUBLE)
@@ -239,11 +253,13 @@ PROGRAM ufc_j2_step
         EXIT
       END IF
       ! forcing term: the first iteration carries the jump of the prescribed
-      ! values and is solved to 1e-6; then eta = 0.9 (|R_k|/|R_k-1|)^2
+      ! values and is solved to 1e-6 (1e-3 from a predicted iterate, which
+      ! meets them already); then eta = 0.9 (|R_k|/|R_k-1|)^2
       ! (Eisenstat & Walker 1996), safeguarded, capped at 1e-3, and never
       ! tighter than reaching the Newton target needs
       IF (r_prev < 0.0_C_DOUBLE) THEN
         eta = 1.0E-6_C_DOUBLE
+        IF (predicted) eta = 1.0E-3_C_DOUBLE
       ELSE
         eta = 0.9_C_DOUBLE * (norms(1) / r_prev)**2
         IF (eta_prev > 0.0_C_DOUBLE .AND. 0.9_C_DOUBLE * eta_prev**2 > 0.1_C_DOUBLE) &
@@ -274,6 +290,9 @@ PROGRAM ufc_j2_step
     IF (ok) THEN
       rc = ufc_ph_j2_commit(ctx, st)
       CALL check('ufc_ph_j2_commit')
+      du_prev = u_try - u_n
+      dlam_prev = dlam
+      have_prev = .TRUE.
       u = u_try
       lam = target
       n_done = n_done + 1
@@ -299,6 +318,7 @@ PROGRAM ufc_j2_step
            MIN(it, max_newton), ' rrel ', rrel, ' reason ', TRIM(reason)
       cutbacks = cutbacks + 1
       n_cut = n_cut + 1
+      have_prev = .FALSE.  ! the retry starts from u_n
       IF (cutbacks > max_cutbacks) THEN
         failed = .TRUE.
         EXIT
