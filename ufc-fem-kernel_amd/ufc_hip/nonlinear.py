@@ -77,7 +77,7 @@ def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
 def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=A.PREC_MIXED,
               tol_r=1e-8, lin_rtol=None, max_newton=25, penalty=1.0e12, max_cutbacks=4,
               restart_file=None, checkpoint_freq=1, resume_from=None, stop_after=None,
-              keep_increments=False, predictor=False):
+              keep_increments=False, predictor=False, replace_rtol=0.0):
     """Drives prescribed values bc_target (reached at load factor 1) on bc_eq
     in n_increments equal increments.  ctx must hold the mesh (build_pattern).
     props: (E, nu, sigma_y0, H) or a dict of Context.j2_init keywords (the
@@ -106,7 +106,9 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     boundary-jump solve to eta_first.  The stress update still goes from the
     committed state over the whole increment u - u_n, so the converged
     increment is the same to the Newton tolerance.  Not used for the first
-    increment, after a resume, or on the retry of a failed attempt."""
+    increment, after a resume, or on the retry of a failed attempt.
+    replace_rtol: the mixed linear solves' FP64 residual replacement interval
+    (Context.pcg; 0: restarted refinement steps)."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
     if isinstance(props, dict):
@@ -160,7 +162,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             r_prev = norms[0]
             ctx.assemble_k(None, nu, precision=precision, material=A.MAT_J2)
             res.assemble_ms.append(ctx.timing().assemble_ms)
-            du, info = ctx.pcg(precision=precision, rtol=eta, b=r)
+            du, info = ctx.pcg(precision=precision, rtol=eta, b=r, replace_rtol=replace_rtol)
             lin += info.niter
             if not info.converged:
                 reason = f"linear solve status {info.status}"
