@@ -108,7 +108,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     increment is the same to the Newton tolerance.  Not used for the first
     increment, after a resume, or on the retry of a failed attempt.
     replace_rtol: the mixed linear solves' FP64 residual replacement interval
-    (Context.pcg; 0: restarted refinement steps)."""
+    (Context.pcg: <= 0 the library's 0.1, >= 1 restarted refinement steps)."""
     bc_eq = np.ascontiguousarray(bc_eq, np.int32)
     bc_target = np.asarray(bc_target, np.float64)
     if isinstance(props, dict):

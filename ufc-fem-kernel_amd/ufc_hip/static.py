@@ -374,8 +374,10 @@ class Context:
             max_outer=20, x=None, b=None, x0=None, check_every=0, replace_rtol=0.0):
         """Jacobi-PCG on the assembled K; returns (x, SolveInfo).  status is
         OK, INVALID (max_iter reached, NM_Solv_Core.f90:419-422) or ERROR
-        (breakdown).  MIXED: replace_rtol > 0 selects residual replacement
-        with the search direction kept instead of restarted refinement steps."""
+        (breakdown).  MIXED: the inner FP32 PCG replaces its residual by the
+        FP64 one each time it has dropped by replace_rtol, keeping the search
+        direction (<= 0: 0.1); replace_rtol >= 1 restarts the FP32 PCG at each
+        refinement step instead."""
         return self._solve("ufc_nm_pcg", None, precision, rtol, max_iter, inner_rtol,
                            max_outer, x, b, x0, check_every, replace_rtol)
 
