@@ -429,6 +429,7 @@ def oracle_j2_replay(pj, props, target, u_incs, state_gpu):
     (stresses; equivalent and plastic strains) against the replayed one."""
     import numpy as np
     from oracle import oracle as O
+    from ufc_hip.nonlinear import to_host
     threads, _ = cpu_threads()
     nn = pj.coords.shape[0]
     state = np.zeros(pj.n_elems * 8 * 13)
@@ -436,6 +437,7 @@ def oracle_j2_replay(pj, props, target, u_incs, state_gpu):
     rels, fails = [], 0
     n = len(u_incs)
     for k, uk in enumerate(u_incs, start=1):
+        uk = to_host(uk)
         nf, state, _, fi = O.j2_internal_force(nn, pj.conn, pj.coords, props, uk, u_prev, state,
                                                nthreads=threads)
         fails += int(nf)

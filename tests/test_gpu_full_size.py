@@ -230,7 +230,7 @@ def test_j2_cube64_oracle_replay(ctx, orc):
         increment (stress, equivalent and plastic strains of all 2,097,152
         Gauss points) within 1e-9 of the oracle's."""
     from ufc_hip import problems as P
-    from ufc_hip.nonlinear import j2_static
+    from ufc_hip.nonlinear import j2_static, to_host
     pr = P.j2_cube(64)
     props = pr.extra["props"]
     target = np.zeros(pr.bc_eq.size)
@@ -246,6 +246,7 @@ def test_j2_cube64_oracle_replay(ctx, orc):
     u_prev = np.zeros(pr.n_dof)
     rels = []
     for k, uk in enumerate(res.u_increments, start=1):
+        uk = to_host(uk)
         nf, state, _, fi = orc.j2_internal_force(nn, pr.conn, pr.coords, props, uk, u_prev,
                                                  state, nthreads=_threads())
         assert nf == 0

@@ -83,7 +83,7 @@ def test_j2_predictor_same_increments(ctx, orc, precision):
     equals the plain job's (1e-7), the oracle's residual of each increment,
     with the state replayed from the GPU's increments, is below 2e-8 of
     ||F_int||, and the linear iterations drop."""
-    from ufc_hip.nonlinear import j2_static
+    from ufc_hip.nonlinear import j2_static, to_host
     pr, target = _problem(12)
     props = pr.extra["props"]
     jobs = {}
@@ -99,11 +99,13 @@ def test_j2_predictor_same_increments(ctx, orc, precision):
     print(f"[gpu-test] j2 12^3 linear iterations {lin}, Newton {newton}", flush=True)
     assert lin[True] < lin[False]
     for a, b in zip(jobs[False].u_increments, jobs[True].u_increments):
+        a, b = to_host(a), to_host(b)
         assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-7
     nn = pr.coords.shape[0]
     state = np.zeros(pr.n_elems * 8 * 13)
     u_prev = np.zeros(pr.n_dof)
     for k, uk in enumerate(jobs[True].u_increments, start=1):
+        uk = to_host(uk)
         nf, state, _, fi = orc.j2_internal_force(nn, pr.conn, pr.coords, props, uk, u_prev, state)
         assert nf == 0
         r = -fi

@@ -51,7 +51,8 @@ class NewtonResult:
     failures: list = field(default_factory=list)  # (load factor tried, reason) per cut
     j2_update_ms: list = field(default_factory=list)  # device time of every J2 update
     assemble_ms: list = field(default_factory=list)   # device time of every tangent assembly
-    u_increments: list = field(default_factory=list)  # converged u per increment (keep_increments)
+    u_increments: list = field(default_factory=list)  # converged u per increment (keep_increments;
+    #                                                   device tensors when the job ran on the GPU)
 
 
 def forcing_term(r_now, r_prev, eta_prev, target, eta_first=1e-6, eta_max=1e-3,
@@ -118,12 +119,17 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
         ctx.j2_init(*props)
         nu = props[1]
     n = ctx.n_dof
-    u = np.zeros(n)
+    # the Newton vectors live on the context's GPU (the C-ABI takes device
+    # pointers): no host round trip of u, R and du per Newton iteration
+    dev = _device(ctx)
+    u = _zeros(n, dev)
     lam = 0.0
     count = 0
     if resume_from is not None:
-        u, rs = R.load(resume_from, ctx=ctx, with_state=True)
+        u_h, rs = R.load(resume_from, ctx=ctx, with_state=True)
+        u = _to(u_h, dev)
         lam, count = rs.lambda_, rs.increment
+    r_buf, du_buf = _zeros(n, dev), _zeros(n, dev)
     step = 1.0 / n_increments
     res = NewtonResult(converged=True)
     cutbacks = 0
@@ -132,11 +138,11 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
         dlam = min(step, 1.0 - lam)
         target = lam + dlam
         ctx.apply_bc(bc_eq, bc_target * target, penalty)
-        u_n = u.copy()
+        u_n = u.clone() if dev is not None else u.copy()
         ok, its, lin, rrel, ny, rabs = False, 0, 0, np.inf, 0, np.inf
         hist = []
         predicted = predictor and du_prev is not None
-        u_try = u_n + (dlam / dlam_prev) * du_prev if predicted else u_n.copy()
+        u_try = u_n + (dlam / dlam_prev) * du_prev if predicted else u_n + 0.0
         r_prev = eta = None
         reason = f"no convergence in {max_newton} Newton iterations"
         for it in range(1, max_newton + 1):
@@ -146,7 +152,7 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
                 reason = f"local Newton failure ({e.message})"
                 break  # cut back
             res.j2_update_ms.append(ctx.timing().j2_update_ms)
-            r, norms = ctx.residual(u_try)
+            r, norms = ctx.residual(u_try, out=r_buf)
             its = it
             rabs = norms[0]
             rrel = norms[0] / norms[1] if norms[1] > 0.0 else float("inf")
@@ -162,7 +168,8 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
             r_prev = norms[0]
             ctx.assemble_k(None, nu, precision=precision, material=A.MAT_J2)
             res.assemble_ms.append(ctx.timing().assemble_ms)
-            du, info = ctx.pcg(precision=precision, rtol=eta, b=r, replace_rtol=replace_rtol)
+            du, info = ctx.pcg(precision=precision, rtol=eta, b=r, x=du_buf,
+                               replace_rtol=replace_rtol)
             lin += info.niter
             if not info.converged:
                 reason = f"linear solve status {info.status}"
@@ -181,8 +188,8 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
                 res.u_increments.append(u)
             cutbacks = 0
             if restart_file is not None and count % checkpoint_freq == 0:
-                R.save(restart_file, u, R.StepState(lam, count, lam, rabs, its, True), ctx=ctx,
-                       with_state=True)
+                R.save(restart_file, to_host(u), R.StepState(lam, count, lam, rabs, its, True),
+                       ctx=ctx, with_state=True)
         else:
             # the trial state is discarded: the next attempt starts again from
             # the committed state and u_n (RT_Step_Exec.f90:455-471)
@@ -194,5 +201,34 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
                 res.converged = False
                 break
             step *= 0.25
-    res.u = u
+    res.u = to_host(u)
     return res
+
+
+def _device(ctx):
+    """The torch device of ctx's GPU, or None (host arrays) without torch/CUDA."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    return torch.device("cuda", ctx.device) if torch.cuda.is_available() else None
+
+
+def _zeros(n, dev):
+    if dev is None:
+        return np.zeros(n)
+    import torch
+    return torch.zeros(n, dtype=torch.float64, device=dev)
+
+
+def _to(a, dev):
+    if dev is None:
+        return np.asarray(a, np.float64)
+    import torch
+    return torch.as_tensor(np.asarray(a, np.float64), device=dev)
+
+
+def to_host(a):
+    """A host float64 array of a numpy array or a (device) tensor (e.g. the
+    entries of NewtonResult.u_increments)."""
+    return a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a, np.float64)
