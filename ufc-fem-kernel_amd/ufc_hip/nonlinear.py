@@ -94,9 +94,10 @@ def j2_static(ctx: Context, bc_eq, bc_target, props, n_increments=10, precision=
     resume_from: a restart file to continue from (load factor, increment count,
     u and the J2 state); increments then resume at the nominal size
     1/n_increments.  stop_after: return after that many converged increments
-    in total (a job that ends early, for restarts).  keep_increments: a copy of
-    the converged u of every increment in res.u_increments (the path a
-    history-dependent material followed, e.g. to replay it elsewhere).
+    in total (a job that ends early, for restarts).  keep_increments: the
+    converged u of every increment in res.u_increments (the path a
+    history-dependent material followed, e.g. to replay it elsewhere; device
+    tensors when the job ran on the GPU, to_host() copies one to the host).
 
     predictor: the first Newton iterate of an increment is extrapolated from
     the previous converged increment, u_n + (dlam / dlam_prev) (u_n - u_n-1),
