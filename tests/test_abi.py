@@ -30,6 +30,18 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_abi._SIGS) == names
 
 
+def test_fortran_module_binds_every_declared_function():
+    """Every entry point include/ufc_hip.h declares has a BIND(C) interface in
+    the ISO_C_BINDING module the Fortran step drivers use, under the same
+    name, and the module binds nothing the header does not declare."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "ufc_hip.h")).read()
+    declared = set(re.findall(r"^(?:int|void|const char\*)\s+(ufc_\w+)\(", hdr, re.M))
+    f90 = open(os.path.join(ROOT, "ufc-fem-kernel_amd", "fortran", "ufc_hip_iface.f90")).read()
+    bound = set(re.findall(r"BIND\(C,\s*NAME='(ufc_\w+)'\)", f90, re.I))
+    assert declared and declared == bound, (declared - bound, bound - declared)
+
+
 def test_status_codes_mirror_reference():
     # IF_Err_Brg.f90:95-109
     from ufc_hip import _abi as A

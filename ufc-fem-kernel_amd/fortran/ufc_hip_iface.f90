@@ -212,7 +212,8 @@ MODULE ufc_hip_iface
             ufc_ph_j2_init, ufc_ph_j2_update, ufc_ph_j2_commit, ufc_ph_j2_get_state, &
             ufc_ph_j2_set_state, ufc_mesh_box, ufc_field, ufc_ap_read_inp, ufc_ap_free, &
             ufc_rt_restart_save, ufc_rt_restart_load, ufc_nm_spmv_t, ufc_rt_asm_set_values, &
-            ufc_rt_asm_set_fp32_view, ufc_rt_asm_set_fp64_view, ufc_status_message
+            ufc_rt_asm_set_fp32_view, ufc_rt_asm_set_fp64_view, ufc_status_message, &
+            ufc_ph_c3d8_eval_ke_noalloc, ufc_mesh_box_numbering
 
   INTERFACE
     INTEGER(C_INT) FUNCTION ufc_ctx_create(device, ctx, st) BIND(C, NAME='ufc_ctx_create')
@@ -394,6 +395,17 @@ MODULE ufc_hip_iface
       TYPE(ufc_status), INTENT(INOUT) :: st
     END FUNCTION
 
+    ! ufc_ph_c3d8_eval_ke without device allocation (no reference counterpart):
+    ! coords, young and ke in args must be device memory of the context's GPU
+    INTEGER(C_INT) FUNCTION ufc_ph_c3d8_eval_ke_noalloc(ctx, algo, args, st) &
+        BIND(C, NAME='ufc_ph_c3d8_eval_ke_noalloc')
+      IMPORT :: C_INT, C_PTR, ufc_ph_ke_algo, ufc_ph_ke_args, ufc_status
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ufc_ph_ke_algo), INTENT(IN) :: algo
+      TYPE(ufc_ph_ke_args), INTENT(INOUT) :: args
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
     INTEGER(C_INT) FUNCTION ufc_ph_c3d8_sensitivity(ctx, args, st) &
         BIND(C, NAME='ufc_ph_c3d8_sensitivity')
       IMPORT :: C_INT, C_PTR, ufc_ph_sens_args, ufc_status
@@ -435,6 +447,17 @@ MODULE ufc_hip_iface
       INTEGER(C_INT64_T), VALUE :: seed
       INTEGER(C_INT32_T), VALUE :: index_base
       TYPE(C_PTR), VALUE :: coords, conn
+      TYPE(ufc_status), INTENT(INOUT) :: st
+    END FUNCTION
+
+    ! lexicographic box numbering of a structured hex mesh in any node
+    ! numbering (host only): perm(n_nodes), 0-based lattice numbers; conn in
+    ! base index_base (UFC_STATUS_UNSUPPORTED when the mesh is not such a box)
+    INTEGER(C_INT) FUNCTION ufc_mesh_box_numbering(n_nodes, n_elems, index_base, conn, perm, st) &
+        BIND(C, NAME='ufc_mesh_box_numbering')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, ufc_status
+      INTEGER(C_INT32_T), VALUE :: n_nodes, n_elems, index_base
+      TYPE(C_PTR), VALUE :: conn, perm
       TYPE(ufc_status), INTENT(INOUT) :: st
     END FUNCTION
 
