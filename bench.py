@@ -70,7 +70,8 @@ def spmv32_bytes(n_dof, nnz, n_nodes, index_bytes=4, n_penalty=0):
 SPMV_EVENT_EVERY = 64
 
 # kernel name of the FP32 SpMV for each neighbour-index width
-SPMV32_KERNEL = {0: "k_spmv_sym<true>", 1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
+# (the symmetric view's kernel is k_spmv_sym<true, T> with T the layout's tile size)
+SPMV32_KERNEL = {0: "k_spmv_sym<true, {tile}>", 1: "k_spmv<float, true, 2>", 2: "k_spmv<float, true, 1>",
                  4: "k_spmv<float, true, 0>"}
 SPMV32_FORM = {0: "FP32 symmetric block-diagonal SpMV, x window in LDS, difference form",
                1: "FP32 stencil-pattern node-block SpMV, difference form",
@@ -837,7 +838,7 @@ def main():
     spmv_ms = mixed[-1]["spmv_ms"]
     b32 = spmv32_bytes(pr.n_dof, ctx.nnz, ctx.n_nodes, ctx.nb_index_bytes,
                        int(np.unique(pr.bc_eq).size))
-    k32 = SPMV32_KERNEL[ctx.nb_index_bytes]
+    k32 = SPMV32_KERNEL[ctx.nb_index_bytes].format(tile=ctx.sym_tile)
     ach = b32 / (spmv_ms * 1e6)
     copy_gbs = copy_bandwidth(torch, dev)
     out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -108,7 +108,10 @@ typedef struct ufc_rt_asm_info {
   int32_t fp32_view;      /* UFC_FP32_VIEW_*: by default the fastest the    */
                           /* mesh allows, in the order SYMDIAG, PATTERN,    */
                           /* OFF16, ID32                                     */
-  int32_t reserved;
+  int32_t sym_tile;       /* nodes per tile of the SYMDIAG layout (one per  */
+                          /* thread of the FP32 SpMV's workgroup): 256, or  */
+                          /* 512 on meshes of at least 800,000 nodes; 0     */
+                          /* when the mesh has no SYMDIAG view              */
 } ufc_rt_asm_info;
 int ufc_ctx_info(ufc_ctx* ctx, ufc_rt_asm_info* info, ufc_status* st);
 

@@ -6,8 +6,9 @@ same seeded mesh and field:
   - row_ptr / col_idx bit-exact with orc_csr_pattern (RT_CSR_FromTripletMerged
     order, RT_Solv_Sparse.f90:336-447);
   - FP64 values within 1e-12 max|K| of orc_assemble;
-  - the FP32 SpMV of the symmetric block-diagonal view (66 zero tiles in
-    front at this size, int32 address arithmetic over 1e9 entries) within
+  - the FP32 SpMV of the symmetric block-diagonal view (512-node tiles at
+    this size, 33 zero tiles in front, int32 address arithmetic over 1e9
+    entries) within
     1e-5 max|y| of the oracle's FP64 K x;
   - the FP64 and the mixed-precision GPU solutions: the oracle's true FP64
     residual ||F - K u|| / ||F|| below 2e-10 for both (NM_Solver_PCG stops

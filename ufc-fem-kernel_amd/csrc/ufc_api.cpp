@@ -335,6 +335,7 @@ bool collect_sym_offsets(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
 // (the SpMV then gathers x from global memory).
 void build_x_window(ufc_ctx* c) {
   c->have_xwin = false;
+  const int32_t tile = c->diag_tile;
   std::vector<int32_t> offs{0};
   for (int s = 0; s < c->n_diag; ++s) {
     offs.push_back(c->h_diag_off[s]);
@@ -346,19 +347,19 @@ void build_x_window(ufc_ctx* c) {
   size_t i = 0;
   while (i < offs.size()) {
     size_t j = i;
-    while (j + 1 < offs.size() && offs[j + 1] - offs[j] <= ufc::kDiagTile) ++j;
+    while (j + 1 < offs.size() && offs[j + 1] - offs[j] <= tile) ++j;
     if (w.nc == ufc::kMaxXClusters) return;
     w.lo[w.nc] = offs[i];
     w.base[w.nc] = total;
-    w.len[w.nc] = offs[j] - offs[i] + ufc::kDiagTile;
+    w.len[w.nc] = offs[j] - offs[i] + tile;
     total += w.len[w.nc];
     ++w.nc;
     i = j + 1;
   }
-  if (total > ufc::kMaxXWin) return;
+  if (total > ufc::max_xwin(tile)) return;
   auto slot = [&](int32_t o) {
     for (int k = 0; k < w.nc; ++k)
-      if (o >= w.lo[k] && o < w.lo[k] + w.len[k] - ufc::kDiagTile + 1) return w.base[k] + o - w.lo[k];
+      if (o >= w.lo[k] && o < w.lo[k] + w.len[k] - tile + 1) return w.base[k] + o - w.lo[k];
     return -1;
   };
   w.self = slot(0);
@@ -373,6 +374,7 @@ void build_x_window(ufc_ctx* c) {
 
 void reset_sym_diagonals(ufc_ctx* c) {
   c->n_diag = 0;
+  c->diag_tile = ufc::kDiagTile;
   c->sym_permuted = false;
   c->h_sym_perm.clear();
   c->sym_perm.release();
@@ -398,11 +400,12 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
   }
   if (all.empty()) return;
   std::sort(all.begin(), all.end());
-  // tile-major entries with ceil(o_last / kDiagTile) zero tiles in front; the
+  // tile-major entries with ceil(o_last / tile) zero tiles in front; the
   // SpMV indexes them with 32-bit integers
-  const int64_t pad_tiles = (int64_t(all.back()) + ufc::kDiagTile - 1) / ufc::kDiagTile;
-  const int64_t tiles = (nn + ufc::kDiagTile - 1) / ufc::kDiagTile + pad_tiles;
-  const int64_t entries = tiles * int64_t(all.size()) * ufc::kDiagTile;
+  const int64_t tile = nn >= ufc::kDiagWideMinNodes ? ufc::kDiagTileWide : ufc::kDiagTile;
+  const int64_t pad_tiles = (int64_t(all.back()) + tile - 1) / tile;
+  const int64_t tiles = (nn + tile - 1) / tile + pad_tiles;
+  const int64_t entries = tiles * int64_t(all.size()) * tile;
   if (entries >= (int64_t(1) << 31)) {
     c->sym_permuted = false;
     c->h_sym_perm.clear();
@@ -411,6 +414,7 @@ void build_sym_diagonals(ufc_ctx* c, const std::vector<int32_t>& nb_idx,
   c->n_diag = int32_t(all.size());
   for (int s = 0; s < ufc::kMaxDiags; ++s) c->h_diag_off[s] = s < c->n_diag ? all[size_t(s)] : 0;
   c->diag_pad_tiles = int32_t(pad_tiles);
+  c->diag_tile = int32_t(tile);
   build_x_window(c);
   c->diag_entries = entries;
   if (c->sym_permuted) {
@@ -621,7 +625,7 @@ int ufc_ctx_info(ufc_ctx* c, ufc_rt_asm_info* info, ufc_status* st) {
     info->max_row_nnz = c->max_row_nnz;
     info->nb_index_bytes = nb_index_bytes(c);
     info->fp32_view = c->fp32_view;
-    info->reserved = 0;
+    info->sym_tile = c->n_diag > 0 ? c->diag_tile : 0;
     return UFC_STATUS_OK;
   });
 }
@@ -1105,7 +1109,7 @@ int ufc_rt_asm_build_pattern(ufc_ctx* c, const ufc_mesh_desc* m, ufc_rt_asm_info
       info->max_row_nnz = c->max_row_nnz;
       info->nb_index_bytes = nb_index_bytes(c);
       info->fp32_view = c->fp32_view;
-      info->reserved = 0;
+      info->sym_tile = c->n_diag > 0 ? c->diag_tile : 0;
     }
     msg = "pattern built: nnz " + std::to_string(nnz);
     return UFC_STATUS_OK;

@@ -662,6 +662,7 @@ struct DiagWrite {
   DiagOffsets off;
   int nd;
   int32_t pad_tiles;
+  int32_t tile_shift;  // log2 of the layout's tile size
   const int32_t* perm;
   const int32_t* iperm;
   const int32_t* nb_idx;
@@ -849,7 +850,9 @@ __global__ __launch_bounds__(64) void k_assemble(
         if (q < 0) continue;  // no neighbour at this offset: the entry stays zero
         const T* src = acc + 9 * (p0 - q0) + 3 * q;
         const int d3 = 3 * dg;
-        const int64_t e = ((bn / kDiagTile + dw.pad_tiles) * dw.nd + sd) * kDiagTile + bn % kDiagTile;
+        const int64_t e =
+            ((((bn >> dw.tile_shift) + dw.pad_tiles) * dw.nd + sd) << dw.tile_shift) +
+            (bn & ((int64_t(1) << dw.tile_shift) - 1));
         dw.da[e] = make_float4(src[0], src[1], src[2], src[d3]);
         dw.db[e] = make_float4(src[d3 + 1], src[d3 + 2], src[2 * d3], src[2 * d3 + 1]);
         dw.dc[e] = src[2 * d3 + 2];
@@ -1012,7 +1015,8 @@ __global__ __launch_bounds__(256) void k_offdiag_compact(int32_t n_nodes,
 // Symmetric block-diagonal values from the difference-form val32: a 32-lane
 // group per node, lane k on neighbour m = nb_idx[q0 + k] > n copies block
 // K(n, m) (node n's rows d, columns 3 m + c) to entry (s, n), o_s = m - n:
-// tile-major, ((n / kDiagTile + pad_tiles) nd + s) kDiagTile + n % kDiagTile,
+// tile-major, ((n / tile + pad_tiles) nd + s) tile + n % tile (tile =
+// 2^tile_shift nodes),
 // as float4 (0,0) (0,1) (0,2) (1,0), float4 (1,1) (1,2) (2,0) (2,1) and
 // float (2,2).  The pairs of the pattern are fixed, so the entries never
 // written stay zero from the allocation.
@@ -1023,6 +1027,7 @@ __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
                                                    const int32_t* __restrict__ nb_idx,
                                                    const float* __restrict__ val, DiagOffsets off,
                                                    int nd, int32_t pad_tiles,
+                                                   int32_t tile_shift,
                                                    const int32_t* __restrict__ perm,
                                                    float4* __restrict__ da,
                                                    float4* __restrict__ db,
@@ -1049,7 +1054,8 @@ __global__ __launch_bounds__(256) void k_diag_fill(int32_t n_nodes,
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) v[3 * d + cc] = val[src + d * s3 + cc];
-    const int64_t e = ((n / kDiagTile + pad_tiles) * nd + s) * kDiagTile + n % kDiagTile;
+    const int64_t e = ((((n >> tile_shift) + pad_tiles) * nd + s) << tile_shift) +
+                      (n & ((int64_t(1) << tile_shift) - 1));
     da[e] = make_float4(v[0], v[1], v[2], v[3]);
     db[e] = make_float4(v[4], v[5], v[6], v[7]);
     dc[e] = v[8];
@@ -1714,6 +1720,7 @@ DiagWrite diag_write(ufc_ctx* c, bool fp32) {
   for (int s = 0; s < kMaxDiags; ++s) dw.off.o[s] = c->h_diag_off[s];
   dw.nd = c->n_diag;
   dw.pad_tiles = c->diag_pad_tiles;
+  dw.tile_shift = tile_shift(c->diag_tile);
   dw.perm = fp32_permuted(c) ? c->sym_perm.get() : nullptr;
   dw.iperm = fp32_permuted(c) ? c->sym_iperm.get() : nullptr;
   dw.nb_idx = c->nb_idx.get();
@@ -1858,7 +1865,8 @@ void launch_fp32_offdiag(ufc_ctx* c) {
       for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
       hipLaunchKernelGGL(k_diag_fill, dim3(cgrid), dim3(256), 0, c->stream, c->n_nodes,
                          c->nb_ptr.get(), c->nb_idx.get(), c->val32.get(), off, int(c->n_diag),
-                         c->diag_pad_tiles, perm ? c->sym_perm.get() : nullptr,
+                         c->diag_pad_tiles, tile_shift(c->diag_tile),
+                         perm ? c->sym_perm.get() : nullptr,
                          reinterpret_cast<float4*>(c->diag_a.get()),
                          reinterpret_cast<float4*>(c->diag_b.get()), c->diag_c.get());
       c->diag_from_assembly = true;  // filled now: a later penalty change keeps them

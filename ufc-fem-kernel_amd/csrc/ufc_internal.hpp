@@ -95,8 +95,15 @@ constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP
 // doubles per Gauss point of the stored J2 tangent (mu', lam', c, n(6)):
 // Dt = lam' 1x1 + 2 mu' I_s - c n x n
 constexpr int kJ2Tangent = 9;
-constexpr int kDiagThreads = 256;   // symmetric FP32 SpMV: one node per thread
-constexpr int kDiagTile = 256;      // nodes per tile of the block-diagonal layout (= kDiagThreads)
+// nodes per tile of the symmetric view's block-diagonal layout, one per
+// thread of the SpMV's workgroup: kDiagTile, or kDiagTileWide on meshes of at
+// least kDiagWideMinNodes nodes (larger tiles keep more of the transposed
+// re-reads one node row back inside the workgroup and stage less x window
+// per node, but halve the workgroups: per PCG iteration 0.5% slower at 88^3,
+// 2% faster at 96^3, 1.2% at 128^3, 19% slower at 64^3)
+constexpr int kDiagTile = 256;
+constexpr int kDiagTileWide = 512;
+constexpr int64_t kDiagWideMinNodes = 800000;
 // tiles one "plane" (ceil(o_last / kDiagTile) tiles) apart that run in a row
 // (128^3 FP32 SpMV: 8 0.276 ms, 16 0.263, 32 0.270)
 constexpr int kDiagPlanes = 16;
@@ -118,7 +125,15 @@ struct DiagOffsets {
 // LDS node slot base[c]; up[s], dn[s] and self are the slots of n + o_s,
 // n - o_s and n for the tile's first node (lane l adds l).
 constexpr int kMaxXClusters = 4;
-constexpr int kMaxXWin = 1664;  // nodes (16 B each: 26 KB); a structured box needs 3 x 516
+// window nodes (16 B each) per tile size: 26 KB for 256-node tiles (a
+// structured box needs 3 x 516), 38 KB for 512-node tiles (3 x 772)
+constexpr int kMaxXWin = 1664;
+constexpr int kMaxXWinWide = 2432;
+__host__ __device__ constexpr int max_xwin(int tile) {
+  return tile == kDiagTileWide ? kMaxXWinWide : kMaxXWin;
+}
+inline int32_t tile_shift(int32_t tile) { return tile == kDiagTileWide ? 9 : 8; }
+static_assert(kDiagTile == 256 && kDiagTileWide == 512, "tile_shift");
 struct XWindow {
   int32_t nc;
   int32_t lo[kMaxXClusters], base[kMaxXClusters], len[kMaxXClusters];
@@ -218,16 +233,17 @@ struct ufc_ctx {
   // block is stored once, for its upper node: K(n, n + o_s) is entry (s, n),
   // zero where n + o_s is no neighbour.  Node n's lower block
   // K(n, n - o_s) = K(n - o_s, n)^T is entry (s, n - o_s).  Tile-major: the
-  // entries of the kDiagTile nodes of tile T = n / kDiagTile + diag_pad_tiles
-  // for diagonal s are the kDiagTile consecutive entries from
-  // (T n_diag + s) kDiagTile; diag_pad_tiles = ceil(o_last / kDiagTile) zero
+  // entries of the diag_tile nodes of tile T = n / diag_tile + diag_pad_tiles
+  // for diagonal s are the diag_tile consecutive entries from
+  // (T n_diag + s) diag_tile; diag_pad_tiles = ceil(o_last / diag_tile) zero
   // tiles in front keep the lower reads of the first nodes in bounds.
   // Entries: diag_a float4 (0,0) (0,1) (0,2) (1,0), diag_b float4 (1,1)
   // (1,2) (2,0) (2,1), diag_c float (2,2) -- three loads per block.
   int32_t n_diag = 0;
   int32_t h_diag_off[ufc::kMaxDiags] = {};
   int32_t diag_pad_tiles = 0;
-  int64_t diag_entries = 0;     // (n_nodes / kDiagTile + 1 + diag_pad_tiles) n_diag kDiagTile
+  int32_t diag_tile = ufc::kDiagTile;  // kDiagTile or kDiagTileWide (a power of two)
+  int64_t diag_entries = 0;     // (n_nodes / diag_tile + 1 + diag_pad_tiles) n_diag diag_tile
   ufc::DBuf<float> diag_a;      // (diag_entries, 4)
   ufc::DBuf<float> diag_b;      // (diag_entries, 4)
   ufc::DBuf<float> diag_c;      // (diag_entries)

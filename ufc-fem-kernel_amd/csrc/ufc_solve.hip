@@ -575,7 +575,7 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 // is replaced by n: its block is zero, and so is x_n - x_n.  Each block is
 // three loads (float4, float4, float); measured 0.26 ms at 128^3 against
 // 0.38 ms for the same one-node-per-thread kernel over all 26 stored blocks.
-// Tile order: a workgroup takes one kDiagTile-node tile t at a time.  Tiles
+// Tile order: a workgroup takes one TILE-node tile t at a time.  Tiles
 // are visited in blocks of kDiagPlanes "planes" of F = diag_pad_tiles tiles
 // (F tiles span the largest offset, one node plane of a structured box):
 // within a block, the tiles p, p + F, p + 2F, ... run one after another, so
@@ -589,7 +589,7 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 // pap != null: the workgroup partials of p.Ap as in k_spmv (fixed order,
 // barrier-free LDS combine).
 // XW: the tile's x window (XWindow: the 27 neighbour offsets' nodes as a few
-// runs of consecutive nodes, 1548 nodes on a structured box) is first loaded
+// runs of consecutive nodes, 3 (TILE + 260) nodes on a 128^3 box) is first loaded
 // into LDS with coalesced loads, and every x_{n +- o_s} and x_n is then an LDS
 // read: 27 global gathers per node become 6 streaming loads per thread.
 // Window nodes outside the mesh hold 0; their blocks are zero.
@@ -606,8 +606,8 @@ __device__ __forceinline__ void store3(float* __restrict__ p, float a, float b, 
 struct SymOrder {
   int32_t F, nfull, rem, npos;
 };
-__host__ __device__ inline SymOrder sym_order(int32_t n_nodes, int32_t F) {
-  const int32_t ntiles = (n_nodes + kDiagTile - 1) / kDiagTile;
+__host__ __device__ inline SymOrder sym_order(int32_t n_nodes, int32_t F, int32_t tile) {
+  const int32_t ntiles = (n_nodes + tile - 1) / tile;
   const int32_t P = (ntiles + F - 1) / F;
   return SymOrder{F, P / kDiagPlanes, P % kDiagPlanes, P * F};
 }
@@ -624,25 +624,25 @@ __device__ __forceinline__ int32_t sym_tile(const SymOrder& o, int32_t w) {
 // 0.272 ms; one at a time, as the compiler schedules it unstaged, drains the
 // loads after every diagonal)
 constexpr int kDiagBatch = 4;
-template <bool XW>
-__global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
+// TILE: the layout's tile size, one node per thread (ufc_ctx::diag_tile)
+template <bool XW, int TILE>
+__global__ __launch_bounds__(TILE) void k_spmv_sym(
     int32_t n_nodes, DiagOffsets off, int nd, int32_t pad_tiles, XWindow win,
     const float4* __restrict__ da, const float4* __restrict__ db, const float* __restrict__ dc,
     const float* __restrict__ pen, const uint32_t* __restrict__ pen_flag,
     const float* __restrict__ x, float* __restrict__ y, const PcgScalars* __restrict__ sc,
     double* __restrict__ pap) {
   if (sc && sc->done) return;
-  constexpr int kWaves = kDiagThreads / WAVE;
+  constexpr int kWaves = TILE / WAVE;
   __shared__ double slot[kWaves];
   __shared__ int arrived;
-  __shared__ float4 xw[XW ? kMaxXWin : 1];
+  __shared__ float4 xw[XW ? max_xwin(TILE) : 1];
   if (pap) {
     if (threadIdx.x == 0) arrived = 0;
     __syncthreads();
   }
-  static_assert(kDiagTile == kDiagThreads, "one node per thread and tile entry");
-  const int32_t ntiles = (n_nodes + kDiagTile - 1) / kDiagTile;
-  const SymOrder ord = sym_order(n_nodes, pad_tiles);
+  const int32_t ntiles = (n_nodes + TILE - 1) / TILE;
+  const SymOrder ord = sym_order(n_nodes, pad_tiles, TILE);
   const int32_t npos = ord.npos;
   const int32_t ppx = (npos + 7) / 8;
   const int32_t lo = int32_t(blockIdx.x % 8) * ppx;
@@ -653,20 +653,20 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
     const int32_t t = sym_tile(ord, w);
     // 32-bit indices: n < n_nodes < 2^31 and the entries < 2^31 (the host
     // checks), which keeps the address arithmetic off 64-bit VALU pairs
-    const int32_t n = t * kDiagTile + int32_t(threadIdx.x);
+    const int32_t n = t * TILE + int32_t(threadIdx.x);
     if (t >= ntiles) continue;  // uniform over the workgroup
     const int tid = int(threadIdx.x);
     if constexpr (XW) {
       // all of a thread's window loads are issued before the LDS writes (in
       // a loop of load, store per slot each load waited for the previous
       // one: 244.4 -> 243.1 us at 128^3)
-      const int32_t t0 = t * kDiagTile;
+      const int32_t t0 = t * TILE;
       const int32_t total = win.base[win.nc - 1] + win.len[win.nc - 1];
-      constexpr int kPer = (kMaxXWin + kDiagThreads - 1) / kDiagThreads;
+      constexpr int kPer = (max_xwin(TILE) + TILE - 1) / TILE;
       float3 q[kPer];
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
-        const int32_t j = tid + k * kDiagThreads;
+        const int32_t j = tid + k * TILE;
         int32_t m = -1;
 #pragma unroll
         for (int c = 0; c < kMaxXClusters; ++c)
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
       }
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
-        const int32_t j = tid + k * kDiagThreads;
+        const int32_t j = tid + k * TILE;
         if (j < total) xw[j] = make_float4(q[k].x, q[k].y, q[k].z, 0.f);
       }
       __syncthreads();
@@ -721,9 +721,9 @@ __global__ __launch_bounds__(kDiagThreads) void k_spmv_sym(
           xu[g] = load3(x + 3 * mu);
           xl[g] = load3(x + 3 * ml);
         }
-        const int32_t e = (tu + se) * kDiagTile + int32_t(threadIdx.x);
-        const int32_t m = n - o + pad_tiles * kDiagTile;  // >= 0: o <= pad_tiles kDiagTile
-        const int32_t el = ((m / kDiagTile) * nd + se) * kDiagTile + m % kDiagTile;
+        const int32_t e = (tu + se) * TILE + int32_t(threadIdx.x);
+        const int32_t m = n - o + pad_tiles * TILE;  // >= 0: o <= pad_tiles TILE
+        const int32_t el = ((m / TILE) * nd + se) * TILE + m % TILE;
         ua[g] = da[e];
         ub[g] = db[e];
         u8[g] = dc[e];
@@ -1310,7 +1310,7 @@ int vec_blocks(int64_t n) {
 // symmetric block-diagonal SpMV grid: 8 x the tile positions per XCD, capped (a
 // multiple of 8 either way)
 int sym_blocks(const ufc_ctx* c) {
-  const int64_t npos = sym_order(c->n_nodes, c->diag_pad_tiles).npos;
+  const int64_t npos = sym_order(c->n_nodes, c->diag_pad_tiles, c->diag_tile).npos;
   const int64_t ppx = (npos + 7) / 8;
   return int(8 * std::min<int64_t>(ppx, kSpmvBlocks32 / 8));
 }
@@ -1330,18 +1330,23 @@ void launch_spmv(ufc_ctx* c, const T* val, const T* x, T* y, bool in_pcg,
     if (c->fp32_view == kFp32ViewSymDiag) {
       DiagOffsets off;
       for (int s = 0; s < kMaxDiags; ++s) off.o[s] = c->h_diag_off[s];
-      if (c->have_xwin)
-        hipLaunchKernelGGL(k_spmv_sym<true>, dim3(sym_blocks(c)), dim3(kDiagThreads), 0,
-                           c->stream, c->n_nodes, off, int(c->n_diag), c->diag_pad_tiles,
-                           c->xwin, reinterpret_cast<const float4*>(c->diag_a.get()),
+      auto launch = [&](auto kernel, int threads) {
+        hipLaunchKernelGGL(kernel, dim3(sym_blocks(c)), dim3(threads), 0, c->stream, c->n_nodes,
+                           off, int(c->n_diag), c->diag_pad_tiles, c->xwin,
+                           reinterpret_cast<const float4*>(c->diag_a.get()),
                            reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
                            c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
-      else
-        hipLaunchKernelGGL(k_spmv_sym<false>, dim3(sym_blocks(c)), dim3(kDiagThreads), 0,
-                           c->stream, c->n_nodes, off, int(c->n_diag), c->diag_pad_tiles,
-                           c->xwin, reinterpret_cast<const float4*>(c->diag_a.get()),
-                           reinterpret_cast<const float4*>(c->diag_b.get()), c->diag_c.get(),
-                           c->pen32.get(), c->pen_flag.get(), x, y, sc, pap);
+      };
+      if (c->diag_tile == kDiagTileWide) {
+        if (c->have_xwin)
+          launch(k_spmv_sym<true, kDiagTileWide>, kDiagTileWide);
+        else
+          launch(k_spmv_sym<false, kDiagTileWide>, kDiagTileWide);
+      } else if (c->have_xwin) {
+        launch(k_spmv_sym<true, kDiagTile>, kDiagTile);
+      } else {
+        launch(k_spmv_sym<false, kDiagTile>, kDiagTile);
+      }
       return;
     }
     // the stencil-pattern view reads the off-diagonal copy of val (val32m)

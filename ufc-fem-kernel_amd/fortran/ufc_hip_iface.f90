@@ -61,7 +61,7 @@ MODULE ufc_hip_iface
     INTEGER(C_INT32_T) :: max_row_nnz = 0
     INTEGER(C_INT32_T) :: nb_index_bytes = 0
     INTEGER(C_INT32_T) :: fp32_view = 0
-    INTEGER(C_INT32_T) :: reserved = 0
+    INTEGER(C_INT32_T) :: sym_tile = 0
   END TYPE ufc_rt_asm_info
 
   TYPE, BIND(C), PUBLIC :: ufc_rt_asm_algo

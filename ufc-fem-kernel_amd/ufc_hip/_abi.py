@@ -72,7 +72,7 @@ class AsmInfo(C.Structure):
     _fields_ = [("n_dof", C.c_int64), ("nnz", C.c_int64), ("n_nodes", C.c_int32),
                 ("n_elems", C.c_int32), ("max_row_nnz", C.c_int32),
                 ("nb_index_bytes", C.c_int32), ("fp32_view", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("sym_tile", C.c_int32)]
 
 
 class KeAlgo(C.Structure):

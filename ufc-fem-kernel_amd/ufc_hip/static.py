@@ -179,6 +179,7 @@ class Context:
         self.n_nodes, self.n_elems = info.n_nodes, info.n_elems
         self.nb_index_bytes = info.nb_index_bytes
         self.fp32_view = info.fp32_view
+        self.sym_tile = info.sym_tile
         return info
 
     def set_fp32_view(self, view):
