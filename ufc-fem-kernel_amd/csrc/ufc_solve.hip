@@ -858,11 +858,18 @@ __device__ __forceinline__ void report(int32_t* prog, int slot, int32_t v) {
   if (prog) __hip_atomic_store(&prog[slot], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// p.Ap partials the update pass adds itself (per thread kFusedPapPer of them,
+// one launch less per iteration than k_fin_pap's pieces; the 128^3
+// symmetric SpMV's 4224: 290.2 -> 289.6 us per iteration, 34 KB from L2 per
+// workgroup)
+constexpr int kFusedPapPer = 18;
+constexpr int kMaxFusedPap = kFusedPapPer * kSolveThreads;
+
 // alpha of iteration it = (r.z of its iterate) / p.Ap, with p.Ap the ordered
 // sum of the kFinPapBlocks pieces (the same in every workgroup), or, nspart >
-// 0, the ordered sum of the nspart <= kSolveBlocks workgroup partials the
-// p.Ap pass wrote, which every workgroup adds itself (one launch less per
-// iteration; 64^3: 1088 partials, 8.7 KB from L2 per workgroup); p.Ap <= 0 or
+// 0, the ordered sum of the nspart <= kMaxFusedPap workgroup partials the
+// p.Ap pass wrote, which every workgroup adds itself (64^3: 1088 partials,
+// 8.7 KB from L2 per workgroup); p.Ap <= 0 or
 // not finite is a breakdown (done = 2, set by workgroup 0).  r -= alpha Ap
 // and the partials of r.r and r.z with z = r/diag; the iterate is updated by
 // the next pass (k_xpupdate), which reads p anyway.  Workgroup 0 leaves
@@ -900,7 +907,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_update(int64_t n, T* __restri
   if (nspart > 0) {
     __shared__ double sh_pap;
     double s[1];
-    sum_partials_bs<1, kSolveThreads, kSolveBlocks / kSolveThreads>(s, spart, nspart);
+    sum_partials_bs<1, kSolveThreads, kFusedPapPer>(s, spart, nspart);
     if (threadIdx.x == 0) sh_pap = s[0];
     __syncthreads();
     pap = sh_pap;
@@ -1429,12 +1436,12 @@ int pcg_run(ufc_ctx* c, const double* b, X* x, const T* ax_or_null, T* r, T* p, 
   UFC_HIP_CHECK(hipGetLastError());
   const int64_t first_launch = c->spmv_launches;
   // p.Ap partials: written by the FP32 SpMV itself (its grid), or by
-  // k_dot_pap (FP64, the vector grid); at most kSolveBlocks of them are added
+  // k_dot_pap (FP64, the vector grid); at most kMaxFusedPap of them are added
   // by the update pass directly, more go through k_fin_pap's eight pieces
   const int nvb = vec_blocks<T>(n);
   double* spart = c->spmv_partials.get();
   const int npap = BLOCK_INDEX ? fp32_spmv_blocks(c) : nvb;
-  const int fused = npap <= kSolveBlocks ? npap : 0;
+  const int fused = npap <= kMaxFusedPap ? npap : 0;
   auto launch_iteration = [&](int it) {
     spmv_event_begin(c);
     if constexpr (BLOCK_INDEX) {
