@@ -428,42 +428,83 @@ __device__ __forceinline__ void elem_rows(const T (&x)[8][3], int a, T lam, T mu
     }
     iso_blocks_from_p(P, lam, mu, kb);
   } else if constexpr (VAR == kVarReduced) {
-    T g0[8][3], dj0;
-    if (c3d8_dndx_at(x, T(0), T(0), T(0), g0, dj0)) {
-      T ga[3];
-      UFC_SELECT_ROW(g0, a, ga);
-      add_iso_blocks(g0, ga, T(8) * lam * dj0, T(8) * mu * dj0, kb);
-    } else {
+    // everything from the trilinear map in the adjugate form of the standard
+    // element (no gradient tables: with them the FP64 kernel ran at one wave
+    // per SIMD)
+    const TriMap<T> m = tri_map(x);
+    // x only enters the hourglass terms through hx = sum_n h_n x_n of the four
+    // base vectors
+    T hx[4][3];
 #pragma unroll
-      for (int b = 0; b < 8; ++b)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) g0[b][j] = T(0);
-    }
+      for (int i = 0; i < 3; ++i) {
+        T t = T(0);
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+          const int hv = q == 0 ? -XS[n] * YS[n] * ZS[n]
+                                : (q == 1 ? YS[n] * ZS[n] : (q == 2 ? XS[n] * ZS[n] : XS[n] * YS[n]));
+          t += T(hv) * x[n][i];
+        }
+        hx[q][i] = t;
+      }
+    // centre point and element volume first: the trilinear map is then no
+    // longer needed while the 8 row blocks accumulate
+    T A0[3][3], d0;
+    const bool ok0 = tri_point(m, T(0), T(0), T(0), A0, d0);
+    // element volume from the 2x2x2 points (the reference's hourglass scale)
     T V = T(0);
-#pragma unroll
+#pragma unroll 1
     for (int gp = 0; gp < 8; ++gp) {
-      T g[8][3], dj;
-      c3d8_dndx(x, gp, g, dj);
-      V += dj;
+      T A[3][3], d;
+      tri_point(m, (gp & 4) ? T(kGaussPt) : T(-kGaussPt), (gp & 2) ? T(kGaussPt) : T(-kGaussPt),
+                (gp & 1) ? T(kGaussPt) : T(-kGaussPt), A, d);
+      V += d;
+    }
+    // the one point at the centre, weight 8: kb[b] += 8 d0 (lam ga gb^T +
+    // mu gb ga^T + mu (ga . gb) I) with the centre gradients g = A0^T s / (8 d0)
+    if (ok0) {
+      const T sa[3] = {sx, sy, sz};
+      T ga[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ga[j] = A0[0][j] * sa[0] + A0[1][j] * sa[1] + A0[2][j] * sa[2];
+      // with h = A0^T s = 8 d0 g: 8 d0 g_a g_b^T = h_a h_b^T / (8 d0)
+      const T w = T(0.125) / d0;
+      const T la = w * lam, mw = w * mu;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const T sb[3] = {T(XS[b]), T(YS[b]), T(ZS[b])};
+        T gb[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gb[j] = A0[0][j] * sb[0] + A0[1][j] * sb[1] + A0[2][j] * sb[2];
+        const T dd = mw * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            kb[b][i][j] += la * ga[i] * gb[j] + mw * ga[j] * gb[i] + (i == j ? dd : T(0));
+      }
     }
     if (V <= T(1.0e-20)) return;
     const T L = T(cbrt(double(V)));
     const T khg = T(0.1) * mu * V / (L * L);
+    // centre gradients g0_n = (1/8) A0^T s_n / d0 (s_n the node signs; zero when
+    // the centre point is degenerate, where A0 is only partly formed)
+    const T r0 = ok0 ? T(0.125) / d0 : T(0);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      T h[8], hx[3] = {T(0), T(0), T(0)};
+    for (int q = 0; q < 4; ++q) {
+      // hx . g0_n = (A0 hx) . s_n r0
+      T ah[3];
 #pragma unroll
-      for (int n = 0; n < 8; ++n) {
-        const int hv = m == 0 ? -XS[n] * YS[n] * ZS[n]
-                              : (m == 1 ? YS[n] * ZS[n] : (m == 2 ? XS[n] * ZS[n] : XS[n] * YS[n]));
-        h[n] = T(hv);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) hx[i] += h[n] * x[n][i];
-      }
+      for (int k = 0; k < 3; ++k)
+        ah[k] = ok0 ? r0 * (A0[k][0] * hx[q][0] + A0[k][1] * hx[q][1] + A0[k][2] * hx[q][2])
+                    : T(0);
       T gam[8], gma = T(0);
 #pragma unroll
       for (int n = 0; n < 8; ++n) {
-        gam[n] = h[n] - (hx[0] * g0[n][0] + hx[1] * g0[n][1] + hx[2] * g0[n][2]);
+        const int hv = q == 0 ? -XS[n] * YS[n] * ZS[n]
+                              : (q == 1 ? YS[n] * ZS[n] : (q == 2 ? XS[n] * ZS[n] : XS[n] * YS[n]));
+        gam[n] = T(hv) - (ah[0] * T(XS[n]) + ah[1] * T(YS[n]) + ah[2] * T(ZS[n]));
         gma = (a == n) ? gam[n] : gma;
       }
 #pragma unroll
