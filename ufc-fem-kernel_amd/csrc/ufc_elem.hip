@@ -1547,7 +1547,8 @@ __device__ __forceinline__ void j2_tangent(double K, double G, double th1, doubl
   for (int i = 0; i < 6; ++i) tc[3 + i] = n[i];
 }
 
-__device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* st, double* tc) {
+__device__ __forceinline__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* st,
+                                         double* tc) {
   const double E = pr.young, nu = pr.nu;
   const double mu = E / (2.0 * (1.0 + nu));
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
@@ -1620,10 +1621,16 @@ __device__ int j2_point(const ufc_ph_j2_props& pr, const double* deps, double* s
 // staged in LDS first, one node slot per lane (slot 8e + a is node a of
 // element e, the same index range as the points), so each lane gathers one
 // node instead of all eight of its element (0.327 -> 0.300 ms per update at
-// 64^3).  The state and tangent are read and written directly: staged through
-// LDS as well, their loads would all have to land before a barrier instead of
-// overlapping the Jacobian work (0.37 ms).
+// 64^3).  The workgroup's points hold one contiguous range of the state
+// (13 doubles each) and of the tangent (kJ2Tangent each): it is read and
+// written with 16-byte accesses that are contiguous across the lanes and
+// passed through LDS, instead of 13 + 13 + 9 loads and stores per lane 104 or
+// 72 bytes apart.
 constexpr int kJ2Threads = 128;
+constexpr int kJ2StateV = kJ2Vals * kJ2Threads / 2;     // 16-byte words of state per workgroup
+constexpr int kJ2TanV = kJ2Tangent * kJ2Threads / 2;   // and of tangent
+static_assert(kJ2Threads % 8 == 0 && (kJ2Vals * 8) % 2 == 0 && (kJ2Tangent * 8) % 2 == 0,
+              "whole elements per workgroup, whole 16-byte words per element");
 __global__ __launch_bounds__(kJ2Threads) void k_j2_update(int32_t n_elems,
                                                          const double* __restrict__ coords,
                                                          const int32_t* __restrict__ conn,
@@ -1634,13 +1641,30 @@ __global__ __launch_bounds__(kJ2Threads) void k_j2_update(int32_t n_elems,
                                                          double* __restrict__ trial,
                                                          double* __restrict__ tangent,
                                                          int32_t* __restrict__ flags) {
-  static_assert(kJ2Threads % 8 == 0, "whole elements per workgroup");
-  __shared__ double sx[kJ2Threads][3];
-  __shared__ double sd[kJ2Threads][3];
+  // LDS: the state records (committed in, trial out), and the nodes'
+  // coordinates and u - u_n, whose space the tangent records take over once
+  // the points are computed (22.5 KB: six workgroups, three waves per SIMD)
+  constexpr int kBufV = kJ2TanV > 3 * kJ2Threads ? kJ2TanV : 3 * kJ2Threads;
+  __shared__ double2 sst[kJ2StateV];
+  __shared__ double2 stn[kBufV];
+  double (*const sx)[3] = reinterpret_cast<double (*)[3]>(stn);
+  double (*const sd)[3] = reinterpret_cast<double (*)[3]>(stn) + kJ2Threads;
   const int64_t npts = int64_t(n_elems) * 8;
-  const int64_t idx = int64_t(blockIdx.x) * kJ2Threads + threadIdx.x;
+  const int64_t first = int64_t(blockIdx.x) * kJ2Threads;
   const int t = int(threadIdx.x);
-  if (idx < npts) {
+  const int64_t idx = first + t;
+  const bool active = idx < npts;
+  // the workgroup's points: a multiple of 8, so whole 16-byte words
+  const int np = int(npts - first < kJ2Threads ? npts - first : kJ2Threads);
+  const int nsv = kJ2Vals * np / 2, ntv = kJ2Tangent * np / 2;
+  constexpr int kPer = (kJ2StateV + kJ2Threads - 1) / kJ2Threads;
+  const double2* c2 = reinterpret_cast<const double2*>(commit + kJ2Vals * first);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int j = t + k * kJ2Threads;
+    if (j < nsv) sst[j] = c2[j];
+  }
+  if (active) {
     const int64_t n = conn[idx];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1649,37 +1673,48 @@ __global__ __launch_bounds__(kJ2Threads) void k_j2_update(int32_t n_elems,
     }
   }
   __syncthreads();
-  if (idx >= npts) return;
-  const int e0 = t & ~7;  // the element's first slot
-  const int gp = t & 7;
-  double x[8][3];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) x[a][k] = sx[e0 + a][k];
-  double g[8][3], dj;
-  double deps[6] = {0, 0, 0, 0, 0, 0};
-  if (c3d8_dndx(x, gp, g, dj)) {
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const double d0 = sd[e0 + a][0], d1 = sd[e0 + a][1], d2 = sd[e0 + a][2];
-      deps[0] += g[a][0] * d0;
-      deps[1] += g[a][1] * d1;
-      deps[2] += g[a][2] * d2;
-      deps[3] += g[a][1] * d0 + g[a][0] * d1;
-      deps[4] += g[a][2] * d1 + g[a][1] * d2;
-      deps[5] += g[a][2] * d0 + g[a][0] * d2;
-    }
-  }
-  double st[13];
-#pragma unroll
-  for (int k = 0; k < 13; ++k) st[k] = commit[13 * idx + k];
+  double* const ss = reinterpret_cast<double*>(sst);
+  double st[kJ2Vals];
   double tc[kJ2Tangent];
-  flags[idx] = j2_point(pr, deps, st, tc);
+  if (active) {
+    const int e0 = t & ~7;  // the element's first slot
+    const int gp = t & 7;
+    double x[8][3];
 #pragma unroll
-  for (int k = 0; k < 13; ++k) trial[13 * idx + k] = st[k];
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-  for (int k = 0; k < kJ2Tangent; ++k) tangent[kJ2Tangent * idx + k] = tc[k];
+      for (int k = 0; k < 3; ++k) x[a][k] = sx[e0 + a][k];
+    double g[8][3], dj;
+    double deps[6] = {0, 0, 0, 0, 0, 0};
+    if (c3d8_dndx(x, gp, g, dj)) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const double d0 = sd[e0 + a][0], d1 = sd[e0 + a][1], d2 = sd[e0 + a][2];
+        deps[0] += g[a][0] * d0;
+        deps[1] += g[a][1] * d1;
+        deps[2] += g[a][2] * d2;
+        deps[3] += g[a][1] * d0 + g[a][0] * d1;
+        deps[4] += g[a][2] * d1 + g[a][1] * d2;
+        deps[5] += g[a][2] * d0 + g[a][0] * d2;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kJ2Vals; ++k) st[k] = ss[kJ2Vals * t + k];
+    flags[idx] = j2_point(pr, deps, st, tc);
+  }
+  __syncthreads();  // every committed record and node read before their space is reused
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < kJ2Vals; ++k) ss[kJ2Vals * t + k] = st[k];
+    double* const sn = reinterpret_cast<double*>(stn);
+#pragma unroll
+    for (int k = 0; k < kJ2Tangent; ++k) sn[kJ2Tangent * t + k] = tc[k];
+  }
+  __syncthreads();
+  double2* const t2 = reinterpret_cast<double2*>(trial + kJ2Vals * first);
+  double2* const g2 = reinterpret_cast<double2*>(tangent + kJ2Tangent * first);
+  for (int j = t; j < nsv; j += kJ2Threads) t2[j] = sst[j];
+  for (int j = t; j < ntv; j += kJ2Threads) g2[j] = stn[j];
 }
 
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }

@@ -95,6 +95,7 @@ constexpr int kMaxDiags = 13;       // upper block diagonals of the symmetric FP
 // doubles per Gauss point of the stored J2 tangent (mu', lam', c, n(6)):
 // Dt = lam' 1x1 + 2 mu' I_s - c n x n
 constexpr int kJ2Tangent = 9;
+constexpr int kJ2Vals = 13;  // J2 state doubles per Gauss point: stress (6), eqps, plastic strain (6)
 // nodes per tile of the symmetric view's block-diagonal layout, one per
 // thread of the SpMV's workgroup: kDiagTile, or kDiagTileWide on meshes of at
 // least kDiagWideMinNodes nodes (larger tiles keep more of the transposed
